@@ -1,0 +1,213 @@
+/*
+ * mapgraph_c.h -- C-ABI of libmapgraph_hip.so, the MI355X (gfx950) GAS graph engine.
+ *
+ * Plain pointers and sizes only; no C++ or HIP types cross this boundary, so the
+ * library can be bound from C, C++, Python (ctypes) or any FFI.  Every entry point
+ * returns an int status (MG_OK == 0); the text of the last error on the calling
+ * thread is available from mg_last_error().  No entry point calls exit() or lets an
+ * exception escape.
+ *
+ * Reference interfaces replaced (paths relative to the MapGraph 0.3.3 tree):
+ *   mg_graph_create      CsrProblem::FromHostProblem     GASengine/csr_problem.cuh:401-621
+ *                        + device CSC build              util.cuh:55-126
+ *   mg_graph_destroy     GraphSlice::~GraphSlice         GASengine/csr_problem.cuh:217-289
+ *   mg_bfs               EnactIterativeSearch<bfs>       GASengine/enactor_vertex_centric.cuh:2910-2966
+ *                        with Algorithms/BFS/bfs.h, then CsrProblem::ExtractResults
+ *                        (csr_problem.cuh:371-396) and bfs::extractResult (bfs.h:257-260)
+ *   mg_sssp              EnactIterativeSearch<sssp>      Algorithms/SSSP/sssp.h, sssp.cu:378-428
+ *   mg_cc                EnactIterativeSearch<cc>        Algorithms/CC/cc.h, cc.cu:300-347
+ *   mg_pagerank          EnactIterativeSearch<pagerank>  Algorithms/PageRank/PageRank.h, PageRank.cu:296-313
+ *   mg_stats.edges_traversed   TEPS count                Algorithms/BFS/bfs.cu:483-493
+ *   mg_stats.device_ms         "Kernel time took"        enactor_vertex_centric.cuh:2874-2883
+ *   mg_options fields          Config parameters         Algorithms/BFS/register.h:27-40,
+ *                                                        Algorithms/PageRank/register.h:27-41
+ *   mg_csr_from_edges    CsrGraph::FromCoo               b40c/graph/csr_graph.cuh:74-164
+ *   mg_mtx_*             ReadMarketStream                b40c/graph/builder/market.cuh:42-166
+ *   mg_pick_sources      random source rule              Algorithms/BFS/bfs.cu:340-361
+ */
+#ifndef MAPGRAPH_C_H_
+#define MAPGRAPH_C_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+#define MG_OK                 0
+#define MG_ERR_INVALID_ARG    1   /* bad pointer, size, vertex id or option            */
+#define MG_ERR_OOM            2   /* device or host allocation failed                   */
+#define MG_ERR_DEVICE         3   /* a HIP runtime call or kernel failed                */
+#define MG_ERR_OVERFLOW       4   /* a frontier queue overflowed and could not regrow   */
+#define MG_ERR_IO             5   /* file could not be opened / read                    */
+#define MG_ERR_FORMAT         6   /* malformed Matrix Market input                      */
+#define MG_ERR_UNSUPPORTED    7   /* the graph lacks what the call needs (e.g. weights) */
+#define MG_ERR_NO_DEVICE      8   /* no HIP device is visible                           */
+
+/* Copies the last error message of the calling thread (NUL-terminated, truncated to
+ * len) and returns its full length. */
+size_t mg_last_error(char *buf, size_t len);
+
+/* Library version as "major.minor.patch". */
+const char *mg_version(void);
+
+/* ---------------------------------------------------------------- device */
+/* Selects the HIP device used by graphs created afterwards on this thread
+ * (reference: cudaInit(device), Algorithms/BFS/bfs.cu:145-202). */
+int mg_device_init(int device);
+/* Number of visible HIP devices (0 when none; never an error). */
+int mg_device_count(void);
+/* Blocks until all work the library queued on the current device has finished. */
+int mg_device_synchronize(void);
+
+/* ---------------------------------------------------------------- graph store */
+/* flags for mg_graph_create */
+#define MG_GRAPH_WEIGHTED   0x1u  /* edge_values is given (int32 per CSR edge)          */
+#define MG_GRAPH_BUILD_CSC  0x2u  /* build the in-edge (CSC) view on the device         */
+#define MG_GRAPH_SYMMETRIC  0x4u  /* caller asserts the CSR is symmetric: in-edges of v
+                                     equal out-edges of v (lets pull steps read the CSR) */
+
+typedef struct mg_graph mg_graph;
+
+/* Copies a host CSR graph into device memory (HBM).  row_offsets has n+1 entries,
+ * col_idx (and edge_values when MG_GRAPH_WEIGHTED) m entries.  Offsets must be
+ * non-decreasing, start at 0, end at m, and every column index must lie in [0, n).
+ * The caller's buffers are not retained. */
+int mg_graph_create(const int32_t *row_offsets, const int32_t *col_idx,
+                    const int32_t *edge_values, int32_t n, int64_t m,
+                    uint32_t flags, mg_graph **out);
+int mg_graph_destroy(mg_graph *g);
+/* n, m and the flags a graph was created with. */
+int mg_graph_info(const mg_graph *g, int32_t *n, int64_t *m, uint32_t *flags);
+/* Device bytes held by the graph (topology + pooled per-search state). */
+int mg_graph_device_bytes(const mg_graph *g, int64_t *bytes);
+/* Copies the device CSC view back (for tests of the device CSC build).  col_offsets
+ * has n+1 entries, row_idx and csr_edge_id m entries; csr_edge_id[k] is the CSR edge
+ * id of CSC slot k (reference d_edgeCSC_indices, util.cuh:85-126). */
+int mg_graph_get_csc(const mg_graph *g, int32_t *col_offsets, int32_t *row_idx,
+                     int32_t *csr_edge_id);
+
+/* ---------------------------------------------------------------- options/stats */
+#define MG_ENGINE_AUTO   0   /* specialised kernels where the program allows it         */
+#define MG_ENGINE_GAS    1   /* the generic GAS enactor driving the Program functors    */
+
+#define MG_PR_GAS        0   /* reference PageRank program: frontier-activated Jacobi,
+                                a vertex stays active while its rank moves by >= 0.01   */
+#define MG_PR_ALL        1   /* all-vertex Jacobi for a fixed number of iterations,
+                                stopping early once the L1 change falls below tol       */
+
+typedef struct mg_options {
+    int32_t iter_num;       /* max iterations; reference "iter_num" (default INT32_MAX)    */
+    int32_t threshold;      /* reference "threshold": frontier size above which the
+                               two-phase / pull path is taken (default 10000)             */
+    int32_t directed;       /* reference "directed": 1 = out-edges only, 0 = out- and
+                               in-edges (needs MG_GRAPH_BUILD_CSC or MG_GRAPH_SYMMETRIC)   */
+    int32_t engine;         /* MG_ENGINE_AUTO or MG_ENGINE_GAS                             */
+    int32_t verbose;        /* reference "verbose": per-iteration frontier trace to stderr */
+    int32_t out_on_device;  /* 1: output pointers are device pointers                      */
+    int32_t pr_mode;        /* MG_PR_GAS or MG_PR_ALL                                      */
+    float   pr_teleport;    /* teleport term t (reference 0.15f); the damping is computed
+                               as (1.0f - t) in float, exactly as PageRank.h:269 writes it */
+    float   pr_tol;         /* MG_PR_ALL: stop when sum |r_new - r| < pr_tol * n           */
+    float   pr_active_eps;  /* MG_PR_GAS activation threshold (reference 0.01f)           */
+    double  max_queue_sizing; /* reference "max_queue_sizing" (queue = sizing * n)         */
+    int32_t alpha;          /* BFS top-down -> bottom-up switch (frontier edges > unexplored/alpha) */
+    int32_t beta;           /* BFS bottom-up -> top-down switch (frontier < n/beta)        */
+    int32_t reserved[8];
+} mg_options;
+
+/* Fills *o with the reference defaults. */
+void mg_options_default(mg_options *o);
+
+typedef struct mg_stats {
+    int64_t edges_traversed;  /* reference TEPS count: sum of out-degrees of the reached
+                                 vertices (Algorithms/BFS/bfs.cu:483-493)                  */
+    int64_t vertices_reached;
+    int64_t edges_examined;   /* edges the kernels actually read (diagnostic)              */
+    int32_t iterations;       /* reference "Total iteration"                               */
+    int32_t overflow_regrows; /* queue regrow-and-replay events                            */
+    double  device_ms;        /* hipEvent time around the search on the graph's stream     */
+    double  host_ms;          /* wall time of the call                                     */
+    int32_t bottom_up_levels; /* BFS levels run bottom-up                                  */
+    int32_t reserved[7];
+} mg_stats;
+
+/* ---------------------------------------------------------------- searches */
+/* BFS depths from src: depth[src] = 0, unreached = -1 (bfs.h:71-78, 241-253). */
+int mg_bfs(mg_graph *g, int32_t src, int32_t *depth_out,
+           const mg_options *opt, mg_stats *stats);
+
+/* SSSP distances from src over int32 edge weights (needs MG_GRAPH_WEIGHTED):
+ * dist[src] = 0, unreached = INT32_MAX (sssp.h:42-43). */
+int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out,
+            const mg_options *opt, mg_stats *stats);
+
+/* Connected components: label[v] = smallest vertex id of v's (weakly) connected
+ * component (cc.h:118-120, 199-213). */
+int mg_cc(mg_graph *g, int32_t *label_out, const mg_options *opt, mg_stats *stats);
+
+/* PageRank in the reference scale: r0 = t, r = t + (1.0f - t) * sum_in r[u]/outdeg[u]
+ * with t = pr_teleport (PageRank.h:108-116, 233-283); dangling mass is dropped.
+ * MG_PR_GAS runs the reference program (frontier-activated, pr_active_eps, until no
+ * vertex moves or iter_num); MG_PR_ALL updates every vertex for iter_num
+ * iterations or until sum |r_new - r| < pr_tol * n.  Needs MG_GRAPH_BUILD_CSC or
+ * MG_GRAPH_SYMMETRIC.  Sums run in a fixed order, so results are deterministic. */
+int mg_pagerank(mg_graph *g, float *rank_out, const mg_options *opt, mg_stats *stats);
+
+/* ---------------------------------------------------------------- host graph tools */
+/* These run on the host only and need no device. */
+
+/* R-MAT edge count ef * 2^scale. */
+int64_t mg_rmat_num_edges(int32_t scale, int32_t edge_factor);
+/* Seeded R-MAT edges: edge k picks, for each of scale levels, one quadrant with
+ * probabilities (a, b, c, 1-a-b-c) from a counter-based hash of (seed, k, level);
+ * no noise, no vertex permutation.  Writes mg_rmat_num_edges() pairs. */
+int mg_gen_rmat_edges(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                      uint64_t seed, int32_t *src, int32_t *dst);
+/* Undirected width x height 4-neighbour grid (vertex id = row * width + col) with
+ * exactly floor(delete_frac * E) edges removed by a seeded hash order.  Writes the
+ * kept undirected edges (u < v) and their count; the buffers need room for
+ * mg_grid_num_edges(). */
+int64_t mg_grid_num_edges(int32_t width, int32_t height);
+int mg_gen_grid_edges(int32_t width, int32_t height, double delete_frac, uint64_t seed,
+                      int32_t *src, int32_t *dst, int64_t *m_out);
+
+/* flags for mg_csr_from_edges */
+#define MG_CSR_SYMMETRIZE   0x1u  /* add (v,u) for every (u,v)                      */
+#define MG_CSR_DROP_LOOPS   0x2u  /* drop (u,u)                                     */
+#define MG_CSR_DEDUPE       0x4u  /* keep one copy of repeated (u,v)                */
+#define MG_CSR_SORT_COLS    0x8u  /* sort each row's columns ascending; without it
+                                     rows keep input order (stable sort by row, as
+                                     CsrGraph::FromCoo, csr_graph.cuh:74-122)      */
+/* Builds a CSR from an edge list.  row_offsets needs n+1 entries; col_idx needs
+ * capacity entries (2*m_in suffices with MG_CSR_SYMMETRIZE, m_in otherwise).
+ * When vals/vals_out are non-NULL, each edge's value travels with it (the
+ * reverse edge of a symmetrized pair gets the same value). */
+int mg_csr_from_edges(int32_t n, int64_t m_in, const int32_t *src, const int32_t *dst,
+                      const int32_t *vals, uint32_t flags, int32_t *row_offsets,
+                      int32_t *col_idx, int32_t *vals_out, int64_t capacity,
+                      int64_t *m_out);
+/* Symmetric hashed edge weights: w = 1 + hash(seed, min(u,v), max(u,v)) mod wmax. */
+int mg_edge_weights_hash(int32_t n, const int32_t *row_offsets, const int32_t *col_idx,
+                         int64_t m, uint64_t seed, int32_t wmax, int32_t *w_out);
+/* Host CSC (in-edge) view by a stable counting sort on the column. */
+int mg_csc_from_csr(int32_t n, int64_t m, const int32_t *row_offsets,
+                    const int32_t *col_idx, int32_t *col_offsets, int32_t *row_idx,
+                    int32_t *csr_edge_id);
+/* k seeded uniform draws (with replacement) among vertices of out-degree > 0. */
+int mg_pick_sources(int32_t n, const int32_t *row_offsets, int32_t k, uint64_t seed,
+                    int32_t *out);
+/* Matrix Market: header pass, then the (0-based) entries.  Values default to 1
+ * when the file has no value column; values are truncated to int32 as the
+ * reference's (Value) cast does (market.cuh:116-121). */
+int mg_mtx_read_header(const char *path, int32_t *n, int64_t *nnz, int32_t *symmetric);
+int mg_mtx_read(const char *path, int32_t *src, int32_t *dst, int32_t *val,
+                int64_t capacity, int64_t *nnz_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MAPGRAPH_C_H_ */

@@ -1,0 +1,154 @@
+// Engine-internal declarations: the device graph store (mg_graph), the pooled
+// per-search workspace and the control block the kernels share.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <mutex>
+
+#include "mg_internal.h"
+
+#define MG_HIP(call)                                                                              \
+    do {                                                                                          \
+        hipError_t _e = (call);                                                                   \
+        if (_e != hipSuccess)                                                                     \
+            return mg::set_error(MG_ERR_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), \
+                                 __FILE__, __LINE__);                                             \
+    } while (0)
+
+namespace mg {
+
+constexpr int BLK = 256;          // threads per workgroup of the multi-workgroup kernels
+constexpr int SMALL_BLK = 1024;   // threads of the single-workgroup small-frontier kernel
+constexpr int CHUNK_T = 256;      // frontier vertices above this degree are split ...
+constexpr int CHUNK = 1024;       // ... into edge chunks of this size
+constexpr int SMALL_Q = 4096;     // small-frontier kernel: max frontier vertices
+constexpr int SMALL_E = 65536;    // small-frontier kernel: max frontier edges
+constexpr int BU_INLINE = 32;     // bottom-up: edges a lane scans before handing the vertex to a wave
+
+enum { DIR_TD = 0, DIR_BU = 1 };
+
+struct ChunkRec {
+    int v, beg, end, pad;
+};
+
+// Control block shared by the kernels of one search (device memory).  Levels are
+// advanced on the device; the host only reads `done` every few queued levels.
+struct Ctl {
+    int level;          // level being processed
+    int done;           // 1 once the search has converged or hit iter_limit
+    int dir;            // DIR_TD or DIR_BU for the current level
+    int blocks_done;    // last-workgroup-done counter of the closing kernel
+    int qsize[3];       // sizes of the three rotating frontier queues
+    int nchunks;        // edge chunks queued this level
+    int nhard;          // bottom-up vertices handed to whole waves this level
+    int iter_limit;     // stop after this many levels
+    int n;
+    int can_bu;         // bottom-up allowed (an in-edge view exists)
+    int alpha, beta;    // direction heuristic parameters
+    int nf_prev;        // frontier size of the previous level
+    int bu_levels;
+    int small_levels;
+    int changed;        // generic "something changed" counter
+    int pad0;
+    long long mf_next;        // sum of out-degrees of vertices found this level
+    long long m_unexplored;   // out-edges of unreached vertices
+    long long edges_traversed;// reference TEPS count
+    long long reached;
+    long long edges_examined;
+    double dsum;              // scratch accumulator (PageRank L1 change)
+    long long mf_cur;         // out-edges of the current frontier
+    long long pad1[3];
+};
+
+// Pooled per-search device state, grown on demand and reused across searches.
+struct Workspace {
+    int n = 0;
+    int nwords = 0;           // 32-bit words of one vertex bitmap (rounded to even)
+    int64_t chunk_cap = 0;
+    int32_t *vals = nullptr;  // depth / dist / label
+    int32_t *q[3] = {nullptr, nullptr, nullptr};
+    int32_t *hard = nullptr;
+    uint32_t *visited = nullptr;
+    uint32_t *bm[3] = {nullptr, nullptr, nullptr};
+    ChunkRec *chunks = nullptr;
+    Ctl *ctl = nullptr;
+    Ctl *h_ctl = nullptr;     // pinned host mirror
+    // PageRank state (allocated on first use)
+    float *rank = nullptr;
+    float *contrib[2] = {nullptr, nullptr};
+    double *partials = nullptr;                       // per-workgroup L1 partial sums
+    int32_t *pr_mid = nullptr;                        // vertices of medium in-degree (a wave each)
+    ChunkRec *pr_chunks = nullptr;                    // in-edge chunks of high in-degree vertices
+    int32_t *pr_bigv = nullptr;                       // first chunk of each high in-degree vertex (+ end)
+    float *pr_part = nullptr;                         // per-chunk partial sums
+    int32_t *pr_pact = nullptr;                       // per-chunk "an in-neighbour changed" flags
+    int pr_nmid = 0, pr_nchunks = 0, pr_nbig = 0;
+    const int32_t *pr_view = nullptr;                 // in-edge offsets the buckets were built for
+};
+
+}  // namespace mg
+
+struct mg_graph {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int32_t n = 0;
+    int64_t m = 0;
+    uint32_t flags = 0;
+    int32_t *row_off = nullptr, *col = nullptr, *w = nullptr;
+    int32_t *col_off = nullptr, *row_idx = nullptr, *csc_eid = nullptr;
+    int32_t min_weight = 0;
+    int64_t topo_bytes = 0;
+    std::mutex mu;
+    mg::Workspace ws;
+};
+
+namespace mg {
+
+// Grows the workspace for graph g; returns an MG_* status.
+int ensure_workspace(mg_graph *g);
+int ensure_pr_workspace(mg_graph *g);
+int alloc_i32(int32_t **p, size_t n);
+int alloc_f32(float **p, size_t n);
+int alloc_chunks(ChunkRec **p, size_t n);
+void free_dev(void *p);
+int grid_blocks(int64_t work_items, int block);
+
+// Per-call bookkeeping shared by the search entry points.
+struct CallTimer {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+// Validates the common search arguments and locks the graph.
+int begin_search(mg_graph *g, const mg_options *opt, mg_options *resolved);
+// Copies a device int32/float result to the caller (host or device pointer).
+int deliver(mg_graph *g, void *dst, const void *src, size_t bytes, bool out_on_device);
+// Polls ctl->done on the host every `k` queued steps; `step` enqueues one step.
+template <class Step>
+int drive(mg_graph *g, Step step, int64_t max_steps);
+
+}  // namespace mg
+
+template <class Step>
+int mg::drive(mg_graph *g, Step step, int64_t max_steps) {
+    Workspace &ws = g->ws;
+    int64_t queued = 0, k = 2;
+    while (queued < max_steps) {
+        for (int i = 0; i < k && queued < max_steps; ++i, ++queued) {
+            int rc = step();
+            if (rc) return rc;
+        }
+        MG_HIP(hipMemcpyAsync(ws.h_ctl, ws.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
+        MG_HIP(hipStreamSynchronize(g->stream));
+        if (ws.h_ctl->done) return MG_OK;
+        if (k < 64) k *= 2;
+    }
+    return MG_OK;
+}

@@ -1,0 +1,293 @@
+// Device graph store and the shared C-ABI plumbing of libmapgraph_hip.so.
+//
+// mg_graph_create plays the part of CsrProblem::FromHostProblem
+// (GASengine/csr_problem.cuh:401-621): it copies the host CSR (and, on request,
+// builds the in-edge CSC view with the CSR edge id of every CSC slot, as
+// util.cuh:55-126 does) into HBM.  Unlike the reference it allocates no O(m)
+// frontier queues: the engine's frontiers are vertex queues and bitmaps of O(n)
+// (the reference sizes three key/value queue pairs at max_queue_sizing * m,
+// csr_problem.cuh:627-827), pooled per graph and reused across searches.
+#include <climits>
+#include <cstring>
+#include <vector>
+
+#include "engine.h"
+
+namespace mg {
+
+static thread_local int t_device = 0;
+
+int grid_blocks(int64_t work_items, int block) {
+    int64_t b = (work_items + block - 1) / block;
+    if (b < 1) b = 1;
+    if (b > 2048) b = 2048;  // 256 CUs x 8 workgroups; kernels stride beyond this
+    return (int)b;
+}
+
+template <class T>
+static int dalloc(T **p, size_t count) {
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_error(MG_ERR_OOM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+    }
+    return MG_OK;
+}
+
+template <class T>
+static void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+static void free_workspace(Workspace &ws) {
+    dfree(ws.vals);
+    for (auto &p : ws.q) dfree(p);
+    dfree(ws.hard);
+    dfree(ws.visited);
+    for (auto &p : ws.bm) dfree(p);
+    dfree(ws.chunks);
+    dfree(ws.ctl);
+    if (ws.h_ctl) (void)hipHostFree(ws.h_ctl);
+    dfree(ws.rank);
+    dfree(ws.contrib[0]);
+    dfree(ws.contrib[1]);
+    dfree(ws.partials);
+    dfree(ws.pr_mid);
+    dfree(ws.pr_chunks);
+    dfree(ws.pr_part);
+    dfree(ws.pr_pact);
+    dfree(ws.pr_bigv);
+    ws = Workspace();
+}
+
+int ensure_workspace(mg_graph *g) {
+    Workspace &ws = g->ws;
+    if (ws.ctl) return MG_OK;
+    const int n = g->n;
+    ws.n = n;
+    ws.nwords = ((n + 63) / 64) * 2;
+    ws.chunk_cap = 2 * (g->m / CHUNK + g->m / CHUNK_T) + 64;  // both top-down views
+    int rc = 0;
+    if ((rc = dalloc(&ws.vals, n)) || (rc = dalloc(&ws.q[0], n)) || (rc = dalloc(&ws.q[1], n)) ||
+        (rc = dalloc(&ws.q[2], n)) || (rc = dalloc(&ws.hard, n)) || (rc = dalloc(&ws.visited, ws.nwords)) ||
+        (rc = dalloc(&ws.bm[0], ws.nwords)) || (rc = dalloc(&ws.bm[1], ws.nwords)) ||
+        (rc = dalloc(&ws.bm[2], ws.nwords)) || (rc = dalloc(&ws.chunks, ws.chunk_cap)) ||
+        (rc = dalloc(&ws.ctl, 1))) {
+        free_workspace(ws);
+        return rc;
+    }
+    hipError_t e = hipHostMalloc((void **)&ws.h_ctl, sizeof(Ctl), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        free_workspace(ws);
+        return set_error(MG_ERR_OOM, "hipHostMalloc failed: %s", hipGetErrorString(e));
+    }
+    return MG_OK;
+}
+
+int ensure_pr_workspace(mg_graph *g) {
+    Workspace &ws = g->ws;
+    if (ws.rank) return MG_OK;
+    int rc = 0;
+    const int n = g->n;
+    if ((rc = dalloc(&ws.rank, n)) || (rc = dalloc(&ws.contrib[0], n)) || (rc = dalloc(&ws.contrib[1], n)) ||
+        (rc = dalloc(&ws.partials, 8192)))
+        return rc;
+    return MG_OK;
+}
+
+int alloc_i32(int32_t **p, size_t n) { return dalloc(p, n); }
+int alloc_f32(float **p, size_t n) { return dalloc(p, n); }
+int alloc_chunks(ChunkRec **p, size_t n) { return dalloc(p, n); }
+void free_dev(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+int begin_search(mg_graph *g, const mg_options *opt, mg_options *resolved) {
+    clear_error();
+    if (!g) return set_error(MG_ERR_INVALID_ARG, "null graph");
+    if (opt) *resolved = *opt;
+    else mg_options_default(resolved);
+    if (resolved->iter_num < 0) return set_error(MG_ERR_INVALID_ARG, "iter_num must be >= 0");
+    if (resolved->directed != 0 && resolved->directed != 1)
+        return set_error(MG_ERR_INVALID_ARG, "directed must be 0 or 1");
+    if (resolved->directed == 0 && !(g->flags & (MG_GRAPH_BUILD_CSC | MG_GRAPH_SYMMETRIC)))
+        return set_error(MG_ERR_UNSUPPORTED, "directed=0 needs a graph created with MG_GRAPH_BUILD_CSC or MG_GRAPH_SYMMETRIC");
+    MG_HIP(hipSetDevice(g->device));
+    return ensure_workspace(g);
+}
+
+int deliver(mg_graph *g, void *dst, const void *src, size_t bytes, bool out_on_device) {
+    if (!dst) return MG_OK;
+    MG_HIP(hipMemcpyAsync(dst, src, bytes, out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                          g->stream));
+    MG_HIP(hipStreamSynchronize(g->stream));
+    return MG_OK;
+}
+
+}  // namespace mg
+
+using namespace mg;
+
+extern "C" {
+
+void mg_options_default(mg_options *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->iter_num = INT_MAX;
+    o->threshold = 10000;
+    o->directed = 1;
+    o->engine = MG_ENGINE_AUTO;
+    o->pr_mode = MG_PR_GAS;
+    o->pr_teleport = 0.15f;
+    o->pr_tol = 0.0f;
+    o->pr_active_eps = 0.01f;
+    o->max_queue_sizing = 1.5;
+    o->alpha = 14;
+    o->beta = 24;
+}
+
+int mg_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int mg_device_init(int device) {
+    clear_error();
+    int c = mg_device_count();
+    if (c == 0) return set_error(MG_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= c) return set_error(MG_ERR_INVALID_ARG, "device %d outside [0,%d)", device, c);
+    MG_HIP(hipSetDevice(device));
+    t_device = device;
+    return MG_OK;
+}
+
+int mg_device_synchronize(void) {
+    clear_error();
+    MG_HIP(hipDeviceSynchronize());
+    return MG_OK;
+}
+
+int mg_graph_create(const int32_t *row_offsets, const int32_t *col_idx, const int32_t *edge_values, int32_t n,
+                    int64_t m, uint32_t flags, mg_graph **out) {
+    clear_error();
+    if (!out) return set_error(MG_ERR_INVALID_ARG, "mg_graph_create: out is null");
+    *out = nullptr;
+    if (n < 0 || m < 0 || m > INT32_MAX || !row_offsets || (m > 0 && !col_idx))
+        return set_error(MG_ERR_INVALID_ARG, "mg_graph_create: bad sizes or null arrays (n=%d, m=%lld)", n,
+                         (long long)m);
+    if ((flags & MG_GRAPH_WEIGHTED) && m > 0 && !edge_values)
+        return set_error(MG_ERR_INVALID_ARG, "mg_graph_create: MG_GRAPH_WEIGHTED without edge_values");
+    if (flags & ~(MG_GRAPH_WEIGHTED | MG_GRAPH_BUILD_CSC | MG_GRAPH_SYMMETRIC))
+        return set_error(MG_ERR_INVALID_ARG, "mg_graph_create: unknown flags 0x%x", flags);
+    if (row_offsets[0] != 0 || row_offsets[n] != m)
+        return set_error(MG_ERR_INVALID_ARG, "mg_graph_create: row_offsets must start at 0 and end at m");
+    for (int32_t v = 0; v < n; ++v)
+        if (row_offsets[v + 1] < row_offsets[v])
+            return set_error(MG_ERR_INVALID_ARG, "mg_graph_create: row_offsets decrease at %d", v);
+    for (int64_t e = 0; e < m; ++e)
+        if ((uint32_t)col_idx[e] >= (uint32_t)n)
+            return set_error(MG_ERR_INVALID_ARG, "mg_graph_create: col_idx[%lld]=%d outside [0,%d)", (long long)e,
+                             col_idx[e], n);
+    int ndev = mg_device_count();
+    if (ndev == 0) return set_error(MG_ERR_NO_DEVICE, "no HIP device visible");
+
+    mg_graph *g = new (std::nothrow) mg_graph();
+    if (!g) return set_error(MG_ERR_OOM, "host allocation failed");
+    g->device = t_device;
+    g->n = n;
+    g->m = m;
+    g->flags = flags;
+    int rc = MG_OK;
+    auto fail = [&](int code) {
+        mg_graph_destroy(g);
+        return code;
+    };
+    if (hipSetDevice(g->device) != hipSuccess) return fail(set_error(MG_ERR_DEVICE, "hipSetDevice failed"));
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess)
+        return fail(set_error(MG_ERR_DEVICE, "stream/event creation failed"));
+    if ((rc = dalloc(&g->row_off, (size_t)n + 1)) || (rc = dalloc(&g->col, (size_t)m))) return fail(rc);
+    g->topo_bytes = 4 * ((int64_t)n + 1 + m);
+    if (hipMemcpy(g->row_off, row_offsets, 4 * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        (m && hipMemcpy(g->col, col_idx, 4 * (size_t)m, hipMemcpyHostToDevice) != hipSuccess))
+        return fail(set_error(MG_ERR_DEVICE, "host-to-device copy of the CSR failed"));
+    if (flags & MG_GRAPH_WEIGHTED) {
+        if ((rc = dalloc(&g->w, (size_t)m))) return fail(rc);
+        if (m && hipMemcpy(g->w, edge_values, 4 * (size_t)m, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_error(MG_ERR_DEVICE, "host-to-device copy of the edge values failed"));
+        int32_t mn = INT32_MAX;
+        for (int64_t e = 0; e < m; ++e) mn = edge_values[e] < mn ? edge_values[e] : mn;
+        g->min_weight = m ? mn : 0;
+        g->topo_bytes += 4 * m;
+    }
+    if (flags & MG_GRAPH_BUILD_CSC) {
+        std::vector<int32_t> co((size_t)n + 1), ri((size_t)m), ce((size_t)m);
+        if ((rc = mg_csc_from_csr(n, m, row_offsets, col_idx, co.data(), ri.data(), ce.data()))) return fail(rc);
+        if ((rc = dalloc(&g->col_off, (size_t)n + 1)) || (rc = dalloc(&g->row_idx, (size_t)m)) ||
+            (rc = dalloc(&g->csc_eid, (size_t)m)))
+            return fail(rc);
+        if (hipMemcpy(g->col_off, co.data(), 4 * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+            (m && hipMemcpy(g->row_idx, ri.data(), 4 * (size_t)m, hipMemcpyHostToDevice) != hipSuccess) ||
+            (m && hipMemcpy(g->csc_eid, ce.data(), 4 * (size_t)m, hipMemcpyHostToDevice) != hipSuccess))
+            return fail(set_error(MG_ERR_DEVICE, "host-to-device copy of the CSC failed"));
+        g->topo_bytes += 4 * ((int64_t)n + 1 + 2 * m);
+    }
+    if ((rc = ensure_workspace(g))) return fail(rc);
+    *out = g;
+    return MG_OK;
+}
+
+int mg_graph_destroy(mg_graph *g) {
+    clear_error();
+    if (!g) return MG_OK;
+    (void)hipSetDevice(g->device);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    free_workspace(g->ws);
+    dfree(g->row_off);
+    dfree(g->col);
+    dfree(g->w);
+    dfree(g->col_off);
+    dfree(g->row_idx);
+    dfree(g->csc_eid);
+    if (g->ev0) (void)hipEventDestroy(g->ev0);
+    if (g->ev1) (void)hipEventDestroy(g->ev1);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return MG_OK;
+}
+
+int mg_graph_info(const mg_graph *g, int32_t *n, int64_t *m, uint32_t *flags) {
+    clear_error();
+    if (!g) return set_error(MG_ERR_INVALID_ARG, "null graph");
+    if (n) *n = g->n;
+    if (m) *m = g->m;
+    if (flags) *flags = g->flags;
+    return MG_OK;
+}
+
+int mg_graph_device_bytes(const mg_graph *g, int64_t *bytes) {
+    clear_error();
+    if (!g || !bytes) return set_error(MG_ERR_INVALID_ARG, "null argument");
+    const Workspace &ws = g->ws;
+    int64_t b = g->topo_bytes;
+    if (ws.ctl) b += 4LL * ws.n * 5 + 4LL * ws.nwords * 4 + (int64_t)sizeof(ChunkRec) * ws.chunk_cap;
+    if (ws.rank) b += 12LL * ws.n;
+    *bytes = b;
+    return MG_OK;
+}
+
+int mg_graph_get_csc(const mg_graph *g, int32_t *col_offsets, int32_t *row_idx, int32_t *csr_edge_id) {
+    clear_error();
+    if (!g) return set_error(MG_ERR_INVALID_ARG, "null graph");
+    if (!g->col_off) return set_error(MG_ERR_UNSUPPORTED, "graph has no CSC view (MG_GRAPH_BUILD_CSC)");
+    MG_HIP(hipSetDevice(g->device));
+    if (col_offsets) MG_HIP(hipMemcpy(col_offsets, g->col_off, 4 * ((size_t)g->n + 1), hipMemcpyDeviceToHost));
+    if (row_idx && g->m) MG_HIP(hipMemcpy(row_idx, g->row_idx, 4 * (size_t)g->m, hipMemcpyDeviceToHost));
+    if (csr_edge_id && g->m) MG_HIP(hipMemcpy(csr_edge_id, g->csc_eid, 4 * (size_t)g->m, hipMemcpyDeviceToHost));
+    return MG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+// Shared host/device helpers of the frontier searches.
+#include "frontier.h"
+
+namespace mg {
+
+__global__ void fill_i32(int *p, int v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void fill_u32(uint32_t *p, uint32_t v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void iota_i32(int *p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = (int)i;
+}
+
+int launch_fill(mg_graph *g, int *p, int v, int64_t n) {
+    hipLaunchKernelGGL(fill_i32, dim3(grid_blocks(n, BLK)), dim3(BLK), 0, g->stream, p, v, n);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+int launch_fill(mg_graph *g, uint32_t *p, uint32_t v, int64_t n) {
+    hipLaunchKernelGGL(fill_u32, dim3(grid_blocks(n, BLK)), dim3(BLK), 0, g->stream, p, v, n);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+FrontierArgs make_frontier_args(mg_graph *g, int directed) {
+    Workspace &ws = g->ws;
+    FrontierArgs fa{};
+    fa.ctl = ws.ctl;
+    fa.off[0] = g->row_off;
+    fa.adj[0] = g->col;
+    fa.eid[0] = nullptr;
+    fa.nviews = 1;
+    if (directed == 0 && !(g->flags & MG_GRAPH_SYMMETRIC) && g->col_off) {
+        fa.off[1] = g->col_off;
+        fa.adj[1] = g->row_idx;
+        fa.eid[1] = g->csc_eid;
+        fa.nviews = 2;
+    } else {
+        fa.off[1] = fa.off[0];
+        fa.adj[1] = fa.adj[0];
+        fa.eid[1] = nullptr;
+    }
+    for (int i = 0; i < 3; ++i) {
+        fa.bm[i] = ws.bm[i];
+        fa.q[i] = ws.q[i];
+    }
+    fa.chunks = ws.chunks;
+    fa.chunk_cap = ws.chunk_cap;
+    fa.n = g->n;
+    fa.nwords = ws.nwords;
+    return fa;
+}
+
+// Reference TEPS count (bfs.cu:483-493) and reached-vertex count over a result
+// array: sum of CSR out-degrees of vertices whose value differs from `unreached`.
+__global__ void reached_stats(const int *vals, int unreached, const int *row_off, int n, unsigned long long *out) {
+    unsigned long long te = 0, nr = 0;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        if (vals[v] != unreached) {
+            te += row_off[v + 1] - row_off[v];
+            nr++;
+        }
+    te = (unsigned long long)wave_sum_ll((long long)te);
+    nr = (unsigned long long)wave_sum_ll((long long)nr);
+    if (lane_id() == 0) {
+        if (te) atomicAdd(&out[0], te);
+        if (nr) atomicAdd(&out[1], nr);
+    }
+}
+
+int launch_reached_stats(mg_graph *g, const int *vals, int unreached, mg_stats *st) {
+    if (!st) return MG_OK;
+    unsigned long long *d = (unsigned long long *)g->ws.chunks;  // 16 bytes of scratch
+    MG_HIP(hipMemsetAsync(d, 0, 16, g->stream));
+    hipLaunchKernelGGL(reached_stats, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, g->stream, vals, unreached,
+                       g->row_off, g->n, d);
+    MG_HIP(hipGetLastError());
+    unsigned long long h[2];
+    MG_HIP(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, g->stream));
+    MG_HIP(hipStreamSynchronize(g->stream));
+    st->edges_traversed = (int64_t)h[0];
+    st->vertices_reached = (int64_t)h[1];
+    return MG_OK;
+}
+
+// Records the closing event, fills *stats from the control block and copies
+// `bytes` of the device result `result` to `out`.  `vals` (nullable) is the int
+// result the TEPS count reads.
+int finish_search(mg_graph *g, mg_stats *stats, const int *vals, int unreached, void *out, const void *result,
+                  size_t bytes, bool out_on_device, const CallTimer &timer) {
+    Workspace &ws = g->ws;
+    MG_HIP(hipEventRecord(g->ev1, g->stream));
+    MG_HIP(hipEventSynchronize(g->ev1));
+    float ms = 0;
+    MG_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+    MG_HIP(hipMemcpy(ws.h_ctl, ws.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    if (ws.h_ctl->changed & (1 << 30))
+        return set_error(MG_ERR_OVERFLOW, "edge-chunk queue overflow (capacity %lld)", (long long)ws.chunk_cap);
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->iterations = ws.h_ctl->level;
+        stats->bottom_up_levels = ws.h_ctl->bu_levels;
+        stats->edges_examined = ws.h_ctl->edges_examined;
+        stats->device_ms = ms;
+        if (vals) {
+            int rc = launch_reached_stats(g, vals, unreached, stats);
+            if (rc) return rc;
+        }
+    }
+    int rc = deliver(g, out, result, bytes, out_on_device);
+    if (rc) return rc;
+    if (stats) stats->host_ms = timer.ms();
+    return MG_OK;
+}
+
+}  // namespace mg

@@ -1,0 +1,233 @@
+"""GPU parity: every search runs on the device through the C-ABI and is compared with
+the CPU oracle on the same seeded inputs.  BFS depths, SSSP distances and CC labels
+must be bit-exact; PageRank is compared in normalised units p = r / n with a
+max-abs tolerance of 1e-6 (the FP32 bar), and its error against the float64
+oracle is checked too."""
+import numpy as np
+import pytest
+
+from conftest import load_mtx, random_graph
+
+pytestmark = pytest.mark.gpu
+INT_MAX = 2**31 - 1
+PR_TOL = 1e-6  # max-abs in normalised units
+
+
+def _graph(mg, g, **kw):
+    return mg.Graph(g, **kw)
+
+
+def _check_bfs(mg, oracle, g, src, directed=1, **gkw):
+    with _graph(mg, g, **gkw) as dg:
+        got, st = dg.bfs(src, mg.options(directed=directed), stats=True)
+    csc = g.csc() if directed == 0 else None
+    exp = oracle.bfs(g.row_offsets, g.col_idx, src, directed=directed, csc=csc)
+    assert np.array_equal(got, exp), f"BFS mismatch at {np.nonzero(got != exp)[0][:10]}"
+    assert st.edges_traversed == oracle.teps_edges(g.row_offsets, exp)
+    assert st.vertices_reached == int((exp >= 0).sum())
+    return st
+
+
+# ---------------------------------------------------------------- fixtures
+@pytest.mark.parametrize("name", ["small.mtx", "trivialGraph.mtx", "chesapeake.mtx"])
+def test_fixtures_bfs_sssp_cc(mg, oracle, device, name):
+    g = load_mtx(mg, name)
+    for directed in (1, 0):
+        with mg.Graph(g, build_csc=True) as dg:
+            for src in range(min(g.n, 5)):
+                d = dg.bfs(src, mg.options(directed=directed))
+                assert np.array_equal(d, oracle.bfs(g.row_offsets, g.col_idx, src, directed=directed))
+                s = dg.sssp(src, mg.options(directed=directed))
+                assert np.array_equal(s, oracle.sssp(g.row_offsets, g.col_idx, g.values, src, directed=directed))
+            lab = dg.cc()
+            assert np.array_equal(lab, oracle.cc(g.row_offsets, g.col_idx))
+
+
+def test_small_known_answers_on_gpu(mg, device):
+    g = load_mtx(mg, "small.mtx")
+    with mg.Graph(g) as dg:
+        assert dg.bfs(0).tolist() == [0, 1, 1, 1, 2, 2, 2]
+        assert dg.sssp(0).tolist() == [0, 1, 4, 1, 2, 2, 2]
+        assert dg.cc().tolist() == [0] * 7
+
+
+# ---------------------------------------------------------------- random graphs
+@pytest.mark.parametrize("seed", [1, 2])
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_bfs_random(mg, oracle, device, seed, symmetric):
+    g = random_graph(mg, 5000, 40000, seed, symmetric=symmetric)
+    src = int(np.argmax(g.degrees()))
+    _check_bfs(mg, oracle, g, src, symmetric=symmetric)
+    if not symmetric:  # directed search with the CSC view enables bottom-up
+        _check_bfs(mg, oracle, g, src, build_csc=True)
+        _check_bfs(mg, oracle, g, src, directed=0, build_csc=True)
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_sssp_random(mg, oracle, device, seed, symmetric):
+    g = random_graph(mg, 4000, 30000, seed, symmetric=symmetric, wmax=64)
+    src = int(np.argmax(g.degrees()))
+    with mg.Graph(g, build_csc=True) as dg:
+        got = dg.sssp(src)
+        assert np.array_equal(got, oracle.sssp(g.row_offsets, g.col_idx, g.values, src))
+        got0 = dg.sssp(src, mg.options(directed=0))
+        assert np.array_equal(got0, oracle.sssp(g.row_offsets, g.col_idx, g.values, src, directed=0))
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_cc_random(mg, oracle, device, symmetric):
+    g = random_graph(mg, 20000, 15000, 5, symmetric=symmetric)
+    with mg.Graph(g, symmetric=symmetric) as dg:
+        assert np.array_equal(dg.cc(), oracle.cc(g.row_offsets, g.col_idx))
+
+
+# ---------------------------------------------------------------- R-MAT / grid
+@pytest.fixture(scope="module")
+def s16(mg):
+    return mg.rmat_graph(16, 16, seed=42, weight_max=64)
+
+
+def test_rmat16_bfs_from_0(mg, oracle, device, s16):
+    st = _check_bfs(mg, oracle, s16, 0, symmetric=True)
+    assert st.bottom_up_levels > 0  # the direction switch is exercised
+    # top-down only (no in-edge view) gives the same depths
+    _check_bfs(mg, oracle, s16, 0)
+
+
+def test_rmat16_bfs_many_sources(mg, oracle, device, s16):
+    srcs = mg.pick_sources(s16, 8, 7).tolist() + [int(np.argmin(s16.degrees()))]  # incl. an isolated vertex
+    with mg.Graph(s16, symmetric=True) as dg:
+        for s in srcs:
+            assert np.array_equal(dg.bfs(s), oracle.bfs(s16.row_offsets, s16.col_idx, s))
+
+
+def test_rmat16_sssp_cc(mg, oracle, device, s16):
+    with mg.Graph(s16, symmetric=True) as dg:
+        d, st = dg.sssp(0, stats=True)
+        exp = oracle.sssp(s16.row_offsets, s16.col_idx, s16.values, 0)
+        assert np.array_equal(d, exp)
+        assert st.edges_traversed == oracle.teps_edges(s16.row_offsets, exp, INT_MAX)
+        assert np.array_equal(dg.cc(), oracle.cc(s16.row_offsets, s16.col_idx))
+
+
+def test_grid_bfs_sssp(mg, oracle, device):
+    g = mg.grid_graph(256, seed=3, weight_max=16)
+    with mg.Graph(g, symmetric=True) as dg:
+        d, st = dg.bfs(0, stats=True)
+        exp = oracle.bfs(g.row_offsets, g.col_idx, 0)
+        assert np.array_equal(d, exp)
+        assert st.iterations >= exp.max()
+        s = dg.sssp(0)
+        assert np.array_equal(s, oracle.sssp(g.row_offsets, g.col_idx, g.values, 0))
+        assert np.array_equal(dg.cc(), oracle.cc(g.row_offsets, g.col_idx))
+
+
+# ---------------------------------------------------------------- limits and errors
+def test_iter_num_limits_levels(mg, oracle, device, s16):
+    with mg.Graph(s16, symmetric=True) as dg:
+        full = oracle.bfs(s16.row_offsets, s16.col_idx, 0)
+        for k in (1, 2):
+            got = dg.bfs(0, mg.options(iter_num=k))
+            exp = np.where(full <= k, full, -1)
+            assert np.array_equal(got, exp)
+        assert dg.bfs(0, mg.options(iter_num=0)).tolist() == [0] + [-1] * (s16.n - 1)
+
+
+def test_tiny_and_empty_graphs(mg, device):
+    g1 = mg.HostCSR(np.array([0, 0], np.int32), np.array([], np.int32), np.array([], np.int32))
+    with mg.Graph(g1, weighted=True) as dg:
+        assert dg.bfs(0).tolist() == [0]
+        assert dg.sssp(0).tolist() == [0]
+        assert dg.cc().tolist() == [0]
+    g2 = mg.HostCSR(np.zeros(6, np.int32), np.array([], np.int32))
+    with mg.Graph(g2, symmetric=True) as dg:
+        assert dg.bfs(3).tolist() == [-1, -1, -1, 0, -1]
+        assert dg.cc().tolist() == [0, 1, 2, 3, 4]
+
+
+def test_errors_are_status_codes(mg, device, s16):
+    unweighted = mg.HostCSR(s16.row_offsets, s16.col_idx)
+    with mg.Graph(unweighted) as dg:
+        with pytest.raises(mg.MgError) as e:
+            dg.bfs(s16.n)
+        assert e.value.code == mg.MG_ERR_INVALID_ARG
+        with pytest.raises(mg.MgError) as e:
+            dg.sssp(0)
+        assert e.value.code == mg.MG_ERR_UNSUPPORTED
+        with pytest.raises(mg.MgError) as e:
+            dg.bfs(0, mg.options(directed=0))
+        assert e.value.code == mg.MG_ERR_UNSUPPORTED
+        with pytest.raises(mg.MgError) as e:
+            dg.pagerank()
+        assert e.value.code == mg.MG_ERR_UNSUPPORTED
+    neg = mg.HostCSR(np.array([0, 1, 1], np.int32), np.array([1], np.int32), np.array([-3], np.int32))
+    with mg.Graph(neg) as dg:
+        with pytest.raises(mg.MgError) as e:
+            dg.sssp(0)
+        assert e.value.code == mg.MG_ERR_UNSUPPORTED
+
+
+def test_repeated_searches_reuse_state(mg, oracle, device, s16):
+    with mg.Graph(s16, symmetric=True) as dg:
+        b0 = dg.device_bytes()
+        exp = oracle.bfs(s16.row_offsets, s16.col_idx, 5)
+        for _ in range(5):
+            assert np.array_equal(dg.bfs(5), exp)
+            dg.sssp(5)
+            dg.cc()
+        assert dg.device_bytes() == b0
+
+
+def test_device_csc_matches_host(mg, device):
+    g = mg.rmat_graph(12, seed=2, symmetric=False)
+    with mg.Graph(g, build_csc=True) as dg:
+        co, ri, ce = dg.csc()
+    hco, hri, hce = g.csc()
+    assert np.array_equal(co, hco) and np.array_equal(ri, hri) and np.array_equal(ce, hce)
+
+
+# ---------------------------------------------------------------- PageRank
+def _pr_err(got, exp, n):
+    return float(np.max(np.abs(got.astype(np.float64) - exp.astype(np.float64)))) / n
+
+
+def test_pagerank_all_vertex_jacobi(mg, oracle, device):
+    g = mg.rmat_graph(14, 16, seed=42, symmetric=False)
+    with mg.Graph(g, build_csc=True) as dg:
+        got, st = dg.pagerank(mg.options(pr_mode=mg.MG_PR_ALL, iter_num=50), stats=True)
+    assert st.iterations == 50
+    exp, it, _ = oracle.pr_jacobi_f32(g.row_offsets, g.col_idx, iters=50)
+    assert _pr_err(got, exp, g.n) <= PR_TOL
+    ref64, _, _ = oracle.pr_jacobi_f64(g.row_offsets, g.col_idx, iters=50)
+    assert _pr_err(got, ref64, g.n) <= PR_TOL
+    # low in-degree vertices are summed in the oracle's order: bit-exact there
+    indeg = np.bincount(g.col_idx, minlength=g.n)
+    small = indeg <= 32
+    assert np.max(np.abs(got[small] - exp[small]) / np.maximum(exp[small], 1e-30)) < 1e-5
+
+
+def test_pagerank_gas_program(mg, oracle, device):
+    g = mg.rmat_graph(14, 16, seed=7, symmetric=False)
+    with mg.Graph(g, build_csc=True) as dg:
+        got, st = dg.pagerank(stats=True)
+    exp, it = oracle.pr_gas(g.row_offsets, g.col_idx)
+    assert _pr_err(got, exp, g.n) <= PR_TOL
+    assert abs(st.iterations - it) <= 1
+
+
+def test_pagerank_tolerance_stop(mg, oracle, device):
+    g = mg.rmat_graph(12, 16, seed=5, symmetric=False)
+    with mg.Graph(g, build_csc=True) as dg:
+        got, st = dg.pagerank(mg.options(pr_mode=mg.MG_PR_ALL, iter_num=1000, pr_tol=1e-6), stats=True)
+    exp, it, l1 = oracle.pr_jacobi_f32(g.row_offsets, g.col_idx, iters=1000, tol=1e-6)
+    assert it < 1000 and abs(st.iterations - it) <= 1
+    assert _pr_err(got, exp, g.n) <= PR_TOL
+
+
+def test_pagerank_symmetric_graph_uses_csr(mg, oracle, device):
+    g = mg.rmat_graph(12, 8, seed=3)
+    with mg.Graph(g, symmetric=True) as dg:
+        got = dg.pagerank(mg.options(pr_mode=mg.MG_PR_ALL, iter_num=20))
+    exp, _, _ = oracle.pr_jacobi_f32(g.row_offsets, g.col_idx, iters=20)
+    assert _pr_err(got, exp, g.n) <= PR_TOL

@@ -1,0 +1,221 @@
+"""Pins the CPU oracle (oracle/mg_oracle.c) before it is trusted as the checker.
+
+* Known answers for the reference fixtures (smallRegressionGraphs), computed by hand
+  from the files' edges (0-based, source 0).
+* Independent library routines on random graphs: scipy.sparse.csgraph BFS order,
+  Dijkstra and connected components.
+* The two SSSP restatements against each other, and brute force on tiny graphs.
+* PageRank: closed forms (2-cycle, star, chain), an exact float64 power iteration
+  with numpy, and the Gauss-Seidel fixed point.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.csgraph as cg
+
+from conftest import load_mtx, random_graph
+
+INT_MAX = 2**31 - 1
+
+
+def to_scipy(g, weights=None):
+    n = g.n
+    data = np.ones(g.m) if weights is None else weights.astype(np.float64)
+    rows = np.repeat(np.arange(n), np.diff(g.row_offsets))
+    return sp.csr_matrix((data, (rows, g.col_idx)), shape=(n, n))
+
+
+# ---------------------------------------------------------------- fixtures (KATs)
+def test_small_known_answers(mg, oracle):
+    g = load_mtx(mg, "small.mtx")
+    assert g.n == 7 and g.m == 11
+    assert oracle.bfs(g.row_offsets, g.col_idx, 0).tolist() == [0, 1, 1, 1, 2, 2, 2]
+    assert oracle.sssp(g.row_offsets, g.col_idx, g.values, 0).tolist() == [0, 1, 4, 1, 2, 2, 2]
+    assert oracle.sssp(g.row_offsets, g.col_idx, g.values, 0, quadratic=True).tolist() == [0, 1, 4, 1, 2, 2, 2]
+    assert oracle.cc(g.row_offsets, g.col_idx).tolist() == [0] * 7
+
+
+def test_trivial_known_answers(mg, oracle):
+    g = load_mtx(mg, "trivialGraph.mtx")
+    assert oracle.bfs(g.row_offsets, g.col_idx, 0).tolist() == [0, 1, 2, 3, 4]
+    assert oracle.sssp(g.row_offsets, g.col_idx, g.values, 0).tolist() == [0, 1, 2, 3, 4]
+    assert oracle.cc(g.row_offsets, g.col_idx).tolist() == [0] * 5
+    # from the last vertex only the 4 <-> 3 (0-based) cycle is reachable
+    assert oracle.bfs(g.row_offsets, g.col_idx, 4).tolist() == [-1, -1, -1, 1, 0]
+
+
+def test_chesapeake_known_answers(mg, oracle):
+    g = load_mtx(mg, "chesapeake.mtx")
+    assert g.n == 39 and g.m == 170
+    # lower triangle only: as loaded (directed), vertex 0 has no out-edges
+    d = oracle.bfs(g.row_offsets, g.col_idx, 0, directed=1)
+    assert d[0] == 0 and (d[1:] == -1).all()
+    und = oracle.bfs(g.row_offsets, g.col_idx, 0, directed=0)
+    expect = [0, 2, 2, 2, 2, 2, 1, 1, 2, 2, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2,
+              2, 1, 1, 2, 1, 2, 1]
+    assert und.tolist() == expect
+    a = to_scipy(g)
+    ref = cg.shortest_path(a + a.T, unweighted=True, indices=0)
+    assert np.array_equal(und, ref.astype(np.int64))
+    assert oracle.cc(g.row_offsets, g.col_idx).tolist() == [0] * 39
+
+
+def test_sssp_unreached_convention(mg, oracle):
+    g = load_mtx(mg, "trivialGraph.mtx")
+    ref_cpu = oracle.sssp(g.row_offsets, g.col_idx, g.values, 4, unreached=100000000, quadratic=True)
+    assert ref_cpu.tolist() == [100000000, 100000000, 100000000, 1, 0]  # sssp.cu:138-139
+    gpu_conv = oracle.sssp(g.row_offsets, g.col_idx, g.values, 4)
+    assert gpu_conv.tolist() == [INT_MAX, INT_MAX, INT_MAX, 1, 0]  # sssp.h:43
+
+
+# ---------------------------------------------------------------- library routines
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_bfs_matches_scipy(mg, oracle, seed, symmetric):
+    g = random_graph(mg, 300, 700, seed, symmetric=symmetric)
+    src = int(np.argmax(g.degrees()))
+    d = oracle.bfs(g.row_offsets, g.col_idx, src)
+    order, pred = cg.breadth_first_order(to_scipy(g), src, directed=True, return_predecessors=True)
+    expect = np.full(g.n, -1)
+    expect[src] = 0
+    for v in order[1:]:
+        expect[v] = expect[pred[v]] + 1
+    assert np.array_equal(d, expect)
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_sssp_matches_scipy_dijkstra(mg, oracle, seed, symmetric):
+    g = random_graph(mg, 400, 1500, seed, symmetric=symmetric, wmax=64)
+    src = int(np.argmax(g.degrees()))
+    d = oracle.sssp(g.row_offsets, g.col_idx, g.values, src)
+    dq = oracle.sssp(g.row_offsets, g.col_idx, g.values, src, quadratic=True)
+    ref = cg.dijkstra(to_scipy(g, g.values), directed=True, indices=src)
+    expect = np.where(np.isinf(ref), INT_MAX, ref).astype(np.int64)
+    assert np.array_equal(d, expect)
+    assert np.array_equal(dq, expect)
+
+
+def test_sssp_undirected_uses_edge_ids(mg, oracle):
+    # directed=0 relaxes in-edges with the weight of the CSR edge they belong to
+    g = random_graph(mg, 200, 500, 9, symmetric=False, wmax=16)
+    d = oracle.sssp(g.row_offsets, g.col_idx, g.values, 0, directed=0)
+    dq = oracle.sssp(g.row_offsets, g.col_idx, g.values, 0, directed=0, quadratic=True)
+    a = to_scipy(g, g.values)
+    ref = cg.dijkstra(a, directed=False, indices=0)
+    assert np.array_equal(d, np.where(np.isinf(ref), INT_MAX, ref).astype(np.int64))
+    assert np.array_equal(d, dq)
+
+
+@pytest.mark.parametrize("seed", [6, 7])
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_cc_matches_scipy(mg, oracle, seed, symmetric):
+    g = random_graph(mg, 500, 400, seed, symmetric=symmetric)
+    lab = oracle.cc(g.row_offsets, g.col_idx)
+    ncomp, comp = cg.connected_components(to_scipy(g), directed=True, connection="weak")
+    expect = np.empty(g.n, np.int64)
+    for c in range(ncomp):
+        members = np.nonzero(comp == c)[0]
+        expect[members] = members.min()
+    assert np.array_equal(lab, expect)
+
+
+def test_bfs_brute_force_tiny(mg, oracle):
+    rng = np.random.default_rng(11)
+    for trial in range(20):
+        n = int(rng.integers(1, 9))
+        g = random_graph(mg, n, int(rng.integers(0, 20)), 100 + trial, symmetric=bool(trial % 2))
+        a = np.zeros((n, n), bool)
+        for u in range(n):
+            a[u, g.col_idx[g.row_offsets[u]:g.row_offsets[u + 1]]] = True
+        src = int(rng.integers(0, n))
+        expect = np.full(n, -1)
+        reach = np.zeros(n, bool)
+        reach[src] = True
+        expect[src] = 0
+        for k in range(1, n + 1):  # k-step reachability by boolean matrix powers
+            nxt = (reach[:, None] & a).any(0) | reach
+            expect[nxt & ~reach] = k
+            reach = nxt
+        assert np.array_equal(oracle.bfs(g.row_offsets, g.col_idx, src), expect)
+
+
+def test_teps_count(mg, oracle):
+    g = load_mtx(mg, "trivialGraph.mtx")
+    d = oracle.bfs(g.row_offsets, g.col_idx, 4)
+    assert oracle.teps_edges(g.row_offsets, d) == 2  # vertices 3 and 4, one out-edge each
+
+
+# ---------------------------------------------------------------- PageRank
+def _csr(edges, n):
+    import mirrorofmapgraph_amd as mg
+    s = np.array([e[0] for e in edges], np.int32)
+    d = np.array([e[1] for e in edges], np.int32)
+    return mg.csr_from_edges(n, s, d, mg.MG_CSR_SORT_COLS)
+
+
+def test_pr_two_cycle_closed_form(oracle):
+    g = _csr([(0, 1), (1, 0)], 2)
+    # r = 0.15 + 0.85 r has the fixed point r = 1
+    r64, _, _ = oracle.pr_jacobi_f64(g.row_offsets, g.col_idx, iters=400)
+    assert np.allclose(r64, 1.0, atol=1e-12)
+    r32, _, _ = oracle.pr_jacobi_f32(g.row_offsets, g.col_idx, iters=400)
+    assert np.allclose(r32, 1.0, atol=1e-6)
+
+
+def test_pr_star_closed_form(oracle):
+    k = 5  # leaves 1..k point to hub 0; the hub has no out-edges (dangling)
+    g = _csr([(i, 0) for i in range(1, k + 1)], k + 1)
+    r, _, _ = oracle.pr_jacobi_f64(g.row_offsets, g.col_idx, iters=10)
+    assert np.allclose(r[1:], 0.15)
+    assert np.isclose(r[0], 0.15 + 0.85 * 0.15 * k)
+    rg, it = oracle.pr_gas(g.row_offsets, g.col_idx)
+    assert np.allclose(rg, r, atol=1e-6)
+
+
+def test_pr_chain_exact_iterations(oracle):
+    # 0 -> 1 -> 2: after t Jacobi iterations r2 = 0.15 + 0.85*r1_prev
+    g = _csr([(0, 1), (1, 2)], 3)
+    r, _, _ = oracle.pr_jacobi_f64(g.row_offsets, g.col_idx, iters=1)
+    assert np.allclose(r, [0.15, 0.15 + 0.85 * 0.15, 0.15 + 0.85 * 0.15])
+    r, _, _ = oracle.pr_jacobi_f64(g.row_offsets, g.col_idx, iters=2)
+    assert np.allclose(r, [0.15, 0.2775, 0.15 + 0.85 * 0.2775])
+
+
+def test_pr_f64_matches_numpy_power_iteration(mg, oracle):
+    g = random_graph(mg, 300, 1500, 21, symmetric=False)
+    a = to_scipy(g)  # a[u, v] = 1 for u -> v
+    deg = np.asarray(a.sum(1)).ravel()
+    inv = np.where(deg > 0, 1.0 / np.maximum(deg, 1), 0.0)
+    r = np.full(g.n, 0.15)
+    for _ in range(30):
+        r = 0.15 + 0.85 * (a.T @ (r * inv))
+    got, it, _ = oracle.pr_jacobi_f64(g.row_offsets, g.col_idx, iters=30)
+    assert it == 30
+    assert np.allclose(got, r, rtol=0, atol=1e-12)
+    got32, _, _ = oracle.pr_jacobi_f32(g.row_offsets, g.col_idx, iters=30)
+    assert np.max(np.abs(got32 - r)) < 1e-4
+
+
+def test_pr_gauss_seidel_fixed_point(mg, oracle):
+    g = random_graph(mg, 200, 1000, 22, symmetric=False)
+    r, sweeps = oracle.pr_gauss_seidel(g.row_offsets, g.col_idx)
+    assert sweeps > 1
+    # at the stop every vertex moved by < 0.01 in the last sweep, so r is close to
+    # the float64 fixed point
+    ref, _, _ = oracle.pr_jacobi_f64(g.row_offsets, g.col_idx, iters=500)
+    assert np.max(np.abs(r - ref)) < 0.1
+
+
+def test_pr_gas_semantics(mg, oracle):
+    g = random_graph(mg, 300, 1200, 23, symmetric=False)
+    rg, it = oracle.pr_gas(g.row_offsets, g.col_idx)
+    # the program stops when no vertex moves by >= 0.01: one more Jacobi step from
+    # rg moves nothing by 0.01 or more on any vertex with in-edges that changed
+    assert it > 1
+    ref, _, _ = oracle.pr_jacobi_f64(g.row_offsets, g.col_idx, iters=500)
+    assert np.max(np.abs(rg - ref)) < 0.2
+    # iter_num caps the iterations; one iteration equals one all-vertex Jacobi step
+    r1, it1 = oracle.pr_gas(g.row_offsets, g.col_idx, iter_num=1)
+    j1, _, _ = oracle.pr_jacobi_f32(g.row_offsets, g.col_idx, iters=1)
+    assert it1 == 1 and np.array_equal(r1, j1)
