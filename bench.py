@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark: SSSP from vertex 0 + connected components on the seeded R-MAT scale-22
+graph (edge factor 16, (0.57, 0.19, 0.19, 0.05), symmetrized, self-loops and
+duplicates removed, int32 CSR, weights in [1, 64]) on one MI355X.
+
+One step = one SSSP search from vertex 0 plus one CC labelling, graph resident in
+HBM, results left on the device.  value = (traversed edges of SSSP + of CC) / wall
+time of the K timed steps, in GTEPS (traversed edges: the reference's count, sum of
+out-degrees of reached vertices, bfs.cu:483-493).  BFS from vertex 0 is timed
+beside it.  The roofline object covers the SSSP search, the step's dominant part:
+algorithmic bytes 12*m_r + 24*n_r per search over its device time (HIP events on
+the graph's stream, around every kernel of the search).  The CPU baseline runs the
+oracle's restatements of the reference CPU code on the host, single-threaded.
+
+Prints one JSON line.  Only one GPU is supported.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "GTEPS & %HBM roofline, R-MAT s22 BFS/SSSP; batched vs plain; p99 query ms"
+
+
+def bytes_bfs(m_r, n_r):
+    return 8 * m_r + 20 * n_r
+
+
+def bytes_sssp(m_r, n_r):
+    return 12 * m_r + 24 * n_r
+
+
+def bytes_cc(m, n):
+    return 8 * m + 20 * n
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--verbose", action="store_true")
+    a = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}, WORLD_SIZE {world}); "
+              "the engine does not shard a graph over several GPUs", file=sys.stderr)
+        return 2
+
+    import mirrorofmapgraph_amd as mg
+
+    mg.check(mg.lib().mg_device_init(0))
+    log = (lambda *x: print(*x, file=sys.stderr, flush=True)) if a.verbose else (lambda *x: None)
+
+    t0 = time.perf_counter()
+    g = mg.rmat_graph(a.scale, 16, seed=a.seed, weight_max=64)
+    log(f"graph s{a.scale}: n={g.n} m={g.m} built in {time.perf_counter() - t0:.1f}s")
+    dg = mg.Graph(g, symmetric=True)
+    src = 0
+
+    def step():
+        _, s1 = dg.sssp(src, stats=True, copy=False)
+        _, s2 = dg.cc(stats=True, copy=False)
+        return s1, s2
+
+    for _ in range(a.warmup):
+        s1, s2 = step()
+    te_sssp, n_sssp = s1.edges_traversed, s1.vertices_reached
+    te_cc = g.m  # every vertex is labelled; its out-edges count once
+
+    sssp_ms, cc_ms = [], []
+    mg.check(mg.lib().mg_device_synchronize())
+    t = time.perf_counter()
+    for _ in range(a.steps):
+        s1, s2 = step()
+        sssp_ms.append(s1.device_ms)
+        cc_ms.append(s2.device_ms)
+    mg.check(mg.lib().mg_device_synchronize())
+    wall = time.perf_counter() - t
+    ms_per_step = 1e3 * wall / a.steps
+    value = (te_sssp + te_cc) * a.steps / wall / 1e9
+
+    # BFS from the same source, timed beside the step
+    for _ in range(2):
+        dg.bfs(src, stats=True, copy=False)
+    bfs_ms = []
+    for _ in range(a.steps):
+        _, sb = dg.bfs(src, stats=True, copy=False)
+        bfs_ms.append(sb.device_ms)
+
+    def part(te, ms, nbytes):
+        t_ms = float(np.mean(ms))
+        gbs = nbytes / (t_ms * 1e-3) / 1e9
+        return {"gteps": te / (t_ms * 1e-3) / 1e9, "device_ms": t_ms, "device_ms_min": float(np.min(ms)),
+                "algorithmic_bytes": int(nbytes), "achieved_GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+
+    parts = {
+        "sssp": part(te_sssp, sssp_ms, bytes_sssp(te_sssp, n_sssp)) | {"iterations": s1.iterations},
+        "cc": part(te_cc, cc_ms, bytes_cc(g.m, g.n)),
+        "bfs": part(sb.edges_traversed, bfs_ms, bytes_bfs(sb.edges_traversed, sb.vertices_reached))
+               | {"levels": sb.iterations, "bottom_up_levels": sb.bottom_up_levels},
+    }
+    ss = parts["sssp"]
+    roofline = {"bound": "hbm", "achieved": ss["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ss["hbm_frac"], "traffic": None,
+                "kernel": "sssp search (sssp_small + sssp_level_a + sssp_level_b launches of one search)"}
+
+    cpu = None
+    if not a.no_cpu:
+        import oracle
+
+        t = time.perf_counter()
+        oracle.sssp(g.row_offsets, g.col_idx, g.values, src)
+        t_sssp = time.perf_counter() - t
+        t = time.perf_counter()
+        oracle.cc(g.row_offsets, g.col_idx)
+        t_cc = time.perf_counter() - t
+        cpu = {"value": (te_sssp + te_cc) / (t_sssp + t_cc) / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
+               "sample": f"the full step on the same graph: SSSP from vertex 0 (binary-heap Dijkstra restatement of "
+                         f"CPUSSSP, {t_sssp:.2f}s) + CC (CPUCC restatement, {t_cc:.2f}s), one thread",
+               "cpu_model": cpu_model()}
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "GTEPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "rmat22_sssp_cc", "scale": a.scale, "edge_factor": 16, "rmat": [0.57, 0.19, 0.19, 0.05],
+                   "symmetrized": True, "n": g.n, "m": g.m, "weights": [1, 64], "source": src, "seed": a.seed,
+                   "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu, "parts": parts,
+    }
+    print(json.dumps(line))
+    dg.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

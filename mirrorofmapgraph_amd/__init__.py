@@ -338,20 +338,25 @@ class Graph:
         check(lib().mg_graph_get_csc(self.handle, _p(co), _p(ri), _p(ce)))
         return co, ri[: self.m], ce[: self.m]
 
-    def _run(self, fn, *args, opt: Options | None, out: np.ndarray, stats: bool):
+    def _run(self, fn, *args, opt: Options | None, out: np.ndarray | None, stats: bool):
+        """out=None leaves the result on the device (no copy back)."""
         st = Stats()
         o = opt if opt is not None else options()
-        check(fn(self.handle, *args, out.ctypes.data_as(C.c_void_p), C.byref(o), C.byref(st)))
+        ptr = None if out is None else out.ctypes.data_as(C.c_void_p)
+        check(fn(self.handle, *args, ptr, C.byref(o), C.byref(st)))
         return (out, st) if stats else out
 
-    def bfs(self, src: int, opt: Options | None = None, stats: bool = False):
-        return self._run(lib().mg_bfs, src, opt=opt, out=np.empty(self.n, np.int32), stats=stats)
+    def _out(self, copy: bool, dtype=np.int32):
+        return np.empty(self.n, dtype) if copy else None
 
-    def sssp(self, src: int, opt: Options | None = None, stats: bool = False):
-        return self._run(lib().mg_sssp, src, opt=opt, out=np.empty(self.n, np.int32), stats=stats)
+    def bfs(self, src: int, opt: Options | None = None, stats: bool = False, copy: bool = True):
+        return self._run(lib().mg_bfs, src, opt=opt, out=self._out(copy), stats=stats)
 
-    def cc(self, opt: Options | None = None, stats: bool = False):
-        return self._run(lib().mg_cc, opt=opt, out=np.empty(self.n, np.int32), stats=stats)
+    def sssp(self, src: int, opt: Options | None = None, stats: bool = False, copy: bool = True):
+        return self._run(lib().mg_sssp, src, opt=opt, out=self._out(copy), stats=stats)
 
-    def pagerank(self, opt: Options | None = None, stats: bool = False):
-        return self._run(lib().mg_pagerank, opt=opt, out=np.empty(self.n, np.float32), stats=stats)
+    def cc(self, opt: Options | None = None, stats: bool = False, copy: bool = True):
+        return self._run(lib().mg_cc, opt=opt, out=self._out(copy), stats=stats)
+
+    def pagerank(self, opt: Options | None = None, stats: bool = False, copy: bool = True):
+        return self._run(lib().mg_pagerank, opt=opt, out=self._out(copy, np.float32), stats=stats)

@@ -3,19 +3,22 @@
 //
 // The reference runs expand (expand_atomic, LDS/warp/CTA gathering) then contract
 // (contract_atomic: label check-and-set, bitmask/hash culls) per level with a host
-// round trip per level.  Here the level loop lives on the device:
-//   * bfs_small    one 1024-thread workgroup runs whole levels back to back while
-//                  the frontier is small (long-diameter graphs spend almost all
-//                  levels here, with no launch per level);
-//   * bfs_level_a  one level over the whole GPU: top-down (frontier queue, tiled
-//                  through LDS, hubs split into edge chunks) or bottom-up (every
-//                  unvisited vertex scans its in-edges for a parent in the frontier
-//                  bitmap, which stays L2-resident: 512 KB at 4M vertices);
-//   * bfs_level_b  the edge chunks / long bottom-up scans of the level, then the
-//                  last workgroup closes the level (direction choice, queue
-//                  rotation, termination) on the device.
-// Depths are unique, so any traversal order yields the reference's labels
-// (bfs.h:241-253: label = iteration + 1 on first visit; bfs.cu:56-143 on the CPU).
+// round trip per level.  Here the level loop lives on the device; one step is
+//   * bfs_small    one 1024-thread workgroup: closes the previous multi-workgroup
+//                  level, then runs whole top-down levels back to back while the
+//                  frontier is small (frontier queues in HBM, counters in LDS);
+//                  long-diameter graphs spend almost all levels here;
+//   * bfs_level_a  one level over the whole GPU: top-down (frontier bitmap slices
+//                  compacted into LDS, edges gathered through an LDS prefix scan,
+//                  hubs split into edge chunks) or bottom-up (every unvisited
+//                  vertex scans its in-edges for a parent in the frontier bitmap,
+//                  which stays L2-resident: 512 KB at 4M vertices);
+//   * bfs_level_b  the edge chunks / long bottom-up scans of the level.
+// The direction follows Beamer, Asanovic and Patterson (SC'12): bottom-up once the
+// next frontier's out-edges exceed the unexplored edges / alpha, top-down again
+// when the frontier falls below n / beta.  Depths are unique, so any traversal
+// order yields the reference's labels (bfs.h:241-253: label = iteration + 1 on
+// first visit; bfs.cu:56-143 on the CPU).
 #include "frontier.h"
 
 namespace mg {
@@ -28,45 +31,41 @@ struct BfsArgs {
     int *depth;
     uint32_t *visited;
     int *hard;
+    int G;               // workgroups of the level kernels
 };
 
-// Top-down visit: claim u through the visited bitmap, label it, mark it in the next
-// frontier bitmap and append it to the next queue.
+// Visit of edge (v, u) at level L: claim u through the visited bitmap, label it and
+// mark it in the next-frontier bitmap.
 struct BfsPolicy {
     const BfsArgs &a;
-    long long mf = 0;
+    long long nf = 0, mf = 0;
     __device__ explicit BfsPolicy(const BfsArgs &args) : a(args) {}
     __device__ int load(int) const { return 0; }
-    __device__ void visit(int, int, int, int, int u, bool valid, int L) {
+    __device__ bool visit(int, int, int, int, int u, bool valid, int L) {
         bool own = false;
         if (valid && !test_bit(a.visited, u)) own = set_bit_first(a.visited, u);
         if (own) {
             a.depth[u] = L + 1;
             atomicOr(&a.fa.bm[(L + 1) % 3][u >> 5], 1u << (u & 31));
+            ++nf;
             mf += view_degree(a.fa, u);
         }
-        wave_append(own, u, a.fa.q[(L + 1) % 3], &a.fa.ctl->qsize[(L + 1) % 3]);
+        return own;
     }
 };
 
-// Closes level L (one thread): direction choice (Beamer, Asanovic, Patterson 2012:
-// go bottom-up when the next frontier's edges exceed unexplored/alpha, back
-// top-down when the frontier falls below n/beta), then queue rotation.
-__device__ void bfs_level_end(const BfsArgs &a) {
+// Closes level L given the size and out-edges of frontier L+1 (one thread).
+__device__ void bfs_close(const BfsArgs &a, int L, long long nf, long long mf) {
     Ctl *c = a.fa.ctl;
-    const int L = c->level;
-    const long long mf = ld_dev(&c->mf_next);
     c->m_unexplored -= mf;
     int dir = c->dir;
     if (dir == DIR_BU) c->bu_levels++;
-    const int nf = ld_dev(&c->qsize[(L + 1) % 3]);
     if (c->can_bu) {
         if (dir == DIR_TD && mf > c->m_unexplored / c->alpha) dir = DIR_BU;
-        else if (dir == DIR_BU && (long long)nf * c->beta < c->n) dir = DIR_TD;
+        else if (dir == DIR_BU && nf * c->beta < c->n) dir = DIR_TD;
     }
     c->dir = dir;
-    c->mf_cur = mf;
-    rotate_level(c, L);
+    advance_level(c, L, nf, mf);
 }
 
 __global__ void bfs_init(BfsArgs a) {
@@ -96,31 +95,73 @@ __global__ void bfs_seed(BfsArgs a, int src, int iter_limit, int can_bu, int alp
     c->mf_cur = d;
 }
 
-// Single-workgroup levels while the frontier is small (top-down only).
+// Closes a pending multi-workgroup level, then runs small top-down levels.
 __global__ __launch_bounds__(SMALL_BLK) void bfs_small(BfsArgs a) {
     Ctl *c = a.fa.ctl;
-    __shared__ int s_go, s_L;
-    __shared__ unsigned long long s_mf;
+    const FrontierArgs &fa = a.fa;
+    __shared__ int s_go, s_L, s_qv, s_fin, s_cnt;
+    if (c->pending) {
+        const int L = c->level;
+        const bool bu = c->dir == DIR_BU;
+        const int na = (bu ? 1 : fa.nviews) * a.G;
+        long long nf = 0, mf = 0, ex = 0;
+        for (int i = threadIdx.x; i < na + a.G; i += SMALL_BLK) {
+            const LevelPart p = fa.part[i < na ? i : 2 * a.G + (i - na)];
+            nf += p.nf;
+            mf += p.mf;
+            ex += p.ex;
+        }
+        nf = block_sum<SMALL_BLK>(nf);
+        mf = block_sum<SMALL_BLK>(mf);
+        ex = block_sum<SMALL_BLK>(ex);
+        if (threadIdx.x == 0) {
+            c->edges_examined += ex;
+            bfs_close(a, L, nf, mf);
+            c->pending = 0;
+            c->qvalid = 0;
+        }
+        __syncthreads();
+    }
     for (;;) {
         if (threadIdx.x == 0) {
             const int L = c->level;
             s_go = !c->done && c->dir == DIR_TD && c->qsize[L % 3] <= SMALL_Q && c->mf_cur <= SMALL_E;
             s_L = L;
-            s_mf = 0;
+            s_qv = c->qvalid;
+            s_fin = c->qsize[L % 3];
+            s_cnt = 0;
         }
         __syncthreads();
         if (!s_go) break;
-        const int L = s_L;
-        clear_prev<SMALL_BLK>(a.fa, L, 0, 1);
+        const int L = s_L, cur = L % 3, prv = (L + 2) % 3;
+        if (!(s_qv & (1 << cur))) {  // entering from multi-workgroup levels: list frontier L
+            queue_from_bitmap<SMALL_BLK>(fa, L, &s_cnt);
+            if (threadIdx.x == 0) {
+                s_fin = s_cnt;
+                s_cnt = 0;
+            }
+        }
+        if (s_qv & (1 << prv)) {
+            const int *qp = fa.q[prv];
+            for (int i = threadIdx.x; i < c->qsize[prv]; i += SMALL_BLK) fa.bm[prv][qp[i] >> 5] = 0;
+        } else {
+            clear_prev_sweep(fa, L, threadIdx.x, SMALL_BLK);
+        }
         __syncthreads();
         BfsPolicy pol(a);
-        for (int view = 0; view < a.fa.nviews; ++view) td_tiles<SMALL_BLK, false>(a.fa, pol, view, L, 0, 1);
-        if (pol.mf) atomicAdd(&s_mf, (unsigned long long)pol.mf);
-        __syncthreads();
+        int *qout = fa.q[(L + 1) % 3];
+        auto emit = [&](int u, bool newly) { wave_append(newly, u, qout, &s_cnt); };
+        const int *qin = fa.q[cur];
+        for (int view = 0; view < fa.nviews; ++view)
+            for (int t0 = 0; t0 < s_fin; t0 += SMALL_BLK) {
+                const int i = t0 + (int)threadIdx.x;
+                expand_tile<SMALL_BLK, false>(fa, pol, view, L, i < s_fin ? qin[i] : -1, emit);
+            }
+        const long long mf = block_sum<SMALL_BLK>(pol.mf);
         if (threadIdx.x == 0) {
-            c->mf_next = (long long)s_mf;
             c->small_levels++;
-            bfs_level_end(a);
+            bfs_close(a, L, s_cnt, mf);
+            c->qvalid = (1 << cur) | (1 << ((L + 1) % 3));
         }
         __syncthreads();
     }
@@ -129,28 +170,26 @@ __global__ __launch_bounds__(SMALL_BLK) void bfs_small(BfsArgs a) {
 // One level over the whole GPU: top-down for `view`, or the bottom-up sweep.
 __global__ __launch_bounds__(BLK) void bfs_level_a(BfsArgs a, int view) {
     Ctl *c = a.fa.ctl;
+    const FrontierArgs &fa = a.fa;
     if (c->done) return;
     const int L = c->level;
     const int dir = c->dir;
     if (dir == DIR_BU && view != 0) return;
-    if (view == 0) clear_prev<BLK>(a.fa, L, blockIdx.x, gridDim.x);
-    long long mf = 0;
+    if (view == 0) clear_prev_sweep(fa, L, blockIdx.x * BLK + threadIdx.x, gridDim.x * BLK);
+    BfsPolicy pol(a);
     if (dir == DIR_TD) {
-        BfsPolicy pol(a);
-        td_tiles<BLK, true>(a.fa, pol, view, L, blockIdx.x, gridDim.x);
-        mf = pol.mf;
+        expand_bitmap<BLK>(fa, pol, view, L, fa.bm[L % 3]);
     } else {
-        const uint32_t *fr = a.fa.bm[L % 3];
-        uint32_t *nx = a.fa.bm[(L + 1) % 3];
-        int *qout = a.fa.q[(L + 1) % 3];
-        int *qsize = &c->qsize[(L + 1) % 3];
+        __shared__ int s_wave[BLK / WAVE];
+        __shared__ int s_hbase;
+        const uint32_t *fr = fa.bm[L % 3];
+        uint32_t *nx = fa.bm[(L + 1) % 3];
         const int lane = lane_id();
-        const int wave = (blockIdx.x * BLK + threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
-        const int n = a.fa.n;
-        for (int base = wave * WAVE; base < n; base += nwaves * WAVE) {
-            const int v = base + lane;
-            const int w = (base >> 5) + (lane >> 5);
-            const uint32_t vw = a.visited[w];
+        const int n = fa.n;
+        for (int base = blockIdx.x * BLK; base < n; base += gridDim.x * BLK) {
+            const int v = base + (int)threadIdx.x;
+            const int w = v >> 5;  // this wave owns the two words of its 64 vertices
+            const uint32_t vw = v < n ? a.visited[w] : ~0u;
             const bool cand = v < n && !((vw >> (lane & 31)) & 1u);
             bool found = false, hard = false;
             if (cand) {
@@ -165,36 +204,40 @@ __global__ __launch_bounds__(BLK) void bfs_level_a(BfsArgs a, int view) {
             }
             if (found) {
                 a.depth[v] = L + 1;
-                mf += a.row_off[v + 1] - a.row_off[v];
+                ++pol.nf;
+                pol.mf += a.row_off[v + 1] - a.row_off[v];
             }
             const uint64_t fm = wave_ballot(found);
             const uint32_t mine = (lane < 32) ? (uint32_t)fm : (uint32_t)(fm >> 32);
-            if ((lane & 31) == 0 && mine) {  // this wave owns words w of its 64 vertices
+            if ((lane & 31) == 0 && mine) {
                 nx[w] = mine;
                 a.visited[w] = vw | mine;
             }
-            wave_append(found, v, qout, qsize);
-            wave_append(hard, v, a.hard, &c->nhard);
+            int htot;
+            const int hex = block_exclusive_scan<BLK>(hard ? 1 : 0, s_wave, &htot);
+            if (htot > 0) {
+                if (threadIdx.x == 0) s_hbase = atomicAdd(&c->nhard, htot);
+                __syncthreads();
+                if (hard) a.hard[s_hbase + hex] = v;
+                __syncthreads();
+            }
         }
     }
-    mf = wave_sum_ll(mf);
-    if (lane_id() == 0 && mf) atomicAdd((unsigned long long *)&c->mf_next, (unsigned long long)mf);
+    write_part<BLK>(fa, view * a.G + blockIdx.x, pol.nf, pol.mf, 0);
 }
 
-// Edge chunks (top-down) or long in-edge scans (bottom-up) of the level; the last
-// workgroup to finish closes the level.
+// Edge chunks (top-down) or long in-edge scans (bottom-up) of the level.
 __global__ __launch_bounds__(BLK) void bfs_level_b(BfsArgs a) {
     Ctl *c = a.fa.ctl;
+    const FrontierArgs &fa = a.fa;
     if (c->done) return;
     const int L = c->level;
-    long long mf = 0;
+    BfsPolicy pol(a);
     if (c->dir == DIR_TD) {
-        BfsPolicy pol(a);
-        td_chunks<BLK>(a.fa, pol, L);
-        mf = pol.mf;
+        expand_chunks<BLK>(fa, pol, L);
     } else {
-        const uint32_t *fr = a.fa.bm[L % 3];
-        uint32_t *nx = a.fa.bm[(L + 1) % 3];
+        const uint32_t *fr = fa.bm[L % 3];
+        uint32_t *nx = fa.bm[(L + 1) % 3];
         const int lane = lane_id();
         const int wave = (blockIdx.x * BLK + threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
         const int nh = c->nhard;
@@ -212,17 +255,13 @@ __global__ __launch_bounds__(BLK) void bfs_level_b(BfsArgs a) {
                 a.depth[v] = L + 1;
                 atomicOr(&a.visited[v >> 5], bit);
                 atomicOr(&nx[v >> 5], bit);
-                a.fa.q[(L + 1) % 3][atomicAdd(&c->qsize[(L + 1) % 3], 1)] = v;
-                mf += a.row_off[v + 1] - a.row_off[v];
+                ++pol.nf;
+                pol.mf += a.row_off[v + 1] - a.row_off[v];
             }
         }
     }
-    mf = wave_sum_ll(mf);
-    if (lane_id() == 0 && mf) atomicAdd((unsigned long long *)&c->mf_next, (unsigned long long)mf);
-    if (last_block(c)) {
-        bfs_level_end(a);
-        __threadfence();
-    }
+    write_part<BLK>(fa, 2 * a.G + blockIdx.x, pol.nf, pol.mf, 0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->pending = 1;
 }
 
 }  // namespace mg
@@ -255,8 +294,8 @@ extern "C" int mg_bfs(mg_graph *g, int32_t src, int32_t *depth_out, const mg_opt
     a.depth = ws.vals;
     a.visited = ws.visited;
     a.hard = ws.hard;
+    a.G = grid_blocks(std::max<int64_t>(g->n / 4, g->m / 64), BLK);
     const long long m_views = g->m * a.fa.nviews;
-    const int G = grid_blocks(std::max<int64_t>(g->n / 4, g->m / 64), BLK);
 
     MG_HIP(hipEventRecord(g->ev0, g->stream));
     hipLaunchKernelGGL(bfs_init, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, g->stream, a);
@@ -266,12 +305,12 @@ extern "C" int mg_bfs(mg_graph *g, int32_t src, int32_t *depth_out, const mg_opt
     auto step = [&]() -> int {
         hipLaunchKernelGGL(bfs_small, dim3(1), dim3(SMALL_BLK), 0, g->stream, a);
         for (int v = 0; v < a.fa.nviews; ++v)
-            hipLaunchKernelGGL(bfs_level_a, dim3(G), dim3(BLK), 0, g->stream, a, v);
-        hipLaunchKernelGGL(bfs_level_b, dim3(G), dim3(BLK), 0, g->stream, a);
+            hipLaunchKernelGGL(bfs_level_a, dim3(a.G), dim3(BLK), 0, g->stream, a, v);
+        hipLaunchKernelGGL(bfs_level_b, dim3(a.G), dim3(BLK), 0, g->stream, a);
         MG_HIP(hipGetLastError());
         return MG_OK;
     };
-    const int max_steps = (int)std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n) + 2;
+    const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n) + 3;
     if ((rc = drive(g, step, max_steps))) return rc;
     return finish_search(g, stats, ws.vals, -1, depth_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
 }

@@ -57,13 +57,10 @@ __device__ __forceinline__ void uf_unite(int *p, int a, int b) {
 struct CcPolicy {
     int *p;
     bool half;  // symmetric graph: unite each undirected edge once (u < v)
-    long long ex = 0;
     __device__ int load(int) const { return 0; }
-    __device__ void visit(int, int v, int, int, int u, bool valid, int) {
-        if (valid && (!half || u < v)) {
-            ++ex;
-            uf_unite(p, v, u);
-        }
+    __device__ bool visit(int, int v, int, int, int u, bool valid, int) {
+        if (valid && (!half || u < v)) uf_unite(p, v, u);
+        return false;
     }
 };
 
@@ -82,18 +79,19 @@ __global__ void cc_init(FrontierArgs fa, int *p) {
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl_reset(fa.ctl, fa.n, 1, fa.n);
 }
 
+// Every vertex's edges, tiled through LDS; hubs become edge chunks.
 __global__ __launch_bounds__(BLK) void cc_hook(FrontierArgs fa, int *p, int half) {
     CcPolicy pol{p, half != 0};
-    td_tiles<BLK, true, CcPolicy, true>(fa, pol, 0, 0, blockIdx.x, gridDim.x);
-    const long long ex = wave_sum_ll(pol.ex);
-    if (lane_id() == 0 && ex) atomicAdd((unsigned long long *)&fa.ctl->edges_examined, (unsigned long long)ex);
+    auto no_emit = [](int, bool) {};
+    for (int base = blockIdx.x * BLK; base < fa.n; base += gridDim.x * BLK) {
+        const int v = base + (int)threadIdx.x;
+        expand_tile<BLK, true>(fa, pol, 0, 0, v < fa.n ? v : -1, no_emit);
+    }
 }
 
 __global__ __launch_bounds__(BLK) void cc_hook_chunks(FrontierArgs fa, int *p, int half) {
     CcPolicy pol{p, half != 0};
-    td_chunks<BLK>(fa, pol, 0);
-    const long long ex = wave_sum_ll(pol.ex);
-    if (lane_id() == 0 && ex) atomicAdd((unsigned long long *)&fa.ctl->edges_examined, (unsigned long long)ex);
+    expand_chunks<BLK>(fa, pol, 0);
 }
 
 __global__ void cc_compress(FrontierArgs fa, int *p) {

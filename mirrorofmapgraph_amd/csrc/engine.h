@@ -31,8 +31,18 @@ constexpr int BU_INLINE = 32;     // bottom-up: edges a lane scans before handin
 
 enum { DIR_TD = 0, DIR_BU = 1 };
 
+constexpr int MAX_GRID = 2048;    // workgroups of a multi-workgroup launch (256 CUs x 8)
+
 struct ChunkRec {
     int v, beg, end, pad;
+};
+
+// Per-workgroup partial counters of one level (summed by the level close).
+struct LevelPart {
+    long long nf;   // vertices entering the next frontier
+    long long mf;   // their out-edges
+    long long ex;   // edges examined
+    long long pad;
 };
 
 // Control block shared by the kernels of one search (device memory).  Levels are
@@ -52,7 +62,9 @@ struct Ctl {
     int nf_prev;        // frontier size of the previous level
     int bu_levels;
     int small_levels;
-    int changed;        // generic "something changed" counter
+    int changed;        // generic "something changed" counter / overflow flag (bit 30)
+    int pending;        // a multi-workgroup level finished and awaits its close
+    int qvalid;         // bit i: queue q[i] lists the frontier of bitmap bm[i]
     int pad0;
     long long mf_next;        // sum of out-degrees of vertices found this level
     long long m_unexplored;   // out-edges of unreached vertices
@@ -75,6 +87,7 @@ struct Workspace {
     uint32_t *visited = nullptr;
     uint32_t *bm[3] = {nullptr, nullptr, nullptr};
     ChunkRec *chunks = nullptr;
+    LevelPart *part = nullptr;  // 3 * MAX_GRID slots
     Ctl *ctl = nullptr;
     Ctl *h_ctl = nullptr;     // pinned host mirror
     // PageRank state (allocated on first use)

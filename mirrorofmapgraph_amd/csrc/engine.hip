@@ -20,7 +20,7 @@ static thread_local int t_device = 0;
 int grid_blocks(int64_t work_items, int block) {
     int64_t b = (work_items + block - 1) / block;
     if (b < 1) b = 1;
-    if (b > 2048) b = 2048;  // 256 CUs x 8 workgroups; kernels stride beyond this
+    if (b > MAX_GRID) b = MAX_GRID;  // kernels stride beyond this
     return (int)b;
 }
 
@@ -48,6 +48,7 @@ static void free_workspace(Workspace &ws) {
     dfree(ws.visited);
     for (auto &p : ws.bm) dfree(p);
     dfree(ws.chunks);
+    dfree(ws.part);
     dfree(ws.ctl);
     if (ws.h_ctl) (void)hipHostFree(ws.h_ctl);
     dfree(ws.rank);
@@ -74,7 +75,7 @@ int ensure_workspace(mg_graph *g) {
         (rc = dalloc(&ws.q[2], n)) || (rc = dalloc(&ws.hard, n)) || (rc = dalloc(&ws.visited, ws.nwords)) ||
         (rc = dalloc(&ws.bm[0], ws.nwords)) || (rc = dalloc(&ws.bm[1], ws.nwords)) ||
         (rc = dalloc(&ws.bm[2], ws.nwords)) || (rc = dalloc(&ws.chunks, ws.chunk_cap)) ||
-        (rc = dalloc(&ws.ctl, 1))) {
+        (rc = dalloc(&ws.part, 3 * MAX_GRID)) || (rc = dalloc(&ws.ctl, 1))) {
         free_workspace(ws);
         return rc;
     }

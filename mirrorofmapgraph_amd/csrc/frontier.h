@@ -1,16 +1,24 @@
-// Frontier machinery shared by the push-style searches (BFS top-down, SSSP, the
-// generic GAS enactor).
+// Frontier machinery shared by the push-style searches (BFS top-down, SSSP, CC and
+// the generic GAS enactor).
 //
-// One level turns frontier queue q[L%3] into q[(L+1)%3].  Three queues and three
-// bitmaps rotate so that, during level L,
-//   * bm[L%3]     holds frontier L (read by bottom-up steps),
-//   * bm[(L+1)%3] collects frontier L+1 (zero at level start; test-and-set dedupe),
-//   * bm[(L+2)%3] still holds frontier L-1 and is cleared (word by word from queue
-//     q[(L+2)%3] when that frontier was small, else by a sweep).
-// Expansion is tiled: a workgroup takes BLOCK frontier vertices, scans their degrees
-// in LDS and walks the flattened edge list edge-parallel; vertices above CHUNK_T
-// edges become CHUNK-edge records that the closing kernel spreads over the GPU,
-// so a 160K-edge hub costs one level ~160 workgroup-chunks, not one workgroup.
+// Representation.  Frontier L is always the bitmap bm[L%3] (n bits: 512 KB at 4M
+// vertices, L2-resident).  Three bitmaps rotate: during level L, bm[L%3] is read,
+// bm[(L+1)%3] collects frontier L+1 (zero at level start; test-and-set dedupe) and
+// bm[(L+2)%3] (frontier L-1) is cleared.  The single-workgroup small-frontier kernel
+// additionally keeps frontier queues (q[L%3]) so a level costs O(frontier), not
+// O(n); multi-workgroup levels never append to a global queue.
+//
+// Contention.  A same-address atomic costs ~10 ns on the device's memory side and
+// they serialise, so nothing here issues one per wave: counters are per-workgroup
+// partial slots summed by one thread, queue/chunk reservations take one atomic per
+// workgroup round, and the level is closed by the next step's single-workgroup
+// kernel instead of a grid-wide last-block counter.
+//
+// Expansion.  A workgroup compacts the set bits of a slice of the frontier bitmap
+// into an LDS vertex list, then walks it in tiles of BLOCK vertices: degrees are
+// scanned in LDS and the flattened edge list is processed edge-parallel; vertices
+// above CHUNK_T edges become CHUNK-edge records that the second level kernel
+// spreads over the whole GPU (a 160K-edge hub becomes ~160 workgroup-chunks).
 #pragma once
 
 #include "devutil.h"
@@ -28,6 +36,7 @@ struct FrontierArgs {
     int *q[3];
     ChunkRec *chunks;
     int64_t chunk_cap;
+    LevelPart *part;     // per-workgroup partial counters, 2 * MAX_GRID slots
     int n, nwords;
 };
 
@@ -49,61 +58,95 @@ __device__ __forceinline__ long long view_degree(const FrontierArgs &fa, int v) 
     return d;
 }
 
-// Clears bm[(L+2)%3] (frontier L-1) during level L.
+// Deterministic workgroup sum (wave butterfly, then waves in order); every thread
+// gets the result.
+template <int BLOCK, class T>
+__device__ T block_sum(T x) {
+    __shared__ T s_w[BLOCK / WAVE];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    __syncthreads();
+    if (lane_id() == 0) s_w[threadIdx.x / WAVE] = x;
+    __syncthreads();
+    T t = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / WAVE; ++w) t += s_w[w];
+    return t;
+}
+
+// Writes this workgroup's level counters into slot `slot`.
 template <int BLOCK>
-__device__ void clear_prev(const FrontierArgs &fa, int L, int block, int nblocks) {
+__device__ __forceinline__ void write_part(const FrontierArgs &fa, int slot, long long nf, long long mf,
+                                           long long ex) {
+    nf = block_sum<BLOCK>(nf);
+    mf = block_sum<BLOCK>(mf);
+    ex = block_sum<BLOCK>(ex);
+    if (threadIdx.x == 0) {
+        LevelPart p;
+        p.nf = nf;
+        p.mf = mf;
+        p.ex = ex;
+        p.pad = 0;
+        fa.part[slot] = p;
+    }
+}
+
+// Sums the partial slots [0, nslots) (one thread; the previous kernels finished).
+__device__ __forceinline__ LevelPart sum_parts(const FrontierArgs &fa, int nslots) {
+    LevelPart s{0, 0, 0, 0};
+    for (int i = 0; i < nslots; ++i) {
+        s.nf += fa.part[i].nf;
+        s.mf += fa.part[i].mf;
+        s.ex += fa.part[i].ex;
+    }
+    return s;
+}
+
+// Clears bm[(L+2)%3] (frontier L-1) during level L by a grid-stride sweep.
+__device__ __forceinline__ void clear_prev_sweep(const FrontierArgs &fa, int L, int tid, int nt) {
     uint32_t *bm = fa.bm[(L + 2) % 3];
-    const int *q = fa.q[(L + 2) % 3];
-    const int prev = fa.ctl->qsize[(L + 2) % 3];
-    const int tid = block * BLOCK + threadIdx.x, nt = nblocks * BLOCK;
-    if (prev > fa.nwords / 8) {
-        for (int i = tid; i < fa.nwords; i += nt) bm[i] = 0;
-    } else {
-        for (int i = tid; i < prev; i += nt) bm[q[i] >> 5] = 0;
-    }
+    for (int i = tid; i < fa.nwords; i += nt) bm[i] = 0;
 }
 
-// Marks u for the next frontier (test-and-set on bm[(L+1)%3]) and appends it once.
-// Every lane of the wave must call this (wave-aggregated append).
-__device__ __forceinline__ bool enqueue_next(const FrontierArgs &fa, int L, int u, bool want) {
-    bool first = false;
-    if (want) {
-        uint32_t *bm = fa.bm[(L + 1) % 3];
-        if (!test_bit(bm, u)) first = set_bit_first(bm, u);
-    }
-    wave_append(first, u, fa.q[(L + 1) % 3], &fa.ctl->qsize[(L + 1) % 3]);
-    return first;
+// Marks u in the next-frontier bitmap; true for the one call that set the bit.
+__device__ __forceinline__ bool mark_next(const FrontierArgs &fa, int L, int u) {
+    uint32_t *bm = fa.bm[(L + 1) % 3];
+    if (test_bit(bm, u)) return false;
+    return set_bit_first(bm, u);
 }
 
-// Top-down tiles of BLOCK frontier vertices.  P provides
-//   int  load(int v)                                   per-vertex payload,
-//   void visit(int view, int v, int payload, int slot, int u, bool valid, int L)
-// visit is called by every lane of the workgroup in lock-step (valid == false for
-// padding lanes) so it may use wave-aggregated appends.  IDENT walks all vertices
-// 0..n-1 instead of the queue.
-template <int BLOCK, bool CHUNKED, class P, bool IDENT = false>
-__device__ void td_tiles(const FrontierArgs &fa, P &pol, int view, int L, int tile0, int tstride) {
+// One tile of up to BLOCK frontier vertices (thread i holds vertex v_i, or -1).
+// P provides
+//   int  load(int v)                                  per-vertex payload,
+//   bool visit(int view, int v, int payload, int slot, int u, bool valid, int L)
+// visit is called by every thread of the workgroup in lock-step (valid == false for
+// padding) and returns true when u newly entered the next frontier.  With CHUNKED,
+// vertices above CHUNK_T edges become chunk records; otherwise they are expanded in
+// the tile.  `emit(u, newly)` is called, also in lock-step, after each visit.
+template <int BLOCK, bool CHUNKED, class P, class Emit>
+__device__ void expand_tile(const FrontierArgs &fa, P &pol, int view, int L, int v, Emit emit) {
     __shared__ int s_beg[BLOCK];
     __shared__ int s_scan[BLOCK];
     __shared__ int s_v[BLOCK];
     __shared__ int s_pay[BLOCK];
     __shared__ int s_wave[BLOCK / WAVE];
+    __shared__ int s_cbase;
     const int *off = fa.off[view], *adj = fa.adj[view];
-    const int *qin = fa.q[L % 3];
-    const int fin = IDENT ? fa.n : fa.ctl->qsize[L % 3];
-    for (int base = tile0 * BLOCK; base < fin; base += tstride * BLOCK) {
-        const int i = base + threadIdx.x;
-        int v = 0, beg = 0, deg = 0, pay = 0;
-        if (i < fin) {
-            v = IDENT ? i : qin[i];
-            beg = off[v];
-            deg = off[v + 1] - beg;
-            if (deg > 0) pay = pol.load(v);
-        }
-        if (CHUNKED && deg > CHUNK_T) {
-            const int nch = (deg + CHUNK - 1) / CHUNK;
-            const int c0 = atomicAdd(&fa.ctl->nchunks, nch);
-            if (c0 + nch <= fa.chunk_cap) {
+    int beg = 0, deg = 0, pay = 0;
+    if (v >= 0) {
+        beg = off[v];
+        deg = off[v + 1] - beg;
+        if (deg > 0) pay = pol.load(v);
+    }
+    if (CHUNKED) {
+        const int nch = deg > CHUNK_T ? (deg + CHUNK - 1) / CHUNK : 0;
+        int ctot;
+        const int cex = block_exclusive_scan<BLOCK>(nch, s_wave, &ctot);
+        if (ctot > 0) {
+            if (threadIdx.x == 0) s_cbase = atomicAdd(&fa.ctl->nchunks, ctot);
+            __syncthreads();
+            const int c0 = s_cbase + cex;
+            if (nch && c0 + nch <= fa.chunk_cap) {
                 for (int c = 0; c < nch; ++c) {
                     ChunkRec r;
                     r.v = v;
@@ -112,43 +155,72 @@ __device__ void td_tiles(const FrontierArgs &fa, P &pol, int view, int L, int ti
                     r.pad = view;
                     fa.chunks[c0 + c] = r;
                 }
-            } else {
+            } else if (nch) {
                 atomicOr(&fa.ctl->changed, 1 << 30);  // capacity is sized so this cannot happen
             }
-            deg = 0;
         }
-        int total;
-        const int ex = block_exclusive_scan<BLOCK>(deg, s_wave, &total);
-        s_beg[threadIdx.x] = beg;
-        s_scan[threadIdx.x] = ex;
-        s_v[threadIdx.x] = v;
-        s_pay[threadIdx.x] = pay;
-        __syncthreads();
-        for (int k0 = 0; k0 < total; k0 += BLOCK) {
-            const int k = k0 + threadIdx.x;
-            const bool valid = k < total;
-            int u = 0, src = 0, p = 0, slot = 0;
-            if (valid) {
-                int lo = 0, hi = BLOCK - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_scan[mid] <= k) lo = mid;
-                    else hi = mid - 1;
-                }
-                slot = s_beg[lo] + (k - s_scan[lo]);
-                u = adj[slot];
-                src = s_v[lo];
-                p = s_pay[lo];
+        if (nch) deg = 0;
+    }
+    int total;
+    const int ex = block_exclusive_scan<BLOCK>(deg, s_wave, &total);
+    s_beg[threadIdx.x] = beg;
+    s_scan[threadIdx.x] = ex;
+    s_v[threadIdx.x] = v;
+    s_pay[threadIdx.x] = pay;
+    __syncthreads();
+    for (int k0 = 0; k0 < total; k0 += BLOCK) {
+        const int k = k0 + threadIdx.x;
+        const bool valid = k < total;
+        int u = 0, src = 0, p = 0, slot = 0;
+        if (valid) {
+            int lo = 0, hi = BLOCK - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_scan[mid] <= k) lo = mid;
+                else hi = mid - 1;
             }
-            pol.visit(view, src, p, slot, u, valid, L);
+            slot = s_beg[lo] + (k - s_scan[lo]);
+            u = adj[slot];
+            src = s_v[lo];
+            p = s_pay[lo];
+        }
+        const bool newly = pol.visit(view, src, p, slot, u, valid, L);
+        emit(u, newly);
+    }
+    __syncthreads();
+}
+
+// Multi-workgroup top-down over the frontier bitmap `fr`: each round a workgroup
+// compacts BLOCK/2 words (BLOCK*16 vertices) into LDS and expands them in tiles.
+template <int BLOCK, class P>
+__device__ void expand_bitmap(const FrontierArgs &fa, P &pol, int view, int L, const uint32_t *fr) {
+    constexpr int WPR = BLOCK / 2;
+    __shared__ int s_q[WPR * 32];
+    __shared__ int s_wave2[BLOCK / WAVE];
+    auto no_emit = [](int, bool) {};
+    for (int w0 = blockIdx.x * WPR; w0 < fa.nwords; w0 += gridDim.x * WPR) {
+        const int w = w0 + (int)threadIdx.x;
+        uint32_t word = (threadIdx.x < WPR && w < fa.nwords) ? fr[w] : 0u;
+        int total;
+        int k = block_exclusive_scan<BLOCK>(__popc(word), s_wave2, &total);
+        while (word) {
+            const int b = __ffs(word) - 1;
+            s_q[k++] = w * 32 + b;
+            word &= word - 1;
+        }
+        __syncthreads();
+        for (int t0 = 0; t0 < total; t0 += BLOCK) {
+            const int i = t0 + (int)threadIdx.x;
+            const int v = i < total ? s_q[i] : -1;
+            expand_tile<BLOCK, true>(fa, pol, view, L, v, no_emit);
         }
         __syncthreads();
     }
 }
 
-// The edge-chunk records of level L, one workgroup per record at a time.
+// The edge-chunk records of the level, one workgroup per record at a time.
 template <int BLOCK, class P>
-__device__ void td_chunks(const FrontierArgs &fa, P &pol, int L) {
+__device__ void expand_chunks(const FrontierArgs &fa, P &pol, int L) {
     const int nch = (int)min((int64_t)fa.ctl->nchunks, fa.chunk_cap);
     for (int ci = blockIdx.x; ci < nch; ci += gridDim.x) {
         const ChunkRec r = fa.chunks[ci];
@@ -162,53 +234,27 @@ __device__ void td_chunks(const FrontierArgs &fa, P &pol, int L) {
     }
 }
 
-// Last-workgroup-done: returns true in exactly one thread of the grid, after every
-// workgroup's writes are visible.  Must be reached by all threads of every block.
-__device__ __forceinline__ bool last_block(Ctl *c) {
-    __shared__ int s_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        s_last = atomicAdd(&c->blocks_done, 1) == (int)gridDim.x - 1;
+// Builds queue q[L%3] of frontier L from its bitmap inside one workgroup (used when
+// the small-frontier kernel takes over from multi-workgroup levels).
+template <int BLOCK>
+__device__ void queue_from_bitmap(const FrontierArgs &fa, int L, int *s_count) {
+    __shared__ int s_wave3[BLOCK / WAVE];
+    const uint32_t *bm = fa.bm[L % 3];
+    int *q = fa.q[L % 3];
+    for (int w0 = 0; w0 < fa.nwords; w0 += BLOCK) {
+        const int w = w0 + (int)threadIdx.x;
+        uint32_t word = w < fa.nwords ? bm[w] : 0u;
+        int total;
+        int k = block_exclusive_scan<BLOCK>(__popc(word), s_wave3, &total) + *s_count;
+        while (word) {
+            const int b = __ffs(word) - 1;
+            q[k++] = w * 32 + b;
+            word &= word - 1;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) *s_count += total;
+        __syncthreads();
     }
-    __syncthreads();
-    if (s_last && threadIdx.x == 0) {
-        __threadfence();
-        c->blocks_done = 0;
-        return true;
-    }
-    return false;
-}
-
-// As last_block, but true in every thread of the last workgroup.
-__device__ __forceinline__ bool last_block_all(Ctl *c) {
-    __shared__ int s_last_all;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        s_last_all = atomicAdd(&c->blocks_done, 1) == (int)gridDim.x - 1;
-    }
-    __syncthreads();
-    const bool last = s_last_all != 0;
-    if (last) {
-        __threadfence();
-        if (threadIdx.x == 0) c->blocks_done = 0;
-    }
-    return last;
-}
-
-// Resets the per-level counters and rotates the queues after level L; returns the
-// size of frontier L+1.
-__device__ __forceinline__ int rotate_level(Ctl *c, int L) {
-    const int nf = ld_dev(&c->qsize[(L + 1) % 3]);
-    c->nf_prev = nf;
-    c->level = L + 1;
-    c->qsize[(L + 2) % 3] = 0;
-    c->mf_next = 0;
-    c->nchunks = 0;
-    c->nhard = 0;
-    if (nf == 0 || L + 1 >= c->iter_limit) c->done = 1;
-    return nf;
 }
 
 // Fills the search-independent fields of the control block.
@@ -231,6 +277,8 @@ __device__ __forceinline__ void ctl_reset(Ctl *c, int n, int iter_limit, int fro
     c->bu_levels = 0;
     c->small_levels = 0;
     c->changed = 0;
+    c->pending = 0;
+    c->qvalid = 1;
     c->mf_next = 0;
     c->m_unexplored = 0;
     c->edges_traversed = 0;
@@ -240,11 +288,25 @@ __device__ __forceinline__ void ctl_reset(Ctl *c, int n, int iter_limit, int fro
     c->mf_cur = 0;
 }
 
+// Advances the control block past level L given the next frontier's size nf and
+// out-edge count mf: rotation, per-level counters, termination.
+__device__ __forceinline__ void advance_level(Ctl *c, int L, long long nf, long long mf) {
+    c->nf_prev = (int)nf;
+    c->mf_cur = mf;
+    c->level = L + 1;
+    c->qsize[(L + 1) % 3] = (int)nf;
+    c->qsize[(L + 2) % 3] = 0;
+    c->mf_next = 0;
+    c->nchunks = 0;
+    c->nhard = 0;
+    if (nf == 0 || L + 1 >= c->iter_limit) c->done = 1;
+}
+
 __global__ void fill_i32(int *p, int v, int64_t n);
 __global__ void fill_u32(uint32_t *p, uint32_t v, int64_t n);
 __global__ void iota_i32(int *p, int64_t n);
 
-// Sets up the three queues/bitmaps of graph g's workspace and the expansion views.
+// Sets up the bitmaps/queues of graph g's workspace and the expansion views.
 FrontierArgs make_frontier_args(mg_graph *g, int directed);
 int launch_fill(mg_graph *g, int *p, int v, int64_t n);
 int launch_fill(mg_graph *g, uint32_t *p, uint32_t v, int64_t n);

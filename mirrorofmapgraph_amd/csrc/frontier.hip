@@ -54,6 +54,7 @@ FrontierArgs make_frontier_args(mg_graph *g, int directed) {
     }
     fa.chunks = ws.chunks;
     fa.chunk_cap = ws.chunk_cap;
+    fa.part = ws.part;
     fa.n = g->n;
     fa.nwords = ws.nwords;
     return fa;
@@ -61,18 +62,19 @@ FrontierArgs make_frontier_args(mg_graph *g, int directed) {
 
 // Reference TEPS count (bfs.cu:483-493) and reached-vertex count over a result
 // array: sum of CSR out-degrees of vertices whose value differs from `unreached`.
-__global__ void reached_stats(const int *vals, int unreached, const int *row_off, int n, unsigned long long *out) {
-    unsigned long long te = 0, nr = 0;
+__global__ __launch_bounds__(1024) void reached_stats(const int *vals, int unreached, const int *row_off, int n,
+                                                      unsigned long long *out) {
+    long long te = 0, nr = 0;
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
         if (vals[v] != unreached) {
             te += row_off[v + 1] - row_off[v];
             nr++;
         }
-    te = (unsigned long long)wave_sum_ll((long long)te);
-    nr = (unsigned long long)wave_sum_ll((long long)nr);
-    if (lane_id() == 0) {
-        if (te) atomicAdd(&out[0], te);
-        if (nr) atomicAdd(&out[1], nr);
+    te = block_sum<1024>(te);
+    nr = block_sum<1024>(nr);
+    if (threadIdx.x == 0) {  // one atomic per workgroup (at most 256 workgroups)
+        if (te) atomicAdd(&out[0], (unsigned long long)te);
+        if (nr) atomicAdd(&out[1], (unsigned long long)nr);
     }
 }
 
@@ -80,8 +82,8 @@ int launch_reached_stats(mg_graph *g, const int *vals, int unreached, mg_stats *
     if (!st) return MG_OK;
     unsigned long long *d = (unsigned long long *)g->ws.chunks;  // 16 bytes of scratch
     MG_HIP(hipMemsetAsync(d, 0, 16, g->stream));
-    hipLaunchKernelGGL(reached_stats, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, g->stream, vals, unreached,
-                       g->row_off, g->n, d);
+    const int blocks = std::min(256, std::max(1, (g->n + 1023) / 1024));
+    hipLaunchKernelGGL(reached_stats, dim3(blocks), dim3(1024), 0, g->stream, vals, unreached, g->row_off, g->n, d);
     MG_HIP(hipGetLastError());
     unsigned long long h[2];
     MG_HIP(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, g->stream));

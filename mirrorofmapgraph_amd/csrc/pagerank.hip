@@ -43,7 +43,8 @@ struct PrArgs {
     const int *bigv;
     float *part;
     int *pact;
-    double *partials;
+    double *partials;    // per-workgroup L1 change
+    LevelPart *cpart;    // per-workgroup count of changed vertices (nf)
     int n, nwords, nmid, nchunks, nbig, G1, G2;
     int mode;
     float base, damp, eps;
@@ -87,19 +88,15 @@ __device__ __forceinline__ double pr_apply(const PrArgs &a, const PrIter &I, int
     return fabs((double)nv - (double)old);
 }
 
-// Deterministic workgroup sum of a double (wave butterfly, then waves in order).
+// Stores a workgroup's L1 change and changed-vertex count in slot `slot`.
 template <int BLOCK>
-__device__ double block_sum_fixed(double x) {
-    __shared__ double s_w[BLOCK / WAVE];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    if (lane_id() == 0) s_w[threadIdx.x / WAVE] = x;
-    __syncthreads();
-    double t = 0;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < BLOCK / WAVE; ++w) t += s_w[w];
-    __syncthreads();
-    return t;
+__device__ __forceinline__ void pr_write_part(const PrArgs &a, int slot, double delta, long long nchanged) {
+    delta = block_sum<BLOCK>(delta);
+    nchanged = block_sum<BLOCK>(nchanged);
+    if (threadIdx.x == 0) {
+        a.partials[slot] = delta;
+        a.cpart[slot].nf = nchanged;
+    }
 }
 
 __global__ void pr_init(PrArgs a, int iter_limit) {
@@ -138,10 +135,7 @@ __global__ __launch_bounds__(BLK) void pr_k1(PrArgs a) {
         }
         delta += pr_apply(a, I, v, s, act, nchanged);
     }
-    nchanged = wave_sum_i(nchanged);
-    if (lane_id() == 0 && nchanged) atomicAdd(&c->changed, nchanged);
-    const double t = block_sum_fixed<BLK>(delta);
-    if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
+    pr_write_part<BLK>(a, blockIdx.x, delta, nchanged);
 }
 
 // Wave sum of edges [beg, end): lane-strided, then xor-butterfly.  `act` is OR-ed
@@ -161,8 +155,7 @@ __device__ __forceinline__ float wave_gather(const PrArgs &a, const PrIter &I, i
     return s;
 }
 
-// Medium vertices (a wave each) and chunks of high in-degree vertices; the last
-// workgroup combines the chunk sums and closes the iteration.
+// Medium vertices (a wave each) and chunks of high in-degree vertices.
 __global__ __launch_bounds__(BLK) void pr_k2(PrArgs a) {
     Ctl *c = a.ctl;
     if (c->done) return;
@@ -188,48 +181,45 @@ __global__ __launch_bounds__(BLK) void pr_k2(PrArgs a) {
             }
         }
     }
-    nchanged = wave_sum_i(nchanged);
-    if (lane == 0 && nchanged) atomicAdd(&c->changed, nchanged);
-    double t = block_sum_fixed<BLK>(delta);
-    if (threadIdx.x == 0) a.partials[a.G1 + blockIdx.x] = t;
-    if (!last_block_all(c)) return;
-    // ---- last workgroup: high in-degree vertices, then the iteration close
-    delta = 0;
-    nchanged = 0;
-    for (int k = threadIdx.x; k < a.nbig; k += BLK) {
+    pr_write_part<BLK>(a, a.G1 + blockIdx.x, delta, nchanged);
+}
+
+// One workgroup: combines the chunk sums of high in-degree vertices (in chunk
+// order), then closes the iteration from the per-workgroup partials.
+__global__ __launch_bounds__(SMALL_BLK) void pr_k3(PrArgs a) {
+    Ctl *c = a.ctl;
+    if (c->done) return;
+    const PrIter I = pr_iter(a);
+    const bool all = a.mode == MG_PR_ALL || I.it == 0;
+    double delta = 0;
+    int nchanged = 0;
+    for (int k = threadIdx.x; k < a.nbig; k += SMALL_BLK) {
         const int c0 = a.bigv[k], c1 = a.bigv[k + 1];
         const int v = a.chunks[c0].v;
-        float s = __hip_atomic_load(&a.part[c0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        bool act = __hip_atomic_load(&a.pact[c0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+        float s = a.part[c0];
+        bool act = a.pact[c0] != 0;
         for (int ci = c0 + 1; ci < c1; ++ci) {
-            s = s + __hip_atomic_load(&a.part[ci], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            act = act || __hip_atomic_load(&a.pact[ci], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+            s = s + a.part[ci];
+            act = act || a.pact[ci] != 0;
         }
         delta += pr_apply(a, I, v, s, act || all, nchanged);
     }
-    nchanged = wave_sum_i(nchanged);
-    if (lane == 0 && nchanged) atomicAdd(&c->changed, nchanged);
-    t = block_sum_fixed<BLK>(delta);
-    if (threadIdx.x == 0) a.partials[a.G1 + a.G2] = t;
-    __syncthreads();
-    // fixed-order sum of the G1 + G2 + 1 partials
-    const int np = a.G1 + a.G2 + 1;
-    double x = 0;
-    for (int i = threadIdx.x; i < np; i += BLK)
-        x += __hip_atomic_load(&a.partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const double l1 = block_sum_fixed<BLK>(x);
+    long long nch = nchanged;
+    const int np = a.G1 + a.G2;
+    for (int i = threadIdx.x; i < np; i += SMALL_BLK) {
+        delta += a.partials[i];
+        nch += a.cpart[i].nf;
+    }
+    const double l1 = block_sum<SMALL_BLK>(delta);
+    nch = block_sum<SMALL_BLK>(nch);
     if (threadIdx.x == 0) {
-        __threadfence();
-        const int nch = __hip_atomic_load(&c->changed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         c->dsum = l1;
-        c->edges_examined += 0;
         c->level = I.it + 1;
-        c->changed = 0;
+        c->changed = (int)nch;
         bool done = I.it + 1 >= c->iter_limit;
         if (a.mode == MG_PR_GAS && nch == 0) done = true;
         if (a.mode == MG_PR_ALL && a.tol_n > 0 && l1 < a.tol_n) done = true;
         c->done = done;
-        __threadfence();
     }
 }
 
@@ -317,6 +307,7 @@ extern "C" int mg_pagerank(mg_graph *g, float *rank_out, const mg_options *opt, 
     a.part = ws.pr_part;
     a.pact = ws.pr_pact;
     a.partials = ws.partials;
+    a.cpart = ws.part;
     a.n = g->n;
     a.nwords = ws.nwords;
     a.nmid = ws.pr_nmid;
@@ -336,6 +327,7 @@ extern "C" int mg_pagerank(mg_graph *g, float *rank_out, const mg_options *opt, 
     auto step = [&]() -> int {
         hipLaunchKernelGGL(pr_k1, dim3(a.G1), dim3(BLK), 0, g->stream, a);
         hipLaunchKernelGGL(pr_k2, dim3(a.G2), dim3(BLK), 0, g->stream, a);
+        hipLaunchKernelGGL(pr_k3, dim3(1), dim3(SMALL_BLK), 0, g->stream, a);
         MG_HIP(hipGetLastError());
         return MG_OK;
     };
