@@ -41,16 +41,30 @@ struct BfsPolicy {
     long long nf = 0, mf = 0;
     __device__ explicit BfsPolicy(const BfsArgs &args) : a(args) {}
     __device__ int load(int) const { return 0; }
+    __device__ bool claim(int u, int L) {
+        if (!set_bit_first(a.visited, u)) return false;
+        a.depth[u] = L + 1;
+        atomicOr(&a.fa.bm[(L + 1) % 3][u >> 5], 1u << (u & 31));
+        ++nf;
+        mf += view_degree(a.fa, u);
+        return true;
+    }
     __device__ bool visit(int, int, int, int, int u, bool valid, int L) {
-        bool own = false;
-        if (valid && !test_bit(a.visited, u)) own = set_bit_first(a.visited, u);
-        if (own) {
-            a.depth[u] = L + 1;
-            atomicOr(&a.fa.bm[(L + 1) % 3][u >> 5], 1u << (u & 31));
-            ++nf;
-            mf += view_degree(a.fa, u);
+        return valid && !test_bit(a.visited, u) && claim(u, L);
+    }
+    // four neighbour ids, then their four visited words, in flight together
+    __device__ void visit_range(int view, int, int, int beg, int end, int stride, int L) {
+        const int *adj = a.fa.adj[view];
+        for (int e = beg; e < end; e += 4 * stride) {
+            int u[4];
+            load4(adj, e, end, stride, u);
+            uint32_t w[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w[k] = u[k] >= 0 ? a.visited[u[k] >> 5] : ~0u;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (u[k] >= 0 && !((w[k] >> (u[k] & 31)) & 1u)) claim(u[k], L);
         }
-        return own;
     }
 };
 
@@ -178,7 +192,7 @@ __global__ __launch_bounds__(BLK) void bfs_level_a(BfsArgs a, int view) {
     if (view == 0) clear_prev_sweep(fa, L, blockIdx.x * BLK + threadIdx.x, gridDim.x * BLK);
     BfsPolicy pol(a);
     if (dir == DIR_TD) {
-        expand_bitmap<BLK>(fa, pol, view, L, fa.bm[L % 3]);
+        expand_lanes<BLK>(fa, pol, view, L, fa.bm[L % 3]);
     } else {
         __shared__ int s_wave[BLK / WAVE];
         __shared__ int s_hbase;
@@ -195,6 +209,8 @@ __global__ __launch_bounds__(BLK) void bfs_level_a(BfsArgs a, int view) {
             if (cand) {
                 const int beg = a.in_off[v], end = a.in_off[v + 1];
                 const int lim = min(end, beg + BU_INLINE);
+                // one probe at a time: on R-MAT most vertices find a parent at the
+                // first in-neighbour (the lists are sorted, hubs have low ids)
                 for (int e = beg; e < lim; ++e)
                     if (test_bit(fr, a.in_adj[e])) {
                         found = true;
@@ -234,7 +250,7 @@ __global__ __launch_bounds__(BLK) void bfs_level_b(BfsArgs a) {
     const int L = c->level;
     BfsPolicy pol(a);
     if (c->dir == DIR_TD) {
-        expand_chunks<BLK>(fa, pol, L);
+        expand_tasks<BLK>(fa, pol, L);
     } else {
         const uint32_t *fr = fa.bm[L % 3];
         uint32_t *nx = fa.bm[(L + 1) % 3];
@@ -311,6 +327,6 @@ extern "C" int mg_bfs(mg_graph *g, int32_t src, int32_t *depth_out, const mg_opt
         return MG_OK;
     };
     const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n) + 3;
-    if ((rc = drive(g, step, max_steps))) return rc;
+    if ((rc = drive(g, step, max_steps, KIND_BFS, (uint64_t)o.directed))) return rc;
     return finish_search(g, stats, ws.vals, -1, depth_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
 }

@@ -8,26 +8,33 @@
 // (cc.cu:289).  The result is, for every vertex, the smallest vertex id of its
 // weakly connected component.
 //
-// Here the same labels come from one union-find pass (no level loop):
-//   1. parent[v] = the first neighbour u < v when one exists, else v;
-//   2. every edge unites its endpoints, always hooking the larger root under the
-//      smaller one with an atomic compare-and-swap;
-//   3. label[v] = root(v).
-// Parents only ever point to smaller ids, so a component's minimum vertex is never
-// hooked under anything and ends as its root: label[v] = min id of the component,
-// the reference's answer.  Each undirected edge of a symmetric graph is united once
-// (from its larger endpoint); a directed graph unites every CSR edge, which gives
-// the weak components as the reference's two-direction sweep does.
+// Here the same labels come from union-find, where every union hooks the larger
+// root under the smaller one with an atomic compare-and-swap.  Parents only ever
+// point to smaller ids, so a component's minimum vertex is never hooked under
+// anything and ends as its root: label[v] = root(v) = min id of the component.
+//
+// Symmetric graphs use the subgraph-sampling scheme of Afforest (Sutton, Ben-Nun
+// and Barak, IPDPS 2018):
+//   1. unite every vertex with its first two neighbours, compress;
+//   2. sample vertices and take the most frequent root as the large component C;
+//   3. unite the remaining edges of every vertex outside C (an edge between C and a
+//      vertex outside C is seen from the outside end, adjacency being symmetric);
+//   4. compress: label[v] = root(v).
+// Other graphs unite every CSR edge, which gives the weak components.
 #include "frontier.h"
 
 namespace mg {
+
+constexpr int CC_SAMPLE_ROUNDS = 2;
+constexpr int CC_SAMPLES = 1024;
 
 __device__ __forceinline__ int ld_parent(const int *p, int x) {
     return __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Root of x with path halving (parents only move to ancestors, so concurrent
-// halving keeps every parent[] entry pointing to an ancestor).
+// halving keeps every parent[] entry pointing to an ancestor).  Loads are
+// device-coherent: other workgroups hook roots during the same kernel.
 __device__ __forceinline__ int uf_find(int *p, int x) {
     for (;;) {
         const int px = ld_parent(p, x);
@@ -56,7 +63,7 @@ __device__ __forceinline__ void uf_unite(int *p, int a, int b) {
 
 struct CcPolicy {
     int *p;
-    bool half;  // symmetric graph: unite each undirected edge once (u < v)
+    bool half;  // unite each undirected edge once (u < v)
     __device__ int load(int) const { return 0; }
     __device__ bool visit(int, int v, int, int, int u, bool valid, int) {
         if (valid && (!half || u < v)) uf_unite(p, v, u);
@@ -65,33 +72,17 @@ struct CcPolicy {
 };
 
 __global__ void cc_init(FrontierArgs fa, int *p) {
-    const int *off = fa.off[0], *adj = fa.adj[0];
-    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < fa.n; v += gridDim.x * blockDim.x) {
-        int par = v;
-        const int end = min(off[v + 1], off[v] + 8);
-        for (int e = off[v]; e < end; ++e)
-            if (adj[e] < v) {
-                par = adj[e];
-                break;
-            }
-        p[v] = par;
-    }
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < fa.n; v += gridDim.x * blockDim.x) p[v] = v;
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl_reset(fa.ctl, fa.n, 1, fa.n);
 }
 
-// Every vertex's edges, tiled through LDS; hubs become edge chunks.
-__global__ __launch_bounds__(BLK) void cc_hook(FrontierArgs fa, int *p, int half) {
-    CcPolicy pol{p, half != 0};
-    auto no_emit = [](int, bool) {};
-    for (int base = blockIdx.x * BLK; base < fa.n; base += gridDim.x * BLK) {
-        const int v = base + (int)threadIdx.x;
-        expand_tile<BLK, true>(fa, pol, 0, 0, v < fa.n ? v : -1, no_emit);
+// Afforest step 1: unite v with its r-th neighbour.
+__global__ void cc_link_round(FrontierArgs fa, int *p, int r) {
+    const int *off = fa.off[0], *adj = fa.adj[0];
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < fa.n; v += gridDim.x * blockDim.x) {
+        const int e = off[v] + r;
+        if (e < off[v + 1]) uf_unite(p, v, adj[e]);
     }
-}
-
-__global__ __launch_bounds__(BLK) void cc_hook_chunks(FrontierArgs fa, int *p, int half) {
-    CcPolicy pol{p, half != 0};
-    expand_chunks<BLK>(fa, pol, 0);
 }
 
 __global__ void cc_compress(FrontierArgs fa, int *p) {
@@ -100,10 +91,81 @@ __global__ void cc_compress(FrontierArgs fa, int *p) {
         for (int pr = p[r]; pr != r; pr = p[r]) r = pr;
         p[v] = r;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        fa.ctl->level = 1;
-        fa.ctl->done = 1;
+}
+
+// Afforest step 2: most frequent root among CC_SAMPLES hashed vertices (one
+// workgroup; bitonic sort in LDS, then the longest run).
+__global__ __launch_bounds__(CC_SAMPLES) void cc_sample(FrontierArgs fa, const int *p, uint64_t seed) {
+    __shared__ int s[CC_SAMPLES];
+    const int t = threadIdx.x;
+    uint64_t h = (seed + (uint64_t)t * 0x9e3779b97f4a7c15ULL);
+    h = (h ^ (h >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    h = (h ^ (h >> 27)) * 0x94d049bb133111ebULL;
+    h ^= h >> 31;
+    s[t] = p[(int)(h % (uint64_t)fa.n)];
+    __syncthreads();
+    for (int k = 2; k <= CC_SAMPLES; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int i = t ^ j;
+            if (i > t) {
+                const int a = s[t], b = s[i];
+                const bool up = (t & k) == 0;
+                if ((a > b) == up) {
+                    s[t] = b;
+                    s[i] = a;
+                }
+            }
+            __syncthreads();
+        }
+    if (t == 0) {
+        int best = s[0], len = 0, run = 0;
+        for (int i = 0; i < CC_SAMPLES; ++i) {
+            run = (i > 0 && s[i] == s[i - 1]) ? run + 1 : 1;
+            if (run > len) {
+                len = run;
+                best = s[i];
+            }
+        }
+        fa.ctl->aux = best;  // the skipped component's root
+        fa.ctl->nf_prev = len;
     }
+}
+
+// Afforest step 3: remaining edges of every vertex outside the sampled component.
+__global__ __launch_bounds__(BLK) void cc_finish(FrontierArgs fa, int *p) {
+    CcPolicy pol{p, false};
+    auto no_emit = [](int, bool) {};
+    const int skip = fa.ctl->aux;
+    const int *off = fa.off[0];
+    for (int base = blockIdx.x * BLK; base < fa.n; base += gridDim.x * BLK) {
+        const int v = base + (int)threadIdx.x;
+        int beg = 0, deg = 0;
+        if (v < fa.n && p[v] != skip) {
+            beg = off[v] + CC_SAMPLE_ROUNDS;
+            deg = max(0, off[v + 1] - beg);
+        }
+        expand_tile_range<BLK, true>(fa, pol, 0, 0, v < fa.n ? v : -1, beg, deg, no_emit);
+    }
+}
+
+// Every CSR edge (general graphs): tiles through LDS, hubs as edge chunks.
+__global__ __launch_bounds__(BLK) void cc_hook(FrontierArgs fa, int *p) {
+    CcPolicy pol{p, false};
+    auto no_emit = [](int, bool) {};
+    for (int base = blockIdx.x * BLK; base < fa.n; base += gridDim.x * BLK) {
+        const int v = base + (int)threadIdx.x;
+        expand_tile<BLK, true>(fa, pol, 0, 0, v < fa.n ? v : -1, no_emit);
+    }
+}
+
+__global__ __launch_bounds__(BLK) void cc_hook_chunks(FrontierArgs fa, int *p) {
+    CcPolicy pol{p, false};
+    expand_chunks<BLK>(fa, pol, 0);
+}
+
+__global__ void cc_done(Ctl *c) {
+    c->level = 1;
+    c->done = 1;
 }
 
 }  // namespace mg
@@ -118,13 +180,25 @@ extern "C" int mg_cc(mg_graph *g, int32_t *label_out, const mg_options *opt, mg_
     std::lock_guard<std::mutex> lock(g->mu);
     Workspace &ws = g->ws;
     FrontierArgs fa = make_frontier_args(g, 1);
-    const int half = (g->flags & MG_GRAPH_SYMMETRIC) ? 1 : 0;
+    int *p = ws.vals;
     const int G = grid_blocks(std::max<int64_t>(g->n / 4, g->m / 64), BLK);
+    const int Gn = grid_blocks(g->n, BLK);
     MG_HIP(hipEventRecord(g->ev0, g->stream));
-    hipLaunchKernelGGL(cc_init, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, g->stream, fa, ws.vals);
-    hipLaunchKernelGGL(cc_hook, dim3(G), dim3(BLK), 0, g->stream, fa, ws.vals, half);
-    hipLaunchKernelGGL(cc_hook_chunks, dim3(G), dim3(BLK), 0, g->stream, fa, ws.vals, half);
-    hipLaunchKernelGGL(cc_compress, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, g->stream, fa, ws.vals);
+    hipLaunchKernelGGL(cc_init, dim3(Gn), dim3(BLK), 0, g->stream, fa, p);
+    if ((g->flags & MG_GRAPH_SYMMETRIC) && o.engine != MG_ENGINE_GAS && g->n > 0) {
+        for (int r = 0; r < CC_SAMPLE_ROUNDS; ++r) {
+            hipLaunchKernelGGL(cc_link_round, dim3(Gn), dim3(BLK), 0, g->stream, fa, p, r);
+            hipLaunchKernelGGL(cc_compress, dim3(Gn), dim3(BLK), 0, g->stream, fa, p);
+        }
+        hipLaunchKernelGGL(cc_sample, dim3(1), dim3(CC_SAMPLES), 0, g->stream, fa, (const int *)p,
+                           (uint64_t)0x5eed);
+        hipLaunchKernelGGL(cc_finish, dim3(G), dim3(BLK), 0, g->stream, fa, p);
+    } else {
+        hipLaunchKernelGGL(cc_hook, dim3(G), dim3(BLK), 0, g->stream, fa, p);
+    }
+    hipLaunchKernelGGL(cc_hook_chunks, dim3(G), dim3(BLK), 0, g->stream, fa, p);
+    hipLaunchKernelGGL(cc_compress, dim3(Gn), dim3(BLK), 0, g->stream, fa, p);
+    hipLaunchKernelGGL(cc_done, dim3(1), dim3(1), 0, g->stream, ws.ctl);
     MG_HIP(hipGetLastError());
     return finish_search(g, stats, ws.vals, -1, label_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
 }

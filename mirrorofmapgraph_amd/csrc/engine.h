@@ -24,10 +24,12 @@ namespace mg {
 constexpr int BLK = 256;          // threads per workgroup of the multi-workgroup kernels
 constexpr int SMALL_BLK = 1024;   // threads of the single-workgroup small-frontier kernel
 constexpr int CHUNK_T = 256;      // frontier vertices above this degree are split ...
-constexpr int CHUNK = 1024;       // ... into edge chunks of this size
+constexpr int CHUNK = 256;        // ... into edge chunks of this size (one workgroup pass each)
 constexpr int SMALL_Q = 4096;     // small-frontier kernel: max frontier vertices
 constexpr int SMALL_E = 65536;    // small-frontier kernel: max frontier edges
 constexpr int BU_INLINE = 32;     // bottom-up: edges a lane scans before handing the vertex to a wave
+constexpr int LANE_T = 32;        // lane/wave scheme: a lane walks vertices of up to LANE_T edges ...
+constexpr int TASK = 256;         // ... longer lists become TASK-edge tasks, one wave each
 
 enum { DIR_TD = 0, DIR_BU = 1 };
 
@@ -65,7 +67,7 @@ struct Ctl {
     int changed;        // generic "something changed" counter / overflow flag (bit 30)
     int pending;        // a multi-workgroup level finished and awaits its close
     int qvalid;         // bit i: queue q[i] lists the frontier of bitmap bm[i]
-    int pad0;
+    int aux;            // search-specific (CC: root of the sampled large component)
     long long mf_next;        // sum of out-degrees of vertices found this level
     long long m_unexplored;   // out-edges of unreached vertices
     long long edges_traversed;// reference TEPS count
@@ -101,6 +103,10 @@ struct Workspace {
     int32_t *pr_pact = nullptr;                       // per-chunk "an in-neighbour changed" flags
     int pr_nmid = 0, pr_nchunks = 0, pr_nbig = 0;
     const int32_t *pr_view = nullptr;                 // in-edge offsets the buckets were built for
+    // Static pull tasks: TASK-edge pieces of the in-edge lists longer than LANE_T
+    ChunkRec *pull_tasks = nullptr;
+    int pull_ntasks = 0;
+    const int32_t *pull_view = nullptr;
 };
 
 }  // namespace mg
@@ -118,6 +124,7 @@ struct mg_graph {
     int64_t topo_bytes = 0;
     std::mutex mu;
     mg::Workspace ws;
+    struct mg_graph_steps *steps = nullptr;  // captured step graphs
 };
 
 namespace mg {
@@ -144,14 +151,67 @@ int begin_search(mg_graph *g, const mg_options *opt, mg_options *resolved);
 // Copies a device int32/float result to the caller (host or device pointer).
 int deliver(mg_graph *g, void *dst, const void *src, size_t bytes, bool out_on_device);
 // Polls ctl->done on the host every `k` queued steps; `step` enqueues one step.
+// A captured run of STEPS_PER_GRAPH consecutive steps of one search kind.  Kernel
+// arguments are fixed per (graph, kind, key), so the executable graph is reused by
+// every later search with the same key.
+constexpr int STEPS_PER_GRAPH = 8;
+enum { KIND_BFS = 0, KIND_SSSP = 1, KIND_PR = 2, KIND_GAS = 3, KIND_BATCH = 4, NKINDS = 8 };
+
+struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    uint64_t key = ~0ULL;
+};
+
 template <class Step>
-int drive(mg_graph *g, Step step, int64_t max_steps);
+int drive(mg_graph *g, Step step, int64_t max_steps, int kind = -1, uint64_t key = 0);
+bool graphs_enabled();
 
 }  // namespace mg
 
+struct mg_graph_steps {
+    mg::StepGraph sg[mg::NKINDS];
+};
+
+// Queues steps until the device reports `done`.  Steps run from a captured HIP graph
+// (kind >= 0) or as direct launches; the host reads the control block only between
+// batches of steps (1, 2, 4, ... graphs), never once per level.
 template <class Step>
-int mg::drive(mg_graph *g, Step step, int64_t max_steps) {
+int mg::drive(mg_graph *g, Step step, int64_t max_steps, int kind, uint64_t key) {
     Workspace &ws = g->ws;
+    if (kind >= 0 && kind < NKINDS && graphs_enabled()) {
+        StepGraph &sg = g->steps->sg[kind];
+        if (!sg.exec || sg.key != key) {
+            if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+            sg.exec = nullptr;
+            MG_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+            int rc = MG_OK;
+            for (int i = 0; i < STEPS_PER_GRAPH && rc == MG_OK; ++i) rc = step();
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(g->stream, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (e != hipSuccess) return set_error(MG_ERR_DEVICE, "stream capture failed: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&sg.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                sg.exec = nullptr;
+                return set_error(MG_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
+            }
+            sg.key = key;
+        }
+        int64_t queued = 0, k = 1;
+        while (queued < max_steps) {
+            for (int i = 0; i < k && queued < max_steps; ++i, queued += STEPS_PER_GRAPH)
+                MG_HIP(hipGraphLaunch(sg.exec, g->stream));
+            MG_HIP(hipMemcpyAsync(ws.h_ctl, ws.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
+            MG_HIP(hipStreamSynchronize(g->stream));
+            if (ws.h_ctl->done) return MG_OK;
+            if (k < 16) k *= 2;
+        }
+        return MG_OK;
+    }
     int64_t queued = 0, k = 2;
     while (queued < max_steps) {
         for (int i = 0; i < k && queued < max_steps; ++i, ++queued) {

@@ -8,6 +8,7 @@
 // (the reference sizes three key/value queue pairs at max_queue_sizing * m,
 // csr_problem.cuh:627-827), pooled per graph and reused across searches.
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -60,6 +61,7 @@ static void free_workspace(Workspace &ws) {
     dfree(ws.pr_part);
     dfree(ws.pr_pact);
     dfree(ws.pr_bigv);
+    dfree(ws.pull_tasks);
     ws = Workspace();
 }
 
@@ -69,7 +71,9 @@ int ensure_workspace(mg_graph *g) {
     const int n = g->n;
     ws.n = n;
     ws.nwords = ((n + 63) / 64) * 2;
-    ws.chunk_cap = 2 * (g->m / CHUNK + g->m / CHUNK_T) + 64;  // both top-down views
+    // records per level: TASK-edge pieces of lists longer than LANE_T (CHUNK of
+    // lists longer than CHUNK_T in the tile path), over both push views
+    ws.chunk_cap = 2 * (g->m / TASK + g->m / (LANE_T + 1)) + 2 * (g->m / CHUNK + g->m / CHUNK_T) + 64;
     int rc = 0;
     if ((rc = dalloc(&ws.vals, n)) || (rc = dalloc(&ws.q[0], n)) || (rc = dalloc(&ws.q[1], n)) ||
         (rc = dalloc(&ws.q[2], n)) || (rc = dalloc(&ws.hard, n)) || (rc = dalloc(&ws.visited, ws.nwords)) ||
@@ -103,6 +107,14 @@ int alloc_f32(float **p, size_t n) { return dalloc(p, n); }
 int alloc_chunks(ChunkRec **p, size_t n) { return dalloc(p, n); }
 void free_dev(void *p) {
     if (p) (void)hipFree(p);
+}
+
+bool graphs_enabled() {
+    static const bool on = [] {
+        const char *e = getenv("MG_NO_GRAPH");
+        return !(e && *e && *e != '0');
+    }();
+    return on;
 }
 
 int begin_search(mg_graph *g, const mg_options *opt, mg_options *resolved) {
@@ -197,6 +209,11 @@ int mg_graph_create(const int32_t *row_offsets, const int32_t *col_idx, const in
 
     mg_graph *g = new (std::nothrow) mg_graph();
     if (!g) return set_error(MG_ERR_OOM, "host allocation failed");
+    g->steps = new (std::nothrow) mg_graph_steps();
+    if (!g->steps) {
+        delete g;
+        return set_error(MG_ERR_OOM, "host allocation failed");
+    }
     g->device = t_device;
     g->n = n;
     g->m = m;
@@ -246,6 +263,12 @@ int mg_graph_destroy(mg_graph *g) {
     if (!g) return MG_OK;
     (void)hipSetDevice(g->device);
     if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->steps) {
+        for (auto &s : g->steps->sg)
+            if (s.exec) (void)hipGraphExecDestroy(s.exec);
+        delete g->steps;
+        g->steps = nullptr;
+    }
     free_workspace(g->ws);
     dfree(g->row_off);
     dfree(g->col);

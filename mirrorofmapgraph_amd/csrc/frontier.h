@@ -28,10 +28,13 @@ namespace mg {
 
 struct FrontierArgs {
     Ctl *ctl;
-    const int *off[2];   // expansion views: [0] out-edges (CSR), [1] in-edges (CSC) when directed == 0
-    const int *adj[2];
-    const int *eid[2];   // CSR edge id of each slot of a view (nullptr: the slot is the edge id)
-    int nviews;
+    // Views: [0] out-edges (CSR); [1] in-edges (CSC) expanded as well when
+    // directed == 0; [2] the in-edges of the traversed graph for pull steps (the CSR
+    // itself when the graph is symmetric, else the CSC; off[2] == nullptr when none).
+    const int *off[3];
+    const int *adj[3];
+    const int *eid[3];   // CSR edge id of each slot of a view (nullptr: the slot is the edge id)
+    int nviews;          // views expanded by push steps (1 or 2)
     uint32_t *bm[3];
     int *q[3];
     ChunkRec *chunks;
@@ -74,6 +77,24 @@ __device__ T block_sum(T x) {
     return t;
 }
 
+// Workgroup minimum; every thread gets the result.
+template <int BLOCK, class T>
+__device__ T block_min(T x) {
+    __shared__ T s_m[BLOCK / WAVE];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const T y = __shfl_xor(x, o);
+        x = y < x ? y : x;
+    }
+    __syncthreads();
+    if (lane_id() == 0) s_m[threadIdx.x / WAVE] = x;
+    __syncthreads();
+    T t = s_m[0];
+#pragma unroll
+    for (int w = 1; w < BLOCK / WAVE; ++w) t = s_m[w] < t ? s_m[w] : t;
+    return t;
+}
+
 // Writes this workgroup's level counters into slot `slot`.
 template <int BLOCK>
 __device__ __forceinline__ void write_part(const FrontierArgs &fa, int slot, long long nf, long long mf,
@@ -102,10 +123,14 @@ __device__ __forceinline__ LevelPart sum_parts(const FrontierArgs &fa, int nslot
     return s;
 }
 
-// Clears bm[(L+2)%3] (frontier L-1) during level L by a grid-stride sweep.
+// Clears bm[(L+2)%3] (frontier L-1) during level L by a sweep of 16-byte stores
+// (bitmaps are 256-byte aligned; nwords is even).
 __device__ __forceinline__ void clear_prev_sweep(const FrontierArgs &fa, int L, int tid, int nt) {
     uint32_t *bm = fa.bm[(L + 2) % 3];
-    for (int i = tid; i < fa.nwords; i += nt) bm[i] = 0;
+    const int n4 = fa.nwords / 4;
+    uint4 *p = reinterpret_cast<uint4 *>(bm);
+    for (int i = tid; i < n4; i += nt) p[i] = make_uint4(0, 0, 0, 0);
+    for (int i = 4 * n4 + tid; i < fa.nwords; i += nt) bm[i] = 0;
 }
 
 // Marks u in the next-frontier bitmap; true for the one call that set the bit.
@@ -123,21 +148,35 @@ __device__ __forceinline__ bool mark_next(const FrontierArgs &fa, int L, int u) 
 // padding) and returns true when u newly entered the next frontier.  With CHUNKED,
 // vertices above CHUNK_T edges become chunk records; otherwise they are expanded in
 // the tile.  `emit(u, newly)` is called, also in lock-step, after each visit.
+// As expand_tile, over the explicit slot range [beg, beg + deg) of vertex v.
+template <int BLOCK, bool CHUNKED, class P, class Emit>
+__device__ void expand_tile_range(const FrontierArgs &fa, P &pol, int view, int L, int v, int beg, int deg,
+                                  Emit emit);
+
 template <int BLOCK, bool CHUNKED, class P, class Emit>
 __device__ void expand_tile(const FrontierArgs &fa, P &pol, int view, int L, int v, Emit emit) {
+    const int *off = fa.off[view];
+    int beg = 0, deg = 0;
+    if (v >= 0) {
+        beg = off[v];
+        deg = off[v + 1] - beg;
+    }
+    expand_tile_range<BLOCK, CHUNKED>(fa, pol, view, L, v, beg, deg, emit);
+}
+
+template <int BLOCK, bool CHUNKED, class P, class Emit>
+__device__ void expand_tile_range(const FrontierArgs &fa, P &pol, int view, int L, int v, int beg, int deg,
+                                  Emit emit) {
     __shared__ int s_beg[BLOCK];
     __shared__ int s_scan[BLOCK];
     __shared__ int s_v[BLOCK];
     __shared__ int s_pay[BLOCK];
     __shared__ int s_wave[BLOCK / WAVE];
     __shared__ int s_cbase;
-    const int *off = fa.off[view], *adj = fa.adj[view];
-    int beg = 0, deg = 0, pay = 0;
-    if (v >= 0) {
-        beg = off[v];
-        deg = off[v + 1] - beg;
-        if (deg > 0) pay = pol.load(v);
-    }
+    const int *adj = fa.adj[view];
+    int pay = 0;
+    if (v < 0) deg = 0;
+    if (deg > 0) pay = pol.load(v);
     if (CHUNKED) {
         const int nch = deg > CHUNK_T ? (deg + CHUNK - 1) / CHUNK : 0;
         int ctot;
@@ -218,6 +257,88 @@ __device__ void expand_bitmap(const FrontierArgs &fa, P &pol, int view, int L, c
     }
 }
 
+// Multi-workgroup push over the frontier bitmap `fr` without workgroup barriers in
+// the edge loop: each round a workgroup compacts BLOCK/2 bitmap words into LDS; a
+// lane walks the edges of each listed vertex of degree <= LANE_T itself, longer
+// lists become TASK-edge task records (one reservation atomic per workgroup round)
+// for expand_tasks.  The policy's visit_range(view, v, payload, beg, end, stride, L)
+// handles slots beg, beg+stride, ... < end of one lane; it must not use wave-wide
+// operations (lanes diverge) and should batch its loads (see load4).
+template <int BLOCK, class P>
+__device__ void expand_lanes(const FrontierArgs &fa, P &pol, int view, int L, const uint32_t *fr) {
+    constexpr int WPR = BLOCK / 2;
+    __shared__ int s_q[WPR * 32];
+    __shared__ int s_wave2[BLOCK / WAVE];
+    __shared__ int s_tbase;
+    const int *off = fa.off[view], *adj = fa.adj[view];
+    for (int w0 = blockIdx.x * WPR; w0 < fa.nwords; w0 += gridDim.x * WPR) {
+        const int w = w0 + (int)threadIdx.x;
+        uint32_t word = (threadIdx.x < WPR && w < fa.nwords) ? fr[w] : 0u;
+        int total;
+        int k = block_exclusive_scan<BLOCK>(__popc(word), s_wave2, &total);
+        while (word) {
+            const int b = __ffs(word) - 1;
+            s_q[k++] = w * 32 + b;
+            word &= word - 1;
+        }
+        __syncthreads();
+        for (int t0 = 0; t0 < total; t0 += BLOCK) {
+            const int i = t0 + (int)threadIdx.x;
+            int v = 0, beg = 0, deg = 0;
+            if (i < total) {
+                v = s_q[i];
+                beg = off[v];
+                deg = off[v + 1] - beg;
+            }
+            const int nt = deg > LANE_T ? (deg + TASK - 1) / TASK : 0;
+            int ttot;
+            const int tex = block_exclusive_scan<BLOCK>(nt, s_wave2, &ttot);
+            if (ttot > 0) {
+                if (threadIdx.x == 0) s_tbase = atomicAdd(&fa.ctl->nchunks, ttot);
+                __syncthreads();
+                const int c0 = s_tbase + tex;
+                if (nt && c0 + nt <= fa.chunk_cap) {
+                    for (int c = 0; c < nt; ++c) {
+                        ChunkRec r;
+                        r.v = v;
+                        r.beg = beg + c * TASK;
+                        r.end = min(beg + (c + 1) * TASK, beg + deg);
+                        r.pad = view;
+                        fa.chunks[c0 + c] = r;
+                    }
+                } else if (nt) {
+                    atomicOr(&fa.ctl->changed, 1 << 30);  // capacity is sized so this cannot happen
+                }
+                __syncthreads();
+            }
+            if (nt == 0 && deg > 0) pol.visit_range(view, v, pol.load(v), beg, beg + deg, 1, L);
+        }
+        __syncthreads();
+    }
+}
+
+// The task records of the level, one wave per record, lanes striding its edges.
+template <int BLOCK, class P>
+__device__ void expand_tasks(const FrontierArgs &fa, P &pol, int L) {
+    const int nch = (int)min((int64_t)fa.ctl->nchunks, fa.chunk_cap);
+    const int lane = lane_id();
+    const int wave = (blockIdx.x * BLOCK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLOCK / WAVE;
+    for (int t = wave; t < nch; t += nwaves) {
+        const ChunkRec r = fa.chunks[t];
+        pol.visit_range(r.pad, r.v, pol.load(r.v), r.beg + lane, r.end, WAVE, L);
+    }
+}
+
+// Edge slots beg, beg+stride, ... < end of `view` in groups of 4 with the neighbour
+// ids loaded together: out[k] = adj[slot_k] (or -1 past the end).
+__device__ __forceinline__ void load4(const int *adj, int e, int end, int stride, int (&u)[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int s = e + k * stride;
+        u[k] = s < end ? adj[s] : -1;
+    }
+}
+
 // The edge-chunk records of the level, one workgroup per record at a time.
 template <int BLOCK, class P>
 __device__ void expand_chunks(const FrontierArgs &fa, P &pol, int L) {
@@ -235,21 +356,45 @@ __device__ void expand_chunks(const FrontierArgs &fa, P &pol, int L) {
 }
 
 // Builds queue q[L%3] of frontier L from its bitmap inside one workgroup (used when
-// the small-frontier kernel takes over from multi-workgroup levels).
+// the small-frontier kernel takes over from multi-workgroup levels).  Each thread
+// takes 16 consecutive words (four 16-byte loads in flight) per round, so a 4M-vertex
+// bitmap is 8 rounds of one workgroup rather than 128 load-then-barrier rounds.
 template <int BLOCK>
 __device__ void queue_from_bitmap(const FrontierArgs &fa, int L, int *s_count) {
+    constexpr int WPT = 16;
     __shared__ int s_wave3[BLOCK / WAVE];
     const uint32_t *bm = fa.bm[L % 3];
     int *q = fa.q[L % 3];
-    for (int w0 = 0; w0 < fa.nwords; w0 += BLOCK) {
-        const int w = w0 + (int)threadIdx.x;
-        uint32_t word = w < fa.nwords ? bm[w] : 0u;
+    for (int w0 = 0; w0 < fa.nwords; w0 += BLOCK * WPT) {
+        const int wb = w0 + (int)threadIdx.x * WPT;
+        uint32_t words[WPT];
+        if (wb + WPT <= fa.nwords) {
+            const uint4 *p = reinterpret_cast<const uint4 *>(bm + wb);
+#pragma unroll
+            for (int j = 0; j < WPT / 4; ++j) {
+                const uint4 x = p[j];
+                words[4 * j] = x.x;
+                words[4 * j + 1] = x.y;
+                words[4 * j + 2] = x.z;
+                words[4 * j + 3] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < WPT; ++j) words[j] = wb + j < fa.nwords ? bm[wb + j] : 0u;
+        }
+        int c = 0;
+#pragma unroll
+        for (int j = 0; j < WPT; ++j) c += __popc(words[j]);
         int total;
-        int k = block_exclusive_scan<BLOCK>(__popc(word), s_wave3, &total) + *s_count;
-        while (word) {
-            const int b = __ffs(word) - 1;
-            q[k++] = w * 32 + b;
-            word &= word - 1;
+        int k = block_exclusive_scan<BLOCK>(c, s_wave3, &total) + *s_count;
+#pragma unroll
+        for (int j = 0; j < WPT; ++j) {
+            uint32_t word = words[j];
+            while (word) {
+                const int b = __ffs(word) - 1;
+                q[k++] = (wb + j) * 32 + b;
+                word &= word - 1;
+            }
         }
         __syncthreads();
         if (threadIdx.x == 0) *s_count += total;
@@ -311,6 +456,7 @@ FrontierArgs make_frontier_args(mg_graph *g, int directed);
 int launch_fill(mg_graph *g, int *p, int v, int64_t n);
 int launch_fill(mg_graph *g, uint32_t *p, uint32_t v, int64_t n);
 int launch_reached_stats(mg_graph *g, const int *vals, int unreached, mg_stats *st);
+int ensure_pull_tasks(mg_graph *g, const int *d_in_off);
 int finish_search(mg_graph *g, mg_stats *stats, const int *vals, int unreached, void *out, const void *result,
                   size_t bytes, bool out_on_device, const CallTimer &timer);
 

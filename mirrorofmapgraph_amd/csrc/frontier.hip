@@ -1,4 +1,6 @@
 // Shared host/device helpers of the frontier searches.
+#include <vector>
+
 #include "frontier.h"
 
 namespace mg {
@@ -48,6 +50,15 @@ FrontierArgs make_frontier_args(mg_graph *g, int directed) {
         fa.adj[1] = fa.adj[0];
         fa.eid[1] = nullptr;
     }
+    fa.off[2] = fa.adj[2] = fa.eid[2] = nullptr;
+    if (g->flags & MG_GRAPH_SYMMETRIC) {
+        fa.off[2] = g->row_off;
+        fa.adj[2] = g->col;
+    } else if (directed == 1 && g->col_off) {
+        fa.off[2] = g->col_off;
+        fa.adj[2] = g->row_idx;
+        fa.eid[2] = g->csc_eid;
+    }
     for (int i = 0; i < 3; ++i) {
         fa.bm[i] = ws.bm[i];
         fa.q[i] = ws.q[i];
@@ -58,6 +69,30 @@ FrontierArgs make_frontier_args(mg_graph *g, int directed) {
     fa.n = g->n;
     fa.nwords = ws.nwords;
     return fa;
+}
+
+// Builds (once per graph and in-edge view) the TASK-edge pieces of every in-edge list
+// longer than LANE_T, for pull steps; the host walks the offsets once.
+int ensure_pull_tasks(mg_graph *g, const int *d_in_off) {
+    Workspace &ws = g->ws;
+    if (ws.pull_view == d_in_off) return MG_OK;
+    const int n = g->n;
+    std::vector<int32_t> off((size_t)n + 1);
+    MG_HIP(hipMemcpy(off.data(), d_in_off, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
+    std::vector<ChunkRec> t;
+    for (int v = 0; v < n; ++v) {
+        if (off[v + 1] - off[v] <= LANE_T) continue;
+        for (int b = off[v]; b < off[v + 1]; b += TASK) t.push_back(ChunkRec{v, b, std::min(b + TASK, off[v + 1]), 2});
+    }
+    free_dev(ws.pull_tasks);
+    ws.pull_tasks = nullptr;
+    ws.pull_view = nullptr;
+    int rc = alloc_chunks(&ws.pull_tasks, t.size());
+    if (rc) return rc;
+    if (!t.empty()) MG_HIP(hipMemcpy(ws.pull_tasks, t.data(), sizeof(ChunkRec) * t.size(), hipMemcpyHostToDevice));
+    ws.pull_ntasks = (int)t.size();
+    ws.pull_view = d_in_off;
+    return MG_OK;
 }
 
 // Reference TEPS count (bfs.cu:483-493) and reached-vertex count over a result

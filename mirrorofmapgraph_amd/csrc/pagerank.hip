@@ -331,7 +331,12 @@ extern "C" int mg_pagerank(mg_graph *g, float *rank_out, const mg_options *opt, 
         MG_HIP(hipGetLastError());
         return MG_OK;
     };
-    if ((rc = drive(g, step, (int64_t)std::max(o.iter_num, 0) + 1))) return rc;
+    // the captured steps bake in the per-call constants: key on all of them
+    uint64_t key = (uint64_t)a.mode;
+    for (uint64_t x : {(uint64_t)__builtin_bit_cast(uint32_t, a.base), (uint64_t)__builtin_bit_cast(uint32_t, a.eps),
+                       (uint64_t)__builtin_bit_cast(uint64_t, a.tol_n), (uint64_t)(uintptr_t)in_off})
+        key = mix64(key ^ x);
+    if ((rc = drive(g, step, (int64_t)std::max(o.iter_num, 0) + 1, KIND_PR, key))) return rc;
     rc = finish_search(g, stats, nullptr, 0, rank_out, ws.rank, 4 * (size_t)g->n, o.out_on_device, timer);
     if (rc) return rc;
     if (stats) {

@@ -5,53 +5,130 @@
 // dist[v] + w for out-edges that improve on dist[nbr] (sssp.h:342-357), contract
 // atomicMin's it into gather_tmp (sssp.h:384-401), apply/post_apply commit the
 // minimum and mark changed vertices (sssp.h:242-280), and changed vertices form the
-// next frontier.  Here one fused pass per level relaxes an edge straight into dist
-// with atomicMin and marks the target in the next-frontier bitmap when it improved;
-// the level loop, frontier rotation and termination stay on the device (same step
-// structure as BFS: a single-workgroup kernel for small frontiers and the level
-// close, two whole-GPU kernels per large level).  Integer shortest distances are
-// unique, so the result equals the reference's: dist[src] = 0, unreached = INT_MAX
-// (sssp.h:42-43, 95-100).
+// next frontier.  Here each level is one of two fused steps, chosen on the device:
+//   * push  (sparse frontiers) relaxes the frontier's out-edges straight into dist
+//           with atomicMin and marks improved targets in the next-frontier bitmap;
+//   * pull  (dense frontiers, when an in-edge view exists) lets every vertex take
+//           the minimum of dist[u] + w over its in-edges from frontier vertices:
+//           no atomics, edges streamed once, the minimum reduced in LDS per tile.
+// Both are Bellman-Ford relaxations; any vertex whose distance drops during a level
+// is in the next frontier, so the fixed point is the same.  Integer shortest
+// distances are unique, so the result equals the reference's: dist[src] = 0,
+// unreached = INT_MAX (sssp.h:42-43, 95-100).  The step structure is BFS's: one
+// single-workgroup kernel for small frontiers and the level close, two whole-GPU
+// kernels per large level.
 #include <climits>
 
 #include "frontier.h"
 
 namespace mg {
 
+enum { DIR_PUSH = DIR_TD, DIR_PULL = DIR_BU };
+
 struct SsspArgs {
     FrontierArgs fa;
     int *dist;
     const int *w;
+    const ChunkRec *pull;  // static pull tasks (in-edge lists longer than LANE_T)
+    int npull;
     int G;
 };
 
-struct SsspPolicy {
+__device__ __forceinline__ long long edge_w(const SsspArgs &a, int view, int slot) {
+    return a.w[edge_id(a.fa, view, slot)];
+}
+
+// Push visit: relax (v, u); mark u when it improved.
+struct SsspPush {
     const SsspArgs &a;
     long long nf = 0, mf = 0, ex = 0;
-    __device__ explicit SsspPolicy(const SsspArgs &args) : a(args) {}
+    __device__ explicit SsspPush(const SsspArgs &args) : a(args) {}
     __device__ int load(int v) const { return a.dist[v]; }
+    // u's distance dropped (to at most nd) this level: mark it once.  A stale
+    // (larger) dist[u] read only causes a redundant relaxation: dist[u] dropped
+    // this level either way, so u belongs to the next frontier.
+    __device__ bool relax(long long nd, int du, int u, int L) {
+        if (nd >= (long long)du) return false;
+        atomicMin(&a.dist[u], (int)nd);
+        if (!mark_next(a.fa, L, u)) return false;
+        ++nf;
+        mf += view_degree(a.fa, u);
+        return true;
+    }
     __device__ bool visit(int view, int, int d, int slot, int u, bool valid, int L) {
-        bool newly = false;
-        if (valid) {
-            ++ex;
-            const long long nd = (long long)d + a.w[edge_id(a.fa, view, slot)];
-            if (nd < (long long)a.dist[u] && (int)nd < atomicMin(&a.dist[u], (int)nd) && mark_next(a.fa, L, u)) {
-                newly = true;
-                ++nf;
-                mf += view_degree(a.fa, u);
+        if (!valid) return false;
+        ++ex;
+        return relax((long long)d + edge_w(a, view, slot), a.dist[u], u, L);
+    }
+    // four targets, weights and target distances in flight together
+    __device__ void visit_range(int view, int, int d, int beg, int end, int stride, int L) {
+        const int *adj = a.fa.adj[view];
+        for (int e = beg; e < end; e += 4 * stride) {
+            int u[4], wt[4], du[4];
+            load4(adj, e, end, stride, u);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                wt[k] = u[k] >= 0 ? (int)edge_w(a, view, e + k * stride) : 0;
+                du[k] = u[k] >= 0 ? a.dist[u[k]] : INT_MIN;
             }
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (u[k] >= 0) {
+                    ++ex;
+                    relax((long long)d + wt[k], du[k], u[k], L);
+                }
         }
-        return newly;
     }
 };
 
-__global__ void sssp_seed(SsspArgs a, int src, int iter_limit) {
+// Pull: the best dist[u] + w over in-edge slots [beg, end) from frontier vertices u.
+// Loads go out four at a time: neighbour ids, then their frontier bits, then the
+// distances and weights of the frontier ones.
+__device__ __forceinline__ long long pull_best(const SsspArgs &a, const uint32_t *fr, int beg, int end, int step) {
+    long long best = INT_MAX;
+    const int *adj = a.fa.adj[2];
+    for (int e = beg; e < end; e += 4 * step) {
+        int u[4];
+        load4(adj, e, end, step, u);
+        bool f[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) f[k] = u[k] >= 0 && test_bit(fr, u[k]);
+        long long cand[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            cand[k] = f[k] ? (long long)a.dist[u[k]] + edge_w(a, 2, e + k * step) : (long long)INT_MAX;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) best = cand[k] < best ? cand[k] : best;
+    }
+    return best;
+}
+
+__device__ void sssp_close(const SsspArgs &a, int L, long long nf, long long mf) {
+    Ctl *c = a.fa.ctl;
+    int dir = c->dir;
+    if (dir == DIR_PULL) c->bu_levels++;
+    if (c->can_bu) {
+        const long long m = c->m_unexplored;  // total edges (set at seed)
+        if (dir == DIR_PUSH && mf * c->alpha > m) dir = DIR_PULL;
+        else if (dir == DIR_PULL && mf * c->beta < m) dir = DIR_PUSH;
+    }
+    c->dir = dir;
+    advance_level(c, L, nf, mf);
+}
+
+__global__ void sssp_seed(SsspArgs a, int src, int iter_limit, int can_pull, long long m) {
     Ctl *c = a.fa.ctl;
     a.dist[src] = 0;
     a.fa.bm[0][src >> 5] = 1u << (src & 31);
     a.fa.q[0][0] = src;
     ctl_reset(c, a.fa.n, iter_limit, 1);
     c->mf_cur = view_degree(a.fa, src);
+    // pull only pays on the densest levels: every in-edge is read and the frontier
+    // ones gather dist[u]; push gathers only along frontier out-edges
+    c->can_bu = can_pull;
+    c->alpha = 4;
+    c->beta = 8;
+    c->m_unexplored = m;
 }
 
 __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
@@ -60,7 +137,7 @@ __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
     __shared__ int s_go, s_L, s_qv, s_fin, s_cnt;
     if (c->pending) {
         const int L = c->level;
-        const int na = fa.nviews * a.G;
+        const int na = (c->dir == DIR_PULL ? 1 : fa.nviews) * a.G;
         long long nf = 0, mf = 0, ex = 0;
         for (int i = threadIdx.x; i < na + a.G; i += SMALL_BLK) {
             const LevelPart p = fa.part[i < na ? i : 2 * a.G + (i - na)];
@@ -73,7 +150,7 @@ __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
         ex = block_sum<SMALL_BLK>(ex);
         if (threadIdx.x == 0) {
             c->edges_examined += ex;
-            advance_level(c, L, nf, mf);
+            sssp_close(a, L, nf, mf);
             c->pending = 0;
             c->qvalid = 0;
         }
@@ -82,7 +159,7 @@ __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
     for (;;) {
         if (threadIdx.x == 0) {
             const int L = c->level;
-            s_go = !c->done && c->qsize[L % 3] <= SMALL_Q && c->mf_cur <= SMALL_E;
+            s_go = !c->done && c->dir == DIR_PUSH && c->qsize[L % 3] <= SMALL_Q && c->mf_cur <= SMALL_E;
             s_L = L;
             s_qv = c->qvalid;
             s_fin = c->qsize[L % 3];
@@ -105,7 +182,7 @@ __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
             clear_prev_sweep(fa, L, threadIdx.x, SMALL_BLK);
         }
         __syncthreads();
-        SsspPolicy pol(a);
+        SsspPush pol(a);
         int *qout = fa.q[(L + 1) % 3];
         auto emit = [&](int u, bool newly) { wave_append(newly, u, qout, &s_cnt); };
         const int *qin = fa.q[cur];
@@ -119,7 +196,7 @@ __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
         if (threadIdx.x == 0) {
             c->small_levels++;
             c->edges_examined += ex;
-            advance_level(c, L, s_cnt, mf);
+            sssp_close(a, L, s_cnt, mf);
             c->qvalid = (1 << cur) | (1 << ((L + 1) % 3));
         }
         __syncthreads();
@@ -128,20 +205,85 @@ __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
 
 __global__ __launch_bounds__(BLK) void sssp_level_a(SsspArgs a, int view) {
     Ctl *c = a.fa.ctl;
+    const FrontierArgs &fa = a.fa;
     if (c->done) return;
     const int L = c->level;
-    if (view == 0) clear_prev_sweep(a.fa, L, blockIdx.x * BLK + threadIdx.x, gridDim.x * BLK);
-    SsspPolicy pol(a);
-    expand_bitmap<BLK>(a.fa, pol, view, L, a.fa.bm[L % 3]);
-    write_part<BLK>(a.fa, view * a.G + blockIdx.x, pol.nf, pol.mf, pol.ex);
+    const int dir = c->dir;
+    if (dir == DIR_PULL && view != 0) return;
+    if (view == 0) clear_prev_sweep(fa, L, blockIdx.x * BLK + threadIdx.x, gridDim.x * BLK);
+    if (dir == DIR_PUSH) {
+        SsspPush pol(a);
+        expand_lanes<BLK>(fa, pol, view, L, fa.bm[L % 3]);
+        write_part<BLK>(fa, view * a.G + blockIdx.x, pol.nf, pol.mf, pol.ex);
+        return;
+    }
+    // pull, vertices of in-degree <= LANE_T, one lane each in vertex order; each
+    // wave owns the two bitmap words of its 64 vertices.  Longer in-edge lists are
+    // the static tasks of sssp_level_b.
+    const uint32_t *fr = fa.bm[L % 3];
+    uint32_t *nx = fa.bm[(L + 1) % 3];
+    const int *off = fa.off[2];
+    long long nf = 0, mf = 0, ex = 0;
+    const int lane = lane_id();
+    for (int base = blockIdx.x * BLK; base < fa.n; base += gridDim.x * BLK) {
+        const int v = base + (int)threadIdx.x;
+        bool found = false;
+        if (v < fa.n) {
+            const int beg = off[v], end = off[v + 1];
+            if (end - beg <= LANE_T) {
+                ex += end - beg;
+                const long long best = pull_best(a, fr, beg, end, 1);
+                if (best < a.dist[v]) {
+                    a.dist[v] = (int)best;
+                    found = true;
+                    ++nf;
+                    mf += view_degree(fa, v);
+                }
+            }
+        }
+        const uint64_t fm = wave_ballot(found);
+        const uint32_t mine = (lane < 32) ? (uint32_t)fm : (uint32_t)(fm >> 32);
+        if ((lane & 31) == 0 && mine) nx[v >> 5] = mine;
+    }
+    write_part<BLK>(fa, blockIdx.x, nf, mf, ex);
 }
 
 __global__ __launch_bounds__(BLK) void sssp_level_b(SsspArgs a) {
     Ctl *c = a.fa.ctl;
+    const FrontierArgs &fa = a.fa;
     if (c->done) return;
-    SsspPolicy pol(a);
-    expand_chunks<BLK>(a.fa, pol, c->level);
-    write_part<BLK>(a.fa, 2 * a.G + blockIdx.x, pol.nf, pol.mf, pol.ex);
+    const int L = c->level;
+    long long nf = 0, mf = 0, ex = 0;
+    if (c->dir == DIR_PUSH) {
+        SsspPush pol(a);
+        expand_tasks<BLK>(fa, pol, L);
+        nf = pol.nf;
+        mf = pol.mf;
+        ex = pol.ex;
+    } else {
+        // static in-edge tasks: wave minimum, one atomicMin per task
+        const uint32_t *fr = fa.bm[L % 3];
+        const int lane = lane_id();
+        const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
+        for (int t = wave; t < a.npull; t += nwaves) {
+            const ChunkRec r = a.pull[t];
+            ex += (r.end - r.beg + WAVE - 1 - lane) / WAVE;
+            long long best = pull_best(a, fr, r.beg + lane, r.end, WAVE);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const long long y = __shfl_xor(best, o);
+                best = y < best ? y : best;
+            }
+            if (lane == 0 && best < a.dist[r.v]) {
+                const int old = atomicMin(&a.dist[r.v], (int)best);
+                if (best < old && set_bit_first(fa.bm[(L + 1) % 3], r.v)) {
+                    ++nf;
+                    mf += view_degree(fa, r.v);
+                }
+            }
+        }
+    }
+    write_part<BLK>(fa, 2 * a.G + blockIdx.x, nf, mf, ex);
     if (blockIdx.x == 0 && threadIdx.x == 0) c->pending = 1;
 }
 
@@ -167,12 +309,18 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
     a.dist = ws.vals;
     a.w = g->w;
     a.G = grid_blocks(std::max<int64_t>(g->n / 4, g->m / 64), BLK);
+    const int can_pull = a.fa.off[2] != nullptr && a.fa.nviews == 1 && o.engine != MG_ENGINE_GAS;
+    if (can_pull) {
+        if ((rc = ensure_pull_tasks(g, a.fa.off[2]))) return rc;
+        a.pull = ws.pull_tasks;
+        a.npull = ws.pull_ntasks;
+    }
 
     MG_HIP(hipEventRecord(g->ev0, g->stream));
     if ((rc = launch_fill(g, ws.vals, INT_MAX, g->n))) return rc;
     for (int i = 0; i < 3; ++i)
         if ((rc = launch_fill(g, ws.bm[i], 0u, ws.nwords))) return rc;
-    hipLaunchKernelGGL(sssp_seed, dim3(1), dim3(1), 0, g->stream, a, src, o.iter_num);
+    hipLaunchKernelGGL(sssp_seed, dim3(1), dim3(1), 0, g->stream, a, src, o.iter_num, can_pull, (long long)g->m);
     MG_HIP(hipGetLastError());
     auto step = [&]() -> int {
         hipLaunchKernelGGL(sssp_small, dim3(1), dim3(SMALL_BLK), 0, g->stream, a);
@@ -184,6 +332,6 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
     };
     // Bellman-Ford with non-negative weights settles within n rounds.
     const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n + 1) + 3;
-    if ((rc = drive(g, step, max_steps))) return rc;
+    if ((rc = drive(g, step, max_steps, KIND_SSSP, (uint64_t)o.directed))) return rc;
     return finish_search(g, stats, ws.vals, INT_MAX, dist_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
 }
