@@ -291,6 +291,7 @@ extern "C" int mg_bfs(mg_graph *g, int32_t src, int32_t *depth_out, const mg_opt
     if (rc) return rc;
     std::lock_guard<std::mutex> lock(g->mu);
     if (src < 0 || src >= g->n) return set_error(MG_ERR_INVALID_ARG, "mg_bfs: source %d outside [0,%d)", src, g->n);
+    if (o.engine == MG_ENGINE_GAS) return gas_bfs(g, src, depth_out, o, stats, timer);
     Workspace &ws = g->ws;
     BfsArgs a{};
     a.fa = make_frontier_args(g, o.directed);

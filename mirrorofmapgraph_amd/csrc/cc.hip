@@ -178,6 +178,12 @@ extern "C" int mg_cc(mg_graph *g, int32_t *label_out, const mg_options *opt, mg_
     int rc = begin_search(g, opt, &o);
     if (rc) return rc;
     std::lock_guard<std::mutex> lock(g->mu);
+    if (o.engine == MG_ENGINE_GAS) {
+        // the reference runs cc with directed = false (cc.cu:289)
+        mg_options og = o;
+        og.directed = (g->flags & (MG_GRAPH_SYMMETRIC | MG_GRAPH_BUILD_CSC)) ? 0 : 1;
+        return gas_cc(g, label_out, og, stats, timer);
+    }
     Workspace &ws = g->ws;
     FrontierArgs fa = make_frontier_args(g, 1);
     int *p = ws.vals;

@@ -166,10 +166,19 @@ template <class Step>
 int drive(mg_graph *g, Step step, int64_t max_steps, int kind = -1, uint64_t key = 0);
 bool graphs_enabled();
 
+// Generic GAS enactor over the ported programs (engine = MG_ENGINE_GAS).
+struct GasStateBase;
+int gas_bfs(mg_graph *g, int src, int *out, const mg_options &o, mg_stats *st, const CallTimer &t);
+int gas_sssp(mg_graph *g, int src, int *out, const mg_options &o, mg_stats *st, const CallTimer &t);
+int gas_cc(mg_graph *g, int *out, const mg_options &o, mg_stats *st, const CallTimer &t);
+int gas_pagerank(mg_graph *g, float *out, const mg_options &o, mg_stats *st, const CallTimer &t);
+void gas_release(mg_graph *g);
+
 }  // namespace mg
 
 struct mg_graph_steps {
     mg::StepGraph sg[mg::NKINDS];
+    mg::GasStateBase *gas[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 // Queues steps until the device reports `done`.  Steps run from a captured HIP graph

@@ -125,6 +125,8 @@ int begin_search(mg_graph *g, const mg_options *opt, mg_options *resolved) {
     if (resolved->iter_num < 0) return set_error(MG_ERR_INVALID_ARG, "iter_num must be >= 0");
     if (resolved->directed != 0 && resolved->directed != 1)
         return set_error(MG_ERR_INVALID_ARG, "directed must be 0 or 1");
+    if (resolved->engine != MG_ENGINE_AUTO && resolved->engine != MG_ENGINE_GAS)
+        return set_error(MG_ERR_INVALID_ARG, "unknown engine %d", resolved->engine);
     if (resolved->directed == 0 && !(g->flags & (MG_GRAPH_BUILD_CSC | MG_GRAPH_SYMMETRIC)))
         return set_error(MG_ERR_UNSUPPORTED, "directed=0 needs a graph created with MG_GRAPH_BUILD_CSC or MG_GRAPH_SYMMETRIC");
     MG_HIP(hipSetDevice(g->device));
@@ -264,6 +266,7 @@ int mg_graph_destroy(mg_graph *g) {
     (void)hipSetDevice(g->device);
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     if (g->steps) {
+        gas_release(g);
         for (auto &s : g->steps->sg)
             if (s.exec) (void)hipGraphExecDestroy(s.exec);
         delete g->steps;

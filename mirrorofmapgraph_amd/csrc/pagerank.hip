@@ -287,6 +287,12 @@ extern "C" int mg_pagerank(mg_graph *g, float *rank_out, const mg_options *opt, 
         return set_error(MG_ERR_UNSUPPORTED, "mg_pagerank: needs the in-edge view (MG_GRAPH_BUILD_CSC) or a symmetric graph");
     if (o.directed == 0 && !sym)
         return set_error(MG_ERR_UNSUPPORTED, "mg_pagerank: directed=0 is supported on symmetric graphs only");
+    if (o.engine == MG_ENGINE_GAS) {
+        if (o.pr_mode != MG_PR_GAS || o.pr_teleport != 0.15f || o.pr_active_eps != 0.01f)
+            return set_error(MG_ERR_UNSUPPORTED, "MG_ENGINE_GAS runs the reference pagerank program (pr_mode "
+                                                 "MG_PR_GAS, teleport 0.15, eps 0.01)");
+        return gas_pagerank(g, rank_out, o, stats, timer);
+    }
     if ((rc = ensure_pr_workspace(g))) return rc;
     const int *in_off = sym ? g->row_off : g->col_off;
     const int *in_adj = sym ? g->col : g->row_idx;

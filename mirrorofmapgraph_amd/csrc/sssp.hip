@@ -126,8 +126,8 @@ __global__ void sssp_seed(SsspArgs a, int src, int iter_limit, int can_pull, lon
     // pull only pays on the densest levels: every in-edge is read and the frontier
     // ones gather dist[u]; push gathers only along frontier out-edges
     c->can_bu = can_pull;
-    c->alpha = 4;
-    c->beta = 8;
+    c->alpha = 2;
+    c->beta = 2;
     c->m_unexplored = m;
 }
 
@@ -303,6 +303,7 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
                          g->min_weight);
     if (src < 0 || src >= g->n)
         return set_error(MG_ERR_INVALID_ARG, "mg_sssp: source %d outside [0,%d)", src, g->n);
+    if (o.engine == MG_ENGINE_GAS) return gas_sssp(g, src, dist_out, o, stats, timer);
     Workspace &ws = g->ws;
     SsspArgs a{};
     a.fa = make_frontier_args(g, o.directed);
