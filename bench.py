@@ -45,6 +45,21 @@ def bytes_cc(m, n):
     return 8 * m + 20 * n
 
 
+COUNTERS = ROOT / "profiles" / "rmat22_sssp_cc_bfs_counters.md"
+
+
+def measured_traffic(kind: str):
+    """HBM bytes of one search as measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+    `bench.py --pmc-pass` (scaled by a copy-kernel calibration); None when absent."""
+    try:
+        for line in COUNTERS.read_text().splitlines():
+            if line.startswith("machine-readable: "):
+                return int(json.loads(line.split(": ", 1)[1])[kind]["traffic_bytes"])
+    except (OSError, KeyError, ValueError):
+        pass
+    return None
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -64,6 +79,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--pmc-pass", action="store_true",
+                    help="for counter runs: one warm-up and one measured SSSP, CC and BFS search, then a "
+                         "256 MiB device-to-device copy (the calibration kernel); prints per-search stats")
     a = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -82,6 +100,29 @@ def main() -> int:
     log(f"graph s{a.scale}: n={g.n} m={g.m} built in {time.perf_counter() - t0:.1f}s")
     dg = mg.Graph(g, symmetric=True)
     src = 0
+
+    if a.pmc_pass:
+        import ctypes as C
+
+        out = {}
+        for name, fn in [("sssp", lambda: dg.sssp(src, stats=True, copy=False)),
+                         ("cc", lambda: dg.cc(stats=True, copy=False)),
+                         ("bfs", lambda: dg.bfs(src, stats=True, copy=False))]:
+            fn()
+            _, st = fn()
+            out[name] = st.as_dict()
+        hip = C.CDLL("libamdhip64.so")  # the runtime the engine already loaded
+        nbytes = 256 << 20
+        p, q = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(q), C.c_size_t(nbytes)) == 0
+        assert hip.hipMemcpy(q, p, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+        assert hip.hipDeviceSynchronize() == 0
+        hip.hipFree(p)
+        hip.hipFree(q)
+        out["calibration_copy_bytes"] = nbytes
+        print(json.dumps(out))
+        dg.close()
+        return 0
 
     def step():
         _, s1 = dg.sssp(src, stats=True, copy=False)
@@ -126,9 +167,11 @@ def main() -> int:
                | {"levels": sb.iterations, "bottom_up_levels": sb.bottom_up_levels},
     }
     ss = parts["sssp"]
+    traffic = measured_traffic("sssp")
     roofline = {"bound": "hbm", "achieved": ss["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ss["hbm_frac"], "traffic": None,
-                "kernel": "sssp search (sssp_small + sssp_level_a + sssp_level_b launches of one search)"}
+                "frac": ss["hbm_frac"], "traffic": traffic,
+                "kernel": "sssp search (sssp_small + sssp_level_a + sssp_level_b launches of one search)",
+                "traffic_source": str(COUNTERS.relative_to(ROOT)) if traffic else None}
 
     cpu = None
     if not a.no_cpu:
