@@ -115,7 +115,9 @@ typedef struct mg_options {
     double  max_queue_sizing; /* reference "max_queue_sizing" (queue = sizing * n)         */
     int32_t alpha;          /* BFS top-down -> bottom-up switch (frontier edges > unexplored/alpha) */
     int32_t beta;           /* BFS bottom-up -> top-down switch (frontier < n/beta)        */
-    int32_t reserved[8];
+    int32_t batch_alpha;    /* mg_bfs_batch push -> pull switch (frontier edges > m/batch_alpha); 0 = 4 */
+    int32_t batch_beta;     /* mg_bfs_batch pull -> push switch (frontier edges < m/batch_beta); 0 = 16  */
+    int32_t reserved[6];
 } mg_options;
 
 /* Fills *o with the reference defaults. */
@@ -155,6 +157,19 @@ int mg_cc(mg_graph *g, int32_t *label_out, const mg_options *opt, mg_stats *stat
  * iterations or until sum |r_new - r| < pr_tol * n.  Needs MG_GRAPH_BUILD_CSC or
  * MG_GRAPH_SYMMETRIC.  Sums run in a fixed order, so results are deterministic. */
 int mg_pagerank(mg_graph *g, float *rank_out, const mg_options *opt, mg_stats *stats);
+
+/* ---------------------------------------------------------------- batched searches */
+#define MG_BATCH_MAX 256  /* sources per batched call */
+
+/* BFS from k <= MG_BATCH_MAX sources in one device-driven search: depth_out is
+ * k x n, row s holding the depths from srcs[s] (the reference runs sources one
+ * after another, Algorithms/BFS/bfs.cu:402-417).  Row s equals mg_bfs(g, srcs[s]).
+ * mg_stats.edges_traversed and vertices_reached sum the per-source counts.
+ * directed=0 needs a graph created with MG_GRAPH_SYMMETRIC.  The dense-level pull
+ * direction needs in-edges (symmetric graph or MG_GRAPH_BUILD_CSC); its switch is
+ * set by mg_options.batch_alpha / batch_beta. */
+int mg_bfs_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t *depth_out,
+                 const mg_options *opt, mg_stats *stats);
 
 /* ---------------------------------------------------------------- host graph tools */
 /* These run on the host only and need no device. */

@@ -24,6 +24,7 @@ MG_ERR_IO = 5
 MG_ERR_FORMAT = 6
 MG_ERR_UNSUPPORTED = 7
 MG_ERR_NO_DEVICE = 8
+MG_BATCH_MAX = 256
 
 MG_GRAPH_WEIGHTED = 0x1
 MG_GRAPH_BUILD_CSC = 0x2
@@ -45,7 +46,7 @@ INT32_MAX = 2**31 - 1
 EXPORTED = [
     "mg_last_error", "mg_version", "mg_device_init", "mg_device_count", "mg_device_synchronize",
     "mg_graph_create", "mg_graph_destroy", "mg_graph_info", "mg_graph_device_bytes", "mg_graph_get_csc",
-    "mg_options_default", "mg_bfs", "mg_sssp", "mg_cc", "mg_pagerank",
+    "mg_options_default", "mg_bfs", "mg_sssp", "mg_cc", "mg_pagerank", "mg_bfs_batch",
     "mg_rmat_num_edges", "mg_gen_rmat_edges", "mg_grid_num_edges", "mg_gen_grid_edges",
     "mg_csr_from_edges", "mg_edge_weights_hash", "mg_csc_from_csr", "mg_pick_sources",
     "mg_mtx_read_header", "mg_mtx_read",
@@ -64,7 +65,8 @@ class Options(C.Structure):
         ("engine", C.c_int32), ("verbose", C.c_int32), ("out_on_device", C.c_int32),
         ("pr_mode", C.c_int32), ("pr_teleport", C.c_float), ("pr_tol", C.c_float),
         ("pr_active_eps", C.c_float), ("max_queue_sizing", C.c_double),
-        ("alpha", C.c_int32), ("beta", C.c_int32), ("reserved", C.c_int32 * 8),
+        ("alpha", C.c_int32), ("beta", C.c_int32), ("batch_alpha", C.c_int32), ("batch_beta", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -130,6 +132,7 @@ def lib() -> C.CDLL:
         "mg_sssp": [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
         "mg_cc": [C.c_void_p, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
         "mg_pagerank": [C.c_void_p, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
+        "mg_bfs_batch": [C.c_void_p, I32P, C.c_int32, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
     }
     for name, argtypes in search.items():
         fn = getattr(L, name, None)  # a missing export is reported by the export test
@@ -360,3 +363,9 @@ class Graph:
 
     def pagerank(self, opt: Options | None = None, stats: bool = False, copy: bool = True):
         return self._run(lib().mg_pagerank, opt=opt, out=self._out(copy, np.float32), stats=stats)
+
+    def bfs_batch(self, srcs, opt: Options | None = None, stats: bool = False, copy: bool = True):
+        """Depths from every source in srcs (at most MG_BATCH_MAX) as a len(srcs) x n array."""
+        s = np.ascontiguousarray(srcs, dtype=np.int32)
+        out = np.empty((len(s), self.n), np.int32) if copy else None
+        return self._run(lib().mg_bfs_batch, _p(s), len(s), opt=opt, out=out, stats=stats)

@@ -173,12 +173,23 @@ int gas_sssp(mg_graph *g, int src, int *out, const mg_options &o, mg_stats *st, 
 int gas_cc(mg_graph *g, int *out, const mg_options &o, mg_stats *st, const CallTimer &t);
 int gas_pagerank(mg_graph *g, float *out, const mg_options &o, mg_stats *st, const CallTimer &t);
 void gas_release(mg_graph *g);
+void batch_release(mg_graph *g);
 
 }  // namespace mg
+
+// Device state of batched searches: three 256-bit masks per vertex and the
+// MG_BATCH_MAX x n result, allocated on the first batched call.
+struct BatchState {
+    void *V = nullptr, *F = nullptr, *N = nullptr;
+    int32_t *out = nullptr, *srcs = nullptr;
+    int32_t n = 0;
+    size_t out_cap = 0;
+};
 
 struct mg_graph_steps {
     mg::StepGraph sg[mg::NKINDS];
     mg::GasStateBase *gas[4] = {nullptr, nullptr, nullptr, nullptr};
+    BatchState batch;
 };
 
 // Queues steps until the device reports `done`.  Steps run from a captured HIP graph

@@ -270,7 +270,7 @@ __device__ void expand_lanes(const FrontierArgs &fa, P &pol, int view, int L, co
     __shared__ int s_q[WPR * 32];
     __shared__ int s_wave2[BLOCK / WAVE];
     __shared__ int s_tbase;
-    const int *off = fa.off[view], *adj = fa.adj[view];
+    const int *off = fa.off[view];
     for (int w0 = blockIdx.x * WPR; w0 < fa.nwords; w0 += gridDim.x * WPR) {
         const int w = w0 + (int)threadIdx.x;
         uint32_t word = (threadIdx.x < WPR && w < fa.nwords) ? fr[w] : 0u;

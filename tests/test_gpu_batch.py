@@ -1,0 +1,130 @@
+"""Batched multi-source BFS on the GPU: row s of mg_bfs_batch must equal a
+single-source search from srcs[s], bit for bit, and the stats must be the sums of
+the per-source counts.  Small cases are checked against the CPU oracle, the
+R-MAT scale-20 case (256 sources, the batched workload's size) against mg_bfs on
+a sample of rows plus size-independent properties of every row."""
+import numpy as np
+import pytest
+
+from conftest import load_mtx, random_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_rows(oracle, g, srcs, directed=1):
+    return np.stack([oracle.bfs(g.row_offsets, g.col_idx, int(s), directed=directed) for s in srcs])
+
+
+def _check_stats(oracle, g, rows, st):
+    assert st.edges_traversed == sum(oracle.teps_edges(g.row_offsets, r) for r in rows)
+    assert st.vertices_reached == int((rows >= 0).sum())
+
+
+@pytest.mark.parametrize("k", [1, 31, 64, 200, 256])
+def test_batch_rmat_symmetric(mg, oracle, device, k):
+    g = mg.rmat_graph(13, 16, seed=7)
+    srcs = mg.pick_sources(g, k, seed=k)
+    with mg.Graph(g, symmetric=True) as dg:
+        got, st = dg.bfs_batch(srcs, stats=True)
+    exp = _oracle_rows(oracle, g, srcs)
+    bad = np.argwhere(got != exp)
+    assert bad.size == 0, f"first mismatches (row, vertex): {bad[:5].tolist()}"
+    _check_stats(oracle, g, exp, st)
+
+
+@pytest.mark.parametrize("build_csc", [True, False])
+def test_batch_directed(mg, oracle, device, build_csc):
+    # directed R-MAT: pull levels read the CSC; without it every level pushes
+    g = mg.rmat_graph(12, 8, seed=3, symmetric=False)
+    srcs = mg.pick_sources(g, 96, seed=5)
+    with mg.Graph(g, build_csc=build_csc) as dg:
+        got, st = dg.bfs_batch(srcs, stats=True)
+    exp = _oracle_rows(oracle, g, srcs)
+    assert np.array_equal(got, exp)
+    _check_stats(oracle, g, exp, st)
+
+
+def test_batch_grid_high_diameter(mg, oracle, device):
+    g = mg.grid_graph(96, seed=11)
+    rng = np.random.default_rng(0)
+    srcs = rng.integers(0, g.n, 77).astype(np.int32)
+    srcs[5] = srcs[6]  # a repeated source gets identical rows
+    with mg.Graph(g, symmetric=True) as dg:
+        got, st = dg.bfs_batch(srcs, stats=True)
+    exp = _oracle_rows(oracle, g, srcs)
+    assert np.array_equal(got, exp)
+    assert np.array_equal(got[5], got[6])
+    _check_stats(oracle, g, exp, st)
+
+
+def test_batch_fixtures_and_isolated_sources(mg, oracle, device):
+    for name in ["small.mtx", "trivialGraph.mtx", "chesapeake.mtx"]:
+        g = load_mtx(mg, name)
+        srcs = np.arange(g.n, dtype=np.int32)[:256]
+        with mg.Graph(g, build_csc=True) as dg:
+            got = dg.bfs_batch(srcs)
+        assert np.array_equal(got, _oracle_rows(oracle, g, srcs)), name
+    # vertices without edges: the row holds only the source
+    g = random_graph(mg, 3000, 2000, 4)
+    iso = np.nonzero(g.degrees() == 0)[0][:40].astype(np.int32)
+    srcs = np.concatenate([iso, mg.pick_sources(g, 40, seed=1)])
+    with mg.Graph(g, symmetric=True) as dg:
+        got = dg.bfs_batch(srcs)
+    assert np.array_equal(got, _oracle_rows(oracle, g, srcs))
+
+
+def test_batch_matches_single_source_and_iter_num(mg, device):
+    g = mg.rmat_graph(14, 16, seed=9)
+    srcs = mg.pick_sources(g, 130, seed=2)
+    with mg.Graph(g, symmetric=True) as dg:
+        for it in (0, 1, 3):
+            opt = mg.options(iter_num=it)
+            got = dg.bfs_batch(srcs, opt)
+            for s in (0, 64, 129):
+                assert np.array_equal(got[s], dg.bfs(int(srcs[s]), opt)), (it, s)
+        # forced push-only and forced early pull give the same rows
+        base = dg.bfs_batch(srcs)
+        assert np.array_equal(base, dg.bfs_batch(srcs, mg.options(batch_alpha=1 << 30, batch_beta=1 << 30)))
+        assert np.array_equal(base, dg.bfs_batch(srcs, mg.options(batch_alpha=1 << 20, batch_beta=1)))
+        # a second, smaller call on the same graph reuses the state
+        assert np.array_equal(base[:3], dg.bfs_batch(srcs[:3]))
+
+
+def test_batch_rejects_bad_arguments(mg, device):
+    g = mg.rmat_graph(10, 8, seed=1)
+    with mg.Graph(g, symmetric=True) as dg:
+        for srcs in ([], np.zeros(mg.MG_BATCH_MAX + 1, np.int32), [0, g.n], [-1]):
+            with pytest.raises(mg.MgError):
+                dg.bfs_batch(srcs)
+    gd = mg.rmat_graph(10, 8, seed=1, symmetric=False)
+    with mg.Graph(gd, build_csc=True) as dg:
+        with pytest.raises(mg.MgError):
+            dg.bfs_batch([0], mg.options(directed=0))
+
+
+def test_batch_rmat20_256_sources(mg, device):
+    g = mg.rmat_graph(20, 16, seed=42)
+    srcs = mg.pick_sources(g, 256, seed=42)
+    with mg.Graph(g, symmetric=True) as dg:
+        got, st = dg.bfs_batch(srcs, stats=True)
+        for s in (0, 1, 77, 128, 255):
+            assert np.array_equal(got[s], dg.bfs(int(srcs[s]))), s
+    # Sampled rows are exact BFS distances: the source is at 0, neighbours differ by
+    # at most 1 (symmetric graph), components are reached whole, and every vertex at
+    # depth d > 0 has a neighbour at d - 1.  Together these fix the distances.
+    ro, ci = g.row_offsets, g.col_idx
+    src_rows = np.repeat(np.arange(g.n, dtype=np.int64), np.diff(ro))
+    nz = np.diff(ro) > 0
+    assert np.all(got[np.arange(256), srcs] == 0)
+    for s in range(0, 256, 16):
+        d = got[s].astype(np.int64)
+        du, dv = d[src_rows], d[ci]
+        both = (du >= 0) & (dv >= 0)
+        assert np.all(np.abs(du[both] - dv[both]) <= 1)
+        assert np.all((du >= 0) == (dv >= 0))
+        mn = np.full(g.n, 1 << 40)
+        mn[nz] = np.minimum.reduceat(np.where(dv >= 0, dv, 1 << 40), ro[:-1][nz])
+        assert np.all(mn[d > 0] == d[d > 0] - 1)
+    assert st.vertices_reached == int((got >= 0).sum())
+    deg = np.diff(ro).astype(np.int64)
+    assert st.edges_traversed == sum(int(deg[row >= 0].sum()) for row in got)
