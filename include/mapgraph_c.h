@@ -171,6 +171,16 @@ int mg_pagerank(mg_graph *g, float *rank_out, const mg_options *opt, mg_stats *s
 int mg_bfs_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t *depth_out,
                  const mg_options *opt, mg_stats *stats);
 
+/* SSSP from k <= MG_BATCH_MAX sources in one device-driven search: dist_out is
+ * k x n, row s equal to mg_sssp(g, srcs[s]) (unreached = INT32_MAX; the
+ * reference runs sources one after another, Algorithms/SSSP/sssp.cu:310-428).
+ * Needs MG_GRAPH_WEIGHTED with non-negative weights; directed=0 needs
+ * MG_GRAPH_SYMMETRIC.  iter_num bounds the relaxation rounds (rows equal the
+ * single-source result when the search converges within it).  mg_stats
+ * edges_traversed / vertices_reached sum the per-source counts. */
+int mg_sssp_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t *dist_out,
+                  const mg_options *opt, mg_stats *stats);
+
 /* ---------------------------------------------------------------- host graph tools */
 /* These run on the host only and need no device. */
 

@@ -46,7 +46,7 @@ INT32_MAX = 2**31 - 1
 EXPORTED = [
     "mg_last_error", "mg_version", "mg_device_init", "mg_device_count", "mg_device_synchronize",
     "mg_graph_create", "mg_graph_destroy", "mg_graph_info", "mg_graph_device_bytes", "mg_graph_get_csc",
-    "mg_options_default", "mg_bfs", "mg_sssp", "mg_cc", "mg_pagerank", "mg_bfs_batch",
+    "mg_options_default", "mg_bfs", "mg_sssp", "mg_cc", "mg_pagerank", "mg_bfs_batch", "mg_sssp_batch",
     "mg_rmat_num_edges", "mg_gen_rmat_edges", "mg_grid_num_edges", "mg_gen_grid_edges",
     "mg_csr_from_edges", "mg_edge_weights_hash", "mg_csc_from_csr", "mg_pick_sources",
     "mg_mtx_read_header", "mg_mtx_read",
@@ -133,6 +133,7 @@ def lib() -> C.CDLL:
         "mg_cc": [C.c_void_p, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
         "mg_pagerank": [C.c_void_p, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
         "mg_bfs_batch": [C.c_void_p, I32P, C.c_int32, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
+        "mg_sssp_batch": [C.c_void_p, I32P, C.c_int32, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
     }
     for name, argtypes in search.items():
         fn = getattr(L, name, None)  # a missing export is reported by the export test
@@ -369,3 +370,9 @@ class Graph:
         s = np.ascontiguousarray(srcs, dtype=np.int32)
         out = np.empty((len(s), self.n), np.int32) if copy else None
         return self._run(lib().mg_bfs_batch, _p(s), len(s), opt=opt, out=out, stats=stats)
+
+    def sssp_batch(self, srcs, opt: Options | None = None, stats: bool = False, copy: bool = True):
+        """Distances from every source in srcs (at most MG_BATCH_MAX) as a len(srcs) x n array."""
+        s = np.ascontiguousarray(srcs, dtype=np.int32)
+        out = np.empty((len(s), self.n), np.int32) if copy else None
+        return self._run(lib().mg_sssp_batch, _p(s), len(s), opt=opt, out=out, stats=stats)

@@ -16,50 +16,9 @@
 // Depths are unique per source, so row s equals a single-source search from
 // srcs[s].  Level close, direction choice and termination run on the device in the
 // next step's single-workgroup kernel; steps are replayed from a captured HIP graph.
-#include "frontier.h"
+#include "batch.h"
 
 namespace mg {
-
-struct M8 {
-    uint4 a, b;
-};
-
-__device__ __forceinline__ M8 ld8(const uint4 *p, int v) { return M8{p[2 * v], p[2 * v + 1]}; }
-__device__ __forceinline__ void st8(uint4 *p, int v, const M8 &m) {
-    p[2 * v] = m.a;
-    p[2 * v + 1] = m.b;
-}
-__device__ __forceinline__ M8 or8(const M8 &x, const M8 &y) {
-    return M8{make_uint4(x.a.x | y.a.x, x.a.y | y.a.y, x.a.z | y.a.z, x.a.w | y.a.w),
-              make_uint4(x.b.x | y.b.x, x.b.y | y.b.y, x.b.z | y.b.z, x.b.w | y.b.w)};
-}
-__device__ __forceinline__ M8 andnot8(const M8 &x, const M8 &y) {
-    return M8{make_uint4(x.a.x & ~y.a.x, x.a.y & ~y.a.y, x.a.z & ~y.a.z, x.a.w & ~y.a.w),
-              make_uint4(x.b.x & ~y.b.x, x.b.y & ~y.b.y, x.b.z & ~y.b.z, x.b.w & ~y.b.w)};
-}
-__device__ __forceinline__ bool any8(const M8 &x) {
-    return (x.a.x | x.a.y | x.a.z | x.a.w | x.b.x | x.b.y | x.b.z | x.b.w) != 0;
-}
-__device__ __forceinline__ uint32_t word8(const M8 &x, int w) {
-    switch (w) {
-        case 0: return x.a.x;
-        case 1: return x.a.y;
-        case 2: return x.a.z;
-        case 3: return x.a.w;
-        case 4: return x.b.x;
-        case 5: return x.b.y;
-        case 6: return x.b.z;
-        default: return x.b.w;
-    }
-}
-__device__ __forceinline__ int popc8(const M8 &x) {
-    return __popc(x.a.x) + __popc(x.a.y) + __popc(x.a.z) + __popc(x.a.w) + __popc(x.b.x) + __popc(x.b.y) +
-           __popc(x.b.z) + __popc(x.b.w);
-}
-__device__ __forceinline__ M8 shfl_xor8(const M8 &x, int o) {
-    return M8{make_uint4(__shfl_xor(x.a.x, o), __shfl_xor(x.a.y, o), __shfl_xor(x.a.z, o), __shfl_xor(x.a.w, o)),
-              make_uint4(__shfl_xor(x.b.x, o), __shfl_xor(x.b.y, o), __shfl_xor(x.b.z, o), __shfl_xor(x.b.w, o))};
-}
 
 struct BatchArgs {
     Ctl *ctl;
@@ -75,33 +34,6 @@ struct BatchArgs {
     M8 full;                     // bits of the k sources
     int n, nwords, k, G;
 };
-
-// Calls fn(v) for the set bits of a bitmap, block-uniformly: each round a
-// workgroup compacts BLK/2 words into LDS and hands them out BLK at a time (fn is
-// called by every thread, with v = -1 for padding).
-template <class Fn>
-__device__ void for_each_bit(const uint32_t *bm, int nwords, Fn fn) {
-    constexpr int WPR = BLK / 2;
-    __shared__ int s_q[WPR * 32];
-    __shared__ int s_w[BLK / WAVE];
-    for (int w0 = blockIdx.x * WPR; w0 < nwords; w0 += gridDim.x * WPR) {
-        const int w = w0 + (int)threadIdx.x;
-        uint32_t word = (threadIdx.x < WPR && w < nwords) ? bm[w] : 0u;
-        int total;
-        int k = block_exclusive_scan<BLK>(__popc(word), s_w, &total);
-        while (word) {
-            const int b = __ffs(word) - 1;
-            s_q[k++] = w * 32 + b;
-            word &= word - 1;
-        }
-        __syncthreads();
-        for (int t0 = 0; t0 < total; t0 += BLK) {
-            const int i = t0 + (int)threadIdx.x;
-            fn(i < total ? s_q[i] : -1);
-        }
-        __syncthreads();
-    }
-}
 
 // push: v's frontier bits that u has not seen yet go to N[u]
 __device__ __forceinline__ void push_edge(const BatchArgs &a, const M8 &fv, int u, uint32_t *act_next) {
@@ -367,7 +299,52 @@ void batch_release(mg_graph *g) {
     free_dev(bs.N);
     free_dev(bs.out);
     free_dev(bs.srcs);
+    free_dev(bs.D);
+    free_dev(bs.tasks);
     bs = BatchState{};
+}
+
+// Allocates the batch state of graph g once, at the maximum batch size, so that
+// captured step graphs keep valid pointers for the graph's lifetime.
+int ensure_batch(mg_graph *g, bool sssp) {
+    BatchState &bs = g->steps->batch;
+    const size_t n = (size_t)g->n;
+    hipError_t e = hipSuccess;
+    if (!bs.V) {
+        e = hipMalloc(&bs.V, 32 * n + 32);
+        if (e == hipSuccess) e = hipMalloc(&bs.F, 32 * n + 32);
+        if (e == hipSuccess) e = hipMalloc(&bs.N, 32 * n + 32);
+        if (e == hipSuccess) e = hipMalloc((void **)&bs.out, 4 * (size_t)MG_BATCH_MAX * n + 4);
+        if (e == hipSuccess) e = hipMalloc((void **)&bs.srcs, 4 * MG_BATCH_MAX);
+        if (e != hipSuccess) {
+            batch_release(g);
+            return set_error(MG_ERR_OOM, "batch state allocation failed: %s", hipGetErrorString(e));
+        }
+        bs.n = g->n;
+        bs.out_cap = (size_t)MG_BATCH_MAX * n;
+    }
+    if (sssp && !bs.D) {
+        bs.task_cap = g->m / SB_TASK + (int64_t)n + 64;
+        e = hipMalloc((void **)&bs.D, 4 * (size_t)MG_BATCH_MAX * n + 64);
+        if (e == hipSuccess) e = hipMalloc((void **)&bs.tasks, sizeof(ChunkRec) * (size_t)bs.task_cap);
+        if (e != hipSuccess) {
+            free_dev(bs.D);
+            free_dev(bs.tasks);
+            bs.D = nullptr;
+            bs.tasks = nullptr;
+            return set_error(MG_ERR_OOM, "batched SSSP state allocation failed: %s", hipGetErrorString(e));
+        }
+    }
+    return MG_OK;
+}
+
+int check_batch_sources(const char *fn, const mg_graph *g, const int32_t *srcs, int32_t k) {
+    if (k < 1 || k > MG_BATCH_MAX || !srcs)
+        return set_error(MG_ERR_INVALID_ARG, "%s: k must be in [1, %d] with a source array", fn, MG_BATCH_MAX);
+    for (int i = 0; i < k; ++i)
+        if (srcs[i] < 0 || srcs[i] >= g->n)
+            return set_error(MG_ERR_INVALID_ARG, "%s: source %d = %d outside [0,%d)", fn, i, srcs[i], g->n);
+    return MG_OK;
 }
 
 int batch_alpha(const mg_options &o) { return o.batch_alpha > 0 ? o.batch_alpha : 4; }
@@ -384,29 +361,13 @@ extern "C" int mg_bfs_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t
     int rc = begin_search(g, opt, &o);
     if (rc) return rc;
     std::lock_guard<std::mutex> lock(g->mu);
-    if (k < 1 || k > MG_BATCH_MAX || !srcs)
-        return set_error(MG_ERR_INVALID_ARG, "mg_bfs_batch: k must be in [1, %d] with a source array", MG_BATCH_MAX);
-    for (int i = 0; i < k; ++i)
-        if (srcs[i] < 0 || srcs[i] >= g->n)
-            return set_error(MG_ERR_INVALID_ARG, "mg_bfs_batch: source %d = %d outside [0,%d)", i, srcs[i], g->n);
+    if ((rc = check_batch_sources("mg_bfs_batch", g, srcs, k))) return rc;
     if (o.directed == 0 && !(g->flags & MG_GRAPH_SYMMETRIC))
         return set_error(MG_ERR_UNSUPPORTED, "mg_bfs_batch: directed=0 needs a symmetric graph");
+    if ((rc = ensure_batch(g, false))) return rc;
     Workspace &ws = g->ws;
     const int n = g->n;
-    // per-graph batch state (masks and the k x n result), grown on demand
-    auto &bs = g->steps->batch;
-    const size_t need_out = (size_t)k * n;
-    if (!bs.V || bs.n < n || bs.out_cap < need_out) {
-        batch_release(g);
-        hipError_t e = hipMalloc((void **)&bs.V, 32 * (size_t)n + 32);
-        if (e == hipSuccess) e = hipMalloc((void **)&bs.F, 32 * (size_t)n + 32);
-        if (e == hipSuccess) e = hipMalloc((void **)&bs.N, 32 * (size_t)n + 32);
-        if (e == hipSuccess) e = hipMalloc((void **)&bs.out, 4 * (size_t)MG_BATCH_MAX * n + 4);
-        if (e == hipSuccess) e = hipMalloc((void **)&bs.srcs, 4 * MG_BATCH_MAX);
-        if (e != hipSuccess) return set_error(MG_ERR_OOM, "mg_bfs_batch: allocation failed: %s", hipGetErrorString(e));
-        bs.n = n;
-        bs.out_cap = (size_t)MG_BATCH_MAX * n;
-    }
+    BatchState &bs = g->steps->batch;
     BatchArgs a{};
     a.ctl = ws.ctl;
     a.off = g->row_off;

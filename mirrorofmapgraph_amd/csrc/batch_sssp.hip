@@ -1,0 +1,345 @@
+// Batched multi-source SSSP: up to 256 sources relax together over shared edge reads.
+//
+// The reference runs one search per source (Algorithms/SSSP/sssp.cu:310-428).  Here the
+// distances are vertex-major, D[v * KS + s] (KS = k rounded up to 16, one 64-byte
+// line per 16 sources), so one wave relaxes one edge for all sources at once: lane l
+// owns sources 4l..4l+3 and moves 16 contiguous bytes of D[u].  Each vertex carries a
+// 256-bit change mask F (sources whose distance dropped since v was last expanded)
+// and N (drops found this iteration).  One iteration is
+//   * tasks: every active vertex v (listed in the bitmap A[L%2]) is cut into
+//     SB_TASK-edge tasks;
+//   * relax: one wave per task; for each edge (v, u, w) and each source s in F[v]:
+//     nd = D[v][s] + w; if nd < D[u][s]: atomicMin(D[u][s], nd), set bit s of N[u]
+//     and mark u in A[(L+1)%2];
+//   * finalize: F[v] = 0 for the old frontier, then F[u] = N[u], N[u] = 0.
+// This is the frontier Bellman-Ford of the single-source path run per source; every
+// source converges to its unique shortest distances, so row s equals mg_sssp from
+// srcs[s].  A distance read while another wave lowers it is only stale-high: the
+// later drop marks the vertex again.  At convergence D is transposed into the
+// k x n result.
+#include "batch.h"
+
+#include <climits>
+
+namespace mg {
+
+struct SBatchArgs {
+    Ctl *ctl;
+    const int *off, *adj, *w;
+    int *D;                // n x KS
+    uint4 *F, *N;          // change masks, 2 uint4 per vertex
+    uint32_t *act[2];
+    ChunkRec *tasks;
+    int64_t task_cap;
+    LevelPart *part;       // [0, G): finalize counts, [G, 2G): relax counts
+    int *out;              // k x n
+    int n, nwords, k, KS, G;
+};
+
+__global__ void sb_init(SBatchArgs a, int iter_limit) {
+    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, nt = (int64_t)gridDim.x * blockDim.x;
+    int4 *D4 = reinterpret_cast<int4 *>(a.D);
+    const int64_t nd4 = (int64_t)a.n * a.KS / 4;
+    for (int64_t i = tid; i < nd4; i += nt) D4[i] = make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
+    for (int64_t i = tid; i < 2 * (int64_t)a.n; i += nt) {
+        a.F[i] = make_uint4(0, 0, 0, 0);
+        a.N[i] = make_uint4(0, 0, 0, 0);
+    }
+    for (int64_t i = tid; i < a.nwords; i += nt) {
+        a.act[0][i] = 0;
+        a.act[1][i] = 0;
+    }
+    if (tid == 0) ctl_reset(a.ctl, a.n, iter_limit, a.k);
+}
+
+__global__ void sb_seed(SBatchArgs a, const int *srcs) {
+    const int s = threadIdx.x;
+    if (s >= a.k) return;
+    const int v = srcs[s];
+    a.D[(int64_t)v * a.KS + s] = 0;
+    uint32_t *fv = reinterpret_cast<uint32_t *>(a.F + 2 * v);
+    atomicOr(&fv[s >> 5], 1u << (s & 31));
+    atomicOr(&a.act[0][v >> 5], 1u << (v & 31));
+}
+
+// One workgroup: closes the previous iteration.
+__global__ __launch_bounds__(SMALL_BLK) void sb_close(SBatchArgs a) {
+    Ctl *c = a.ctl;
+    if (!c->pending) return;
+    long long nf = 0, ex = 0;
+    for (int i = threadIdx.x; i < a.G; i += SMALL_BLK) {
+        nf += a.part[i].nf;
+        ex += a.part[a.G + i].ex;
+    }
+    nf = block_sum<SMALL_BLK>(nf);
+    ex = block_sum<SMALL_BLK>(ex);
+    if (threadIdx.x == 0) {
+        const int L = c->level;
+        c->edges_examined += ex;
+        c->pending = 0;
+        c->nchunks = 0;
+        c->level = L + 1;
+        c->nf_prev = (int)nf;
+        if (nf == 0 || L + 1 >= c->iter_limit) c->done = 1;
+    }
+}
+
+__global__ __launch_bounds__(BLK) void sb_tasks(SBatchArgs a) {
+    Ctl *c = a.ctl;
+    if (c->done) return;
+    const int L = c->level;
+    __shared__ int s_w[BLK / WAVE];
+    __shared__ int s_tb;
+    for_each_bit(a.act[L & 1], a.nwords, [&](int v) {
+        int beg = 0, deg = 0;
+        if (v >= 0) {
+            beg = a.off[v];
+            deg = a.off[v + 1] - beg;
+        }
+        const int nt = (deg + SB_TASK - 1) / SB_TASK;
+        int tot;
+        const int tex = block_exclusive_scan<BLK>(nt, s_w, &tot);
+        if (tot > 0) {
+            if (threadIdx.x == 0) s_tb = atomicAdd(&c->nchunks, tot);
+            __syncthreads();
+            for (int t = 0; t < nt; ++t) {
+                const int64_t slot = (int64_t)s_tb + tex + t;
+                if (slot < a.task_cap)
+                    a.tasks[slot] = ChunkRec{v, beg + t * SB_TASK, min(beg + (t + 1) * SB_TASK, beg + deg), 0};
+                else
+                    atomicOr(&c->changed, 1 << 30);
+            }
+            __syncthreads();
+        }
+    });
+}
+
+// Relaxes one 4-source slice: returns the nibble of sources whose distance dropped.
+__device__ __forceinline__ uint32_t relax4(int *du_ptr, const int4 &dv, const int4 &du, int w, uint32_t nib) {
+    uint32_t imp = 0;
+    const long long c0 = (long long)dv.x + w, c1 = (long long)dv.y + w, c2 = (long long)dv.z + w,
+                    c3 = (long long)dv.w + w;
+    if ((nib & 1) && c0 < du.x) {
+        atomicMin(du_ptr + 0, (int)c0);
+        imp |= 1;
+    }
+    if ((nib & 2) && c1 < du.y) {
+        atomicMin(du_ptr + 1, (int)c1);
+        imp |= 2;
+    }
+    if ((nib & 4) && c2 < du.z) {
+        atomicMin(du_ptr + 2, (int)c2);
+        imp |= 4;
+    }
+    if ((nib & 8) && c3 < du.w) {
+        atomicMin(du_ptr + 3, (int)c3);
+        imp |= 8;
+    }
+    return imp;
+}
+
+// One wave per task; lanes are sources.
+__global__ __launch_bounds__(BLK) void sb_relax(SBatchArgs a) {
+    Ctl *c = a.ctl;
+    if (c->done) return;
+    const int L = c->level;
+    uint32_t *act_next = a.act[(L + 1) & 1];
+    const int lane = lane_id();
+    const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
+    const int nt = (int)min((int64_t)c->nchunks, a.task_cap);
+    const bool has_slice = 4 * lane < a.KS;
+    long long ex = 0;
+    for (int t = wave; t < nt; t += nwaves) {
+        const ChunkRec r = a.tasks[t];
+        const uint32_t fw = reinterpret_cast<const uint32_t *>(a.F + 2 * r.v)[lane >> 3];
+        const uint32_t nib = has_slice ? (fw >> (4 * (lane & 7))) & 0xFu : 0u;
+        if (!__any(nib != 0)) continue;
+        int4 dv = make_int4(0, 0, 0, 0);
+        if (nib) dv = reinterpret_cast<const int4 *>(a.D + (int64_t)r.v * a.KS)[lane];
+        ex += (long long)__popc(nib) * (r.end - r.beg);
+        for (int e = r.beg; e < r.end; e += 4) {
+            int u[4], wt[4];
+            int4 du[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                u[j] = e + j < r.end ? a.adj[e + j] : -1;
+                wt[j] = u[j] >= 0 ? a.w[e + j] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                du[j] = (u[j] >= 0 && nib) ? reinterpret_cast<const int4 *>(a.D + (int64_t)u[j] * a.KS)[lane]
+                                           : make_int4(INT_MIN, INT_MIN, INT_MIN, INT_MIN);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (u[j] < 0) break;  // wave-uniform
+                uint32_t imp = nib ? relax4(a.D + (int64_t)u[j] * a.KS + 4 * lane, dv, du[j], wt[j], nib) : 0u;
+                if (!__any(imp != 0)) continue;
+                // 8 lanes' nibbles form one 32-bit word of N[u]
+                uint32_t word = imp << (4 * (lane & 7));
+                word |= __shfl_xor(word, 1);
+                word |= __shfl_xor(word, 2);
+                word |= __shfl_xor(word, 4);
+                uint32_t *nu = reinterpret_cast<uint32_t *>(a.N + 2 * u[j]);
+                if ((lane & 7) == 0 && word) atomicOr(&nu[lane >> 3], word);
+                if (lane == 0) {
+                    const uint32_t bit = 1u << (u[j] & 31);
+                    if (!(act_next[u[j] >> 5] & bit)) atomicOr(&act_next[u[j] >> 5], bit);
+                }
+            }
+        }
+    }
+    ex = block_sum<BLK>(ex);  // each lane counted its own sources
+    if (threadIdx.x == 0) {
+        LevelPart p{0, 0, ex, 0};
+        a.part[a.G + blockIdx.x] = p;
+    }
+}
+
+__global__ __launch_bounds__(BLK) void sb_fin_a(SBatchArgs a) {
+    Ctl *c = a.ctl;
+    if (c->done) return;
+    for_each_bit(a.act[c->level & 1], a.nwords, [&](int v) {
+        if (v >= 0) st8(a.F, v, M8{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)});
+    });
+}
+
+__global__ __launch_bounds__(BLK) void sb_fin_b(SBatchArgs a) {
+    Ctl *c = a.ctl;
+    if (c->done) return;
+    const int L = c->level;
+    uint32_t *old = a.act[L & 1];
+    for (int i = blockIdx.x * BLK + (int)threadIdx.x; i < a.nwords; i += gridDim.x * BLK) old[i] = 0;
+    long long nf = 0;
+    for_each_bit(a.act[(L + 1) & 1], a.nwords, [&](int u) {
+        if (u < 0) return;
+        st8(a.F, u, ld8(a.N, u));
+        st8(a.N, u, M8{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)});
+        ++nf;
+    });
+    nf = block_sum<BLK>(nf);
+    if (threadIdx.x == 0) {
+        LevelPart p{nf, 0, 0, 0};
+        a.part[blockIdx.x] = p;
+        if (blockIdx.x == 0) c->pending = 1;
+    }
+}
+
+// D (n x KS) -> out (k x n) through a 64 x 64 LDS tile; counts reached
+// (source, vertex) pairs and their out-degrees into part[blockIdx].
+__global__ __launch_bounds__(BLK) void sb_transpose(SBatchArgs a) {
+    __shared__ int tile[64][65];
+    const int tv = (a.n + 63) / 64, ts = (a.k + 63) / 64;
+    long long te = 0, nr = 0;
+    for (int t = blockIdx.x; t < tv * ts; t += gridDim.x) {
+        const int v0 = (t % tv) * 64, s0 = (t / tv) * 64;
+        for (int i = threadIdx.x; i < 64 * 64; i += BLK) {
+            const int r = i / 64, cc = i % 64;
+            const int v = v0 + r, s = s0 + cc;
+            tile[r][cc] = (v < a.n && s < a.k) ? a.D[(int64_t)v * a.KS + s] : INT_MAX;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < 64 * 64; i += BLK) {
+            const int cc = i / 64, r = i % 64;
+            const int v = v0 + r, s = s0 + cc;
+            if (v < a.n && s < a.k) {
+                const int d = tile[r][cc];
+                a.out[(int64_t)s * a.n + v] = d;
+                if (d != INT_MAX) {
+                    ++nr;
+                    te += a.off[v + 1] - a.off[v];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    te = block_sum<BLK>(te);
+    nr = block_sum<BLK>(nr);
+    if (threadIdx.x == 0) {
+        LevelPart p{0, 0, te, nr};
+        a.part[blockIdx.x] = p;
+    }
+}
+
+__global__ __launch_bounds__(SMALL_BLK) void sb_stats(SBatchArgs a, int nblocks) {
+    long long te = 0, nr = 0;
+    for (int i = threadIdx.x; i < nblocks; i += SMALL_BLK) {
+        te += a.part[i].ex;
+        nr += a.part[i].pad;
+    }
+    te = block_sum<SMALL_BLK>(te);
+    nr = block_sum<SMALL_BLK>(nr);
+    if (threadIdx.x == 0) {
+        a.ctl->edges_traversed = te;
+        a.ctl->reached = nr;
+    }
+}
+
+}  // namespace mg
+
+using namespace mg;
+
+extern "C" int mg_sssp_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t *dist_out, const mg_options *opt,
+                             mg_stats *stats) {
+    CallTimer timer;
+    mg_options o;
+    int rc = begin_search(g, opt, &o);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g->mu);
+    if ((rc = check_batch_sources("mg_sssp_batch", g, srcs, k))) return rc;
+    if (!g->w) return set_error(MG_ERR_UNSUPPORTED, "mg_sssp_batch: the graph has no edge weights");
+    if (g->min_weight < 0) return set_error(MG_ERR_UNSUPPORTED, "mg_sssp_batch: negative edge weights");
+    if (o.directed == 0 && !(g->flags & MG_GRAPH_SYMMETRIC))
+        return set_error(MG_ERR_UNSUPPORTED, "mg_sssp_batch: directed=0 needs a symmetric graph");
+    if ((rc = ensure_batch(g, true))) return rc;
+    Workspace &ws = g->ws;
+    BatchState &bs = g->steps->batch;
+    SBatchArgs a{};
+    a.ctl = ws.ctl;
+    a.off = g->row_off;
+    a.adj = g->col;
+    a.w = g->w;
+    a.D = bs.D;
+    a.F = reinterpret_cast<uint4 *>(bs.F);
+    a.N = reinterpret_cast<uint4 *>(bs.N);
+    a.act[0] = ws.bm[0];
+    a.act[1] = ws.bm[1];
+    a.tasks = bs.tasks;
+    a.task_cap = bs.task_cap;
+    a.part = ws.part;
+    a.out = bs.out;
+    a.n = g->n;
+    a.nwords = ws.nwords;
+    a.k = k;
+    a.KS = (k + 15) / 16 * 16;
+    a.G = grid_blocks(std::max<int64_t>(g->n / 4, g->m / 16), BLK);
+
+    MG_HIP(hipMemcpyAsync(bs.srcs, srcs, 4 * (size_t)k, hipMemcpyHostToDevice, g->stream));
+    MG_HIP(hipEventRecord(g->ev0, g->stream));
+    hipLaunchKernelGGL(sb_init, dim3(grid_blocks((int64_t)g->n * a.KS / 4, BLK)), dim3(BLK), 0, g->stream, a,
+                       o.iter_num);
+    hipLaunchKernelGGL(sb_seed, dim3(1), dim3(MG_BATCH_MAX), 0, g->stream, a, (const int *)bs.srcs);
+    MG_HIP(hipGetLastError());
+    auto step = [&]() -> int {
+        hipLaunchKernelGGL(sb_close, dim3(1), dim3(SMALL_BLK), 0, g->stream, a);
+        hipLaunchKernelGGL(sb_tasks, dim3(a.G), dim3(BLK), 0, g->stream, a);
+        hipLaunchKernelGGL(sb_relax, dim3(a.G), dim3(BLK), 0, g->stream, a);
+        hipLaunchKernelGGL(sb_fin_a, dim3(a.G), dim3(BLK), 0, g->stream, a);
+        hipLaunchKernelGGL(sb_fin_b, dim3(a.G), dim3(BLK), 0, g->stream, a);
+        MG_HIP(hipGetLastError());
+        return MG_OK;
+    };
+    const uint64_t key = mix64((uint64_t)k * 2 + (uint64_t)o.directed);
+    const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n) + 3;
+    if ((rc = drive(g, step, max_steps, KIND_BATCH_SSSP, key))) return rc;
+    const int tb = grid_blocks((int64_t)((g->n + 63) / 64) * ((k + 63) / 64) * BLK, BLK);
+    hipLaunchKernelGGL(sb_transpose, dim3(tb), dim3(BLK), 0, g->stream, a);
+    hipLaunchKernelGGL(sb_stats, dim3(1), dim3(SMALL_BLK), 0, g->stream, a, tb);
+    MG_HIP(hipGetLastError());
+    rc = finish_search(g, stats, nullptr, 0, dist_out, bs.out, 4 * (size_t)k * g->n, o.out_on_device, timer);
+    if (rc) return rc;
+    if (stats) {
+        stats->edges_traversed = ws.h_ctl->edges_traversed;
+        stats->vertices_reached = ws.h_ctl->reached;
+    }
+    return MG_OK;
+}

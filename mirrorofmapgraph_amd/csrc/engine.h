@@ -155,7 +155,7 @@ int deliver(mg_graph *g, void *dst, const void *src, size_t bytes, bool out_on_d
 // arguments are fixed per (graph, kind, key), so the executable graph is reused by
 // every later search with the same key.
 constexpr int STEPS_PER_GRAPH = 8;
-enum { KIND_BFS = 0, KIND_SSSP = 1, KIND_PR = 2, KIND_GAS = 3, KIND_BATCH = 4, NKINDS = 8 };
+enum { KIND_BFS = 0, KIND_SSSP = 1, KIND_PR = 2, KIND_GAS = 3, KIND_BATCH = 4, KIND_BATCH_SSSP = 5, NKINDS = 8 };
 
 struct StepGraph {
     hipGraphExec_t exec = nullptr;
@@ -184,6 +184,10 @@ struct BatchState {
     int32_t *out = nullptr, *srcs = nullptr;
     int32_t n = 0;
     size_t out_cap = 0;
+    // batched SSSP: n x MG_BATCH_MAX distances (vertex-major) and edge tasks
+    int32_t *D = nullptr;
+    mg::ChunkRec *tasks = nullptr;
+    int64_t task_cap = 0;
 };
 
 struct mg_graph_steps {

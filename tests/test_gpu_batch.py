@@ -1,6 +1,6 @@
-"""Batched multi-source BFS on the GPU: row s of mg_bfs_batch must equal a
-single-source search from srcs[s], bit for bit, and the stats must be the sums of
-the per-source counts.  Small cases are checked against the CPU oracle, the
+"""Batched multi-source BFS and SSSP on the GPU: row s of mg_bfs_batch /
+mg_sssp_batch must equal a single-source search from srcs[s], bit for bit, and the
+stats must be the sums of the per-source counts.  Small cases are checked against the CPU oracle, the
 R-MAT scale-20 case (256 sources, the batched workload's size) against mg_bfs on
 a sample of rows plus size-independent properties of every row."""
 import numpy as np
@@ -128,3 +128,91 @@ def test_batch_rmat20_256_sources(mg, device):
     assert st.vertices_reached == int((got >= 0).sum())
     deg = np.diff(ro).astype(np.int64)
     assert st.edges_traversed == sum(int(deg[row >= 0].sum()) for row in got)
+
+
+# ---------------------------------------------------------------- batched SSSP
+INT_MAX = 2**31 - 1
+
+
+def _oracle_sssp_rows(oracle, g, srcs):
+    return np.stack([oracle.sssp(g.row_offsets, g.col_idx, g.values, int(s)) for s in srcs])
+
+
+def _check_sssp_stats(g, rows, st):
+    deg = np.diff(g.row_offsets).astype(np.int64)
+    assert st.vertices_reached == int((rows != INT_MAX).sum())
+    assert st.edges_traversed == sum(int(deg[r != INT_MAX].sum()) for r in rows)
+
+
+@pytest.mark.parametrize("k", [1, 17, 64, 256])
+def test_sssp_batch_rmat(mg, oracle, device, k):
+    g = mg.rmat_graph(13, 16, seed=7, weight_max=64)
+    srcs = mg.pick_sources(g, k, seed=k + 100)
+    with mg.Graph(g, symmetric=True) as dg:
+        got, st = dg.sssp_batch(srcs, stats=True)
+    exp = _oracle_sssp_rows(oracle, g, srcs)
+    bad = np.argwhere(got != exp)
+    assert bad.size == 0, f"first mismatches (row, vertex): {bad[:5].tolist()}"
+    _check_sssp_stats(g, exp, st)
+
+
+def test_sssp_batch_directed_and_grid(mg, oracle, device):
+    g = mg.rmat_graph(12, 8, seed=3, symmetric=False, weight_max=1000)
+    srcs = mg.pick_sources(g, 100, seed=9)
+    with mg.Graph(g) as dg:
+        got, st = dg.sssp_batch(srcs, stats=True)
+    exp = _oracle_sssp_rows(oracle, g, srcs)
+    assert np.array_equal(got, exp)
+    _check_sssp_stats(g, exp, st)
+    g = mg.grid_graph(80, seed=5, weight_max=16)
+    srcs = np.random.default_rng(1).integers(0, g.n, 45).astype(np.int32)
+    srcs[1] = srcs[0]
+    with mg.Graph(g, symmetric=True) as dg:
+        got = dg.sssp_batch(srcs)
+    assert np.array_equal(got, _oracle_sssp_rows(oracle, g, srcs))
+
+
+def test_sssp_batch_fixtures_zero_weights_and_errors(mg, oracle, device):
+    for name in ["small.mtx", "chesapeake.mtx"]:
+        g = load_mtx(mg, name)
+        srcs = np.arange(g.n, dtype=np.int32)
+        with mg.Graph(g) as dg:
+            assert np.array_equal(dg.sssp_batch(srcs), _oracle_sssp_rows(oracle, g, srcs)), name
+    g = random_graph(mg, 2000, 12000, 8, wmax=3)
+    g.values = (g.values - 1).astype(np.int32)  # weights 0..2, zero-weight edges included
+    srcs = mg.pick_sources(g, 70, seed=3)
+    with mg.Graph(g, symmetric=True) as dg:
+        assert np.array_equal(dg.sssp_batch(srcs), _oracle_sssp_rows(oracle, g, srcs))
+        with pytest.raises(mg.MgError):
+            dg.sssp_batch([g.n])
+    gu = mg.rmat_graph(10, 8, seed=1)  # no weights
+    with mg.Graph(gu, symmetric=True) as dg:
+        with pytest.raises(mg.MgError):
+            dg.sssp_batch([0])
+
+
+def test_sssp_batch_rmat20_256_sources(mg, device):
+    g = mg.rmat_graph(20, 16, seed=42, weight_max=64)
+    srcs = mg.pick_sources(g, 256, seed=43)
+    with mg.Graph(g, symmetric=True) as dg:
+        got, st = dg.sssp_batch(srcs, stats=True)
+        for s in (0, 100, 255):
+            assert np.array_equal(got[s], dg.sssp(int(srcs[s]))), s
+    # Sampled rows are exact shortest distances: d[src] = 0, no edge can shorten
+    # any distance (d[u] <= d[v] + w), and every reached vertex other than the
+    # source has a tight in-edge (d[u] = d[v] + w for some v) -- with positive
+    # weights these conditions determine the distances.
+    ro, ci, w = g.row_offsets, g.col_idx, g.values.astype(np.int64)
+    src_rows = np.repeat(np.arange(g.n, dtype=np.int64), np.diff(ro))
+    assert np.all(got[np.arange(256), srcs] == 0)
+    for s in range(0, 256, 32):
+        d = got[s].astype(np.int64)
+        dv, du = d[src_rows], d[ci]
+        ok = dv != INT_MAX
+        assert np.all(du[ok] <= dv[ok] + w[ok])
+        tight = np.zeros(g.n, bool)
+        tight[ci[ok & (du == dv + w)]] = True
+        reached = d != INT_MAX
+        reached[srcs[s]] = False
+        assert np.all(tight[reached])
+    _check_sssp_stats(g, got, st)
