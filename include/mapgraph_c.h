@@ -117,7 +117,10 @@ typedef struct mg_options {
     int32_t beta;           /* BFS bottom-up -> top-down switch (frontier < n/beta)        */
     int32_t batch_alpha;    /* mg_bfs_batch push -> pull switch (frontier edges > m/batch_alpha); 0 = 4 */
     int32_t batch_beta;     /* mg_bfs_batch pull -> push switch (frontier edges < m/batch_beta); 0 = 16  */
-    int32_t reserved[6];
+    int32_t sssp_delta;     /* SSSP bucket width: relax only distances below the current
+                               bucket bound, raised by sssp_delta when no such distance is
+                               left; 0 = chosen from the mean weight, -1 = unbucketed     */
+    int32_t reserved[5];
 } mg_options;
 
 /* Fills *o with the reference defaults. */

@@ -66,7 +66,7 @@ class Options(C.Structure):
         ("pr_mode", C.c_int32), ("pr_teleport", C.c_float), ("pr_tol", C.c_float),
         ("pr_active_eps", C.c_float), ("max_queue_sizing", C.c_double),
         ("alpha", C.c_int32), ("beta", C.c_int32), ("batch_alpha", C.c_int32), ("batch_beta", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("sssp_delta", C.c_int32), ("reserved", C.c_int32 * 5),
     ]
 
 

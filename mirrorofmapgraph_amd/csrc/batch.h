@@ -6,6 +6,7 @@
 namespace mg {
 
 constexpr int SB_TASK = 32;  // batched SSSP: edges per wave task
+constexpr int SB_RING = 16;  // batched SSSP: bucket bitmaps in the ring
 
 
 struct M8 {

@@ -301,6 +301,8 @@ void batch_release(mg_graph *g) {
     free_dev(bs.srcs);
     free_dev(bs.D);
     free_dev(bs.tasks);
+    free_dev(bs.FAR);
+    free_dev(bs.fb);
     bs = BatchState{};
 }
 
@@ -327,11 +329,17 @@ int ensure_batch(mg_graph *g, bool sssp) {
         bs.task_cap = g->m / SB_TASK + (int64_t)n + 64;
         e = hipMalloc((void **)&bs.D, 4 * (size_t)MG_BATCH_MAX * n + 64);
         if (e == hipSuccess) e = hipMalloc((void **)&bs.tasks, sizeof(ChunkRec) * (size_t)bs.task_cap);
+        if (e == hipSuccess) e = hipMalloc(&bs.FAR, 32 * n + 32);
+        if (e == hipSuccess) e = hipMalloc((void **)&bs.fb, 4 * (size_t)SB_RING * g->ws.nwords);
         if (e != hipSuccess) {
             free_dev(bs.D);
             free_dev(bs.tasks);
+            free_dev(bs.FAR);
+            free_dev(bs.fb);
             bs.D = nullptr;
             bs.tasks = nullptr;
+            bs.FAR = nullptr;
+            bs.fb = nullptr;
             return set_error(MG_ERR_OOM, "batched SSSP state allocation failed: %s", hipGetErrorString(e));
         }
     }

@@ -3,23 +3,30 @@
 // The reference runs one search per source (Algorithms/SSSP/sssp.cu:310-428).  Here the
 // distances are vertex-major, D[v * KS + s] (KS = k rounded up to 16, one 64-byte
 // line per 16 sources), so one wave relaxes one edge for all sources at once: lane l
-// owns sources 4l..4l+3 and moves 16 contiguous bytes of D[u].  Each vertex carries a
-// 256-bit change mask F (sources whose distance dropped since v was last expanded)
-// and N (drops found this iteration).  One iteration is
-//   * tasks: every active vertex v (listed in the bitmap A[L%2]) is cut into
-//     SB_TASK-edge tasks;
-//   * relax: one wave per task; for each edge (v, u, w) and each source s in F[v]:
-//     nd = D[v][s] + w; if nd < D[u][s]: atomicMin(D[u][s], nd), set bit s of N[u]
-//     and mark u in A[(L+1)%2];
-//   * finalize: F[v] = 0 for the old frontier, then F[u] = N[u], N[u] = 0.
-// This is the frontier Bellman-Ford of the single-source path run per source; every
-// source converges to its unique shortest distances, so row s equals mg_sssp from
-// srcs[s].  A distance read while another wave lowers it is only stale-high: the
-// later drop marks the vertex again.  At convergence D is transposed into the
-// k x n result.
+// owns sources 4l..4l+3 and moves 16 contiguous bytes of D[u].  Each vertex carries
+// 256-bit masks F (pairs (v, s) to expand this iteration), N (pairs improved this
+// iteration) and FAR (pairs waiting in a later bucket).
+//
+// Buckets (delta-stepping, width delta shared by all sources): only distances below
+// the bound T = (b + 1) * delta of the current bucket b are expanded.  One iteration:
+//   * tasks: every active vertex (bitmap A[L%2]) is cut into SB_TASK-edge tasks;
+//   * relax: one wave per task; for each edge (v, u, w) and each s in F[v]:
+//     c = D[v][s] + w; if c < D[u][s]: atomicMin(D[u][s], c), then if c < T the pair
+//     joins N[u] (expanded next iteration) else FAR[u], with u marked in the ring
+//     bitmap of bucket c / delta;
+//   * finalize: F = 0 on the old frontier, F[u] = N[u] on the new one.
+// When an iteration leaves no pair in N, the next bucket's ring bitmap is scanned
+// and its waiting pairs move from FAR to F.  Every improvement is at most
+// max_weight above T, so a ring of SB_RING bitmaps never wraps onto a live bucket
+// when delta >= max_weight / (SB_RING - 2).  Any such schedule is label-correcting:
+// every source converges to its unique shortest distances, so row s equals mg_sssp
+// from srcs[s].  A distance read while another wave lowers it is only stale-high:
+// the later drop marks the pair again.  At the end D is transposed into the k x n
+// result.
 #include "batch.h"
 
 #include <climits>
+#include <cmath>
 
 namespace mg {
 
@@ -27,13 +34,15 @@ struct SBatchArgs {
     Ctl *ctl;
     const int *off, *adj, *w;
     int *D;                // n x KS
-    uint4 *F, *N;          // change masks, 2 uint4 per vertex
+    uint4 *F, *N, *FAR;    // pair masks, 2 uint4 per vertex
     uint32_t *act[2];
+    uint32_t *ring;        // SB_RING bitmaps of nwords words
     ChunkRec *tasks;
     int64_t task_cap;
     LevelPart *part;       // [0, G): finalize counts, [G, 2G): relax counts
     int *out;              // k x n
     int n, nwords, k, KS, G;
+    long long delta;       // bucket width (LLONG_MAX / 4: one bucket, plain Bellman-Ford)
 };
 
 __global__ void sb_init(SBatchArgs a, int iter_limit) {
@@ -44,12 +53,18 @@ __global__ void sb_init(SBatchArgs a, int iter_limit) {
     for (int64_t i = tid; i < 2 * (int64_t)a.n; i += nt) {
         a.F[i] = make_uint4(0, 0, 0, 0);
         a.N[i] = make_uint4(0, 0, 0, 0);
+        a.FAR[i] = make_uint4(0, 0, 0, 0);
     }
     for (int64_t i = tid; i < a.nwords; i += nt) {
         a.act[0][i] = 0;
         a.act[1][i] = 0;
     }
-    if (tid == 0) ctl_reset(a.ctl, a.n, iter_limit, a.k);
+    for (int64_t i = tid; i < (int64_t)SB_RING * a.nwords; i += nt) a.ring[i] = 0;
+    if (tid == 0) {
+        ctl_reset(a.ctl, a.n, iter_limit, a.k);
+        a.ctl->thr = a.delta;
+        a.ctl->far_hi = 0;
+    }
 }
 
 __global__ void sb_seed(SBatchArgs a, const int *srcs) {
@@ -62,26 +77,119 @@ __global__ void sb_seed(SBatchArgs a, const int *srcs) {
     atomicOr(&a.act[0][v >> 5], 1u << (v & 31));
 }
 
-// One workgroup: closes the previous iteration.
+// One workgroup: closes the previous iteration; moves to the next bucket when the
+// frontier ran out (aux = 1 asks sb_activate to refill it).
 __global__ __launch_bounds__(SMALL_BLK) void sb_close(SBatchArgs a) {
     Ctl *c = a.ctl;
     if (!c->pending) return;
-    long long nf = 0, ex = 0;
+    long long nf = 0, ex = 0, hi = 0;
     for (int i = threadIdx.x; i < a.G; i += SMALL_BLK) {
         nf += a.part[i].nf;
-        ex += a.part[a.G + i].ex;
+        const LevelPart p = a.part[a.G + i];
+        ex += p.ex;
+        hi = max(hi, p.mf);
     }
     nf = block_sum<SMALL_BLK>(nf);
     ex = block_sum<SMALL_BLK>(ex);
+    hi = -block_min<SMALL_BLK>(-hi);
     if (threadIdx.x == 0) {
         const int L = c->level;
         c->edges_examined += ex;
+        c->far_hi = max(c->far_hi, hi);
         c->pending = 0;
         c->nchunks = 0;
         c->level = L + 1;
         c->nf_prev = (int)nf;
-        if (nf == 0 || L + 1 >= c->iter_limit) c->done = 1;
+        c->aux = 0;
+        if (nf == 0) {
+            if (c->thr / a.delta - 1 < c->far_hi) {
+                c->thr += a.delta;
+                c->aux = 1;
+            } else {
+                c->done = 1;
+            }
+        }
+        if (L + 1 >= c->iter_limit) c->done = 1;
     }
+}
+
+// Calls fn(v) for each set bit of bm, one vertex per wave (all lanes), and clears
+// the words it reads when `clear` is set.
+template <class Fn>
+__device__ void for_each_bit_wave(uint32_t *bm, int nwords, bool clear, Fn fn) {
+    constexpr int WPR = BLK / 2;
+    __shared__ int s_q[WPR * 32];
+    __shared__ int s_w[BLK / WAVE];
+    const int lane = lane_id(), wv = (int)threadIdx.x / WAVE;
+    for (int w0 = blockIdx.x * WPR; w0 < nwords; w0 += gridDim.x * WPR) {
+        const int w = w0 + (int)threadIdx.x;
+        uint32_t word = 0;
+        if (threadIdx.x < WPR && w < nwords) {
+            word = bm[w];
+            if (clear && word) bm[w] = 0;
+        }
+        int total;
+        int k = block_exclusive_scan<BLK>(__popc(word), s_w, &total);
+        while (word) {
+            s_q[k++] = w * 32 + __ffs(word) - 1;
+            word &= word - 1;
+        }
+        __syncthreads();
+        for (int i = wv; i < total; i += BLK / WAVE) fn(s_q[i], lane);
+        __syncthreads();
+    }
+}
+
+// Combines the lanes' source nibbles into the 32-bit words of a vertex mask.
+__device__ __forceinline__ uint32_t nibble_word(uint32_t nib, int lane) {
+    uint32_t word = nib << (4 * (lane & 7));
+    word |= __shfl_xor(word, 1);
+    word |= __shfl_xor(word, 2);
+    word |= __shfl_xor(word, 4);
+    return word;
+}
+
+// ORs the lanes' nibbles into mask m of vertex v (lanes 0, 8, .., 56 issue the atomics).
+__device__ __forceinline__ void or_nibbles(uint4 *m, int v, uint32_t nib, int lane) {
+    const uint32_t word = nibble_word(nib, lane);
+    if ((lane & 7) == 0 && word) atomicOr(reinterpret_cast<uint32_t *>(m + 2 * v) + (lane >> 3), word);
+}
+
+__device__ __forceinline__ void mark_active(uint32_t *bm, int v) {
+    const uint32_t bit = 1u << (v & 31);
+    if (!(bm[v >> 5] & bit)) atomicOr(&bm[v >> 5], bit);
+}
+
+// New bucket: pairs of the ring bitmap's vertices whose distance lies in the bucket
+// move from FAR to F; pairs already below it were expanded when they dropped.
+__global__ __launch_bounds__(BLK) void sb_activate(SBatchArgs a) {
+    Ctl *c = a.ctl;
+    if (c->done || !c->aux) return;
+    const long long T = c->thr, T0 = T - a.delta;
+    const int L = c->level;
+    uint32_t *bm = a.ring + (int64_t)((T0 / a.delta) % SB_RING) * a.nwords;
+    uint32_t *act = a.act[L & 1];
+    for_each_bit_wave(bm, a.nwords, true, [&](int u, int lane) {
+        const uint32_t fw = reinterpret_cast<const uint32_t *>(a.FAR + 2 * u)[lane >> 3];
+        const uint32_t nib = 4 * lane < a.KS ? (fw >> (4 * (lane & 7))) & 0xFu : 0u;
+        uint32_t go = 0, drop = 0;
+        if (nib) {
+            const int4 d = reinterpret_cast<const int4 *>(a.D + (int64_t)u * a.KS)[lane];
+            const int dd[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                if (nib & (1u << b)) {
+                    if (dd[b] < T0) drop |= 1u << b;
+                    else if (dd[b] < T) go |= 1u << b;
+                }
+        }
+        const uint32_t gw = nibble_word(go, lane), kw = nibble_word(nib & ~(go | drop), lane);
+        if ((lane & 7) == 0 && 4 * lane < a.KS) {
+            reinterpret_cast<uint32_t *>(a.FAR + 2 * u)[lane >> 3] = kw;
+            if (gw) reinterpret_cast<uint32_t *>(a.F + 2 * u)[lane >> 3] = gw;
+        }
+        if (__any(go != 0) && lane == 0) atomicOr(&act[u >> 5], 1u << (u & 31));
+    });
 }
 
 __global__ __launch_bounds__(BLK) void sb_tasks(SBatchArgs a) {
@@ -114,41 +222,18 @@ __global__ __launch_bounds__(BLK) void sb_tasks(SBatchArgs a) {
     });
 }
 
-// Relaxes one 4-source slice: returns the nibble of sources whose distance dropped.
-__device__ __forceinline__ uint32_t relax4(int *du_ptr, const int4 &dv, const int4 &du, int w, uint32_t nib) {
-    uint32_t imp = 0;
-    const long long c0 = (long long)dv.x + w, c1 = (long long)dv.y + w, c2 = (long long)dv.z + w,
-                    c3 = (long long)dv.w + w;
-    if ((nib & 1) && c0 < du.x) {
-        atomicMin(du_ptr + 0, (int)c0);
-        imp |= 1;
-    }
-    if ((nib & 2) && c1 < du.y) {
-        atomicMin(du_ptr + 1, (int)c1);
-        imp |= 2;
-    }
-    if ((nib & 4) && c2 < du.z) {
-        atomicMin(du_ptr + 2, (int)c2);
-        imp |= 4;
-    }
-    if ((nib & 8) && c3 < du.w) {
-        atomicMin(du_ptr + 3, (int)c3);
-        imp |= 8;
-    }
-    return imp;
-}
-
 // One wave per task; lanes are sources.
 __global__ __launch_bounds__(BLK) void sb_relax(SBatchArgs a) {
     Ctl *c = a.ctl;
     if (c->done) return;
     const int L = c->level;
+    const long long T = c->thr, delta = a.delta;
     uint32_t *act_next = a.act[(L + 1) & 1];
     const int lane = lane_id();
     const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
     const int nt = (int)min((int64_t)c->nchunks, a.task_cap);
     const bool has_slice = 4 * lane < a.KS;
-    long long ex = 0;
+    long long ex = 0, hi = 0;
     for (int t = wave; t < nt; t += nwaves) {
         const ChunkRec r = a.tasks[t];
         const uint32_t fw = reinterpret_cast<const uint32_t *>(a.F + 2 * r.v)[lane >> 3];
@@ -172,25 +257,44 @@ __global__ __launch_bounds__(BLK) void sb_relax(SBatchArgs a) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if (u[j] < 0) break;  // wave-uniform
-                uint32_t imp = nib ? relax4(a.D + (int64_t)u[j] * a.KS + 4 * lane, dv, du[j], wt[j], nib) : 0u;
-                if (!__any(imp != 0)) continue;
-                // 8 lanes' nibbles form one 32-bit word of N[u]
-                uint32_t word = imp << (4 * (lane & 7));
-                word |= __shfl_xor(word, 1);
-                word |= __shfl_xor(word, 2);
-                word |= __shfl_xor(word, 4);
-                uint32_t *nu = reinterpret_cast<uint32_t *>(a.N + 2 * u[j]);
-                if ((lane & 7) == 0 && word) atomicOr(&nu[lane >> 3], word);
-                if (lane == 0) {
-                    const uint32_t bit = 1u << (u[j] & 31);
-                    if (!(act_next[u[j] >> 5] & bit)) atomicOr(&act_next[u[j] >> 5], bit);
+                uint32_t near = 0, far = 0, slots = 0;
+                if (nib) {
+                    const long long cs[4] = {(long long)dv.x + wt[j], (long long)dv.y + wt[j],
+                                             (long long)dv.z + wt[j], (long long)dv.w + wt[j]};
+                    const int ds[4] = {du[j].x, du[j].y, du[j].z, du[j].w};
+                    int *dp = a.D + (int64_t)u[j] * a.KS + 4 * lane;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        if ((nib & (1u << b)) && cs[b] < ds[b]) {
+                            atomicMin(dp + b, (int)cs[b]);
+                            if (cs[b] < T) {
+                                near |= 1u << b;
+                            } else {
+                                far |= 1u << b;
+                                const long long bk = cs[b] / delta;
+                                slots |= 1u << (int)(bk % SB_RING);
+                                hi = max(hi, bk);
+                            }
+                        }
+                }
+                const bool any_near = __any(near != 0), any_far = __any(far != 0);
+                if (any_near) {
+                    or_nibbles(a.N, u[j], near, lane);
+                    if (lane == 0) mark_active(act_next, u[j]);
+                }
+                if (any_far) {
+                    or_nibbles(a.FAR, u[j], far, lane);
+#pragma unroll
+                    for (int o = 1; o < WAVE; o <<= 1) slots |= __shfl_xor(slots, o);
+                    if (lane < SB_RING && (slots >> lane) & 1) mark_active(a.ring + (int64_t)lane * a.nwords, u[j]);
                 }
             }
         }
     }
     ex = block_sum<BLK>(ex);  // each lane counted its own sources
+    hi = -block_min<BLK>(-hi);
     if (threadIdx.x == 0) {
-        LevelPart p{0, 0, ex, 0};
+        LevelPart p{0, hi, ex, 0};
         a.part[a.G + blockIdx.x] = p;
     }
 }
@@ -274,6 +378,17 @@ __global__ __launch_bounds__(SMALL_BLK) void sb_stats(SBatchArgs a, int nblocks)
     }
 }
 
+// Bucket width: the option, or about four mean edge weights, raised to
+// max_weight / (SB_RING - 2); LLONG_MAX / 4 when unbucketed (sssp_delta < 0), which
+// no distance reaches.
+long long sssp_delta(const mg_graph *g, const mg_options &o) {
+    if (o.sssp_delta < 0) return LLONG_MAX / 4;
+    const long long d = o.sssp_delta > 0 ? o.sssp_delta : std::llround(4.0 * g->mean_weight);
+    // the bucket ring must span every improvement: delta >= max_weight / (SB_RING - 2)
+    const long long lo = ((long long)g->max_weight + SB_RING - 3) / (SB_RING - 2);
+    return std::max<long long>({1, d, lo});
+}
+
 }  // namespace mg
 
 using namespace mg;
@@ -301,6 +416,8 @@ extern "C" int mg_sssp_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_
     a.D = bs.D;
     a.F = reinterpret_cast<uint4 *>(bs.F);
     a.N = reinterpret_cast<uint4 *>(bs.N);
+    a.FAR = reinterpret_cast<uint4 *>(bs.FAR);
+    a.ring = bs.fb;
     a.act[0] = ws.bm[0];
     a.act[1] = ws.bm[1];
     a.tasks = bs.tasks;
@@ -312,6 +429,7 @@ extern "C" int mg_sssp_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_
     a.k = k;
     a.KS = (k + 15) / 16 * 16;
     a.G = grid_blocks(std::max<int64_t>(g->n / 4, g->m / 16), BLK);
+    a.delta = sssp_delta(g, o);
 
     MG_HIP(hipMemcpyAsync(bs.srcs, srcs, 4 * (size_t)k, hipMemcpyHostToDevice, g->stream));
     MG_HIP(hipEventRecord(g->ev0, g->stream));
@@ -321,6 +439,7 @@ extern "C" int mg_sssp_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_
     MG_HIP(hipGetLastError());
     auto step = [&]() -> int {
         hipLaunchKernelGGL(sb_close, dim3(1), dim3(SMALL_BLK), 0, g->stream, a);
+        hipLaunchKernelGGL(sb_activate, dim3(a.G), dim3(BLK), 0, g->stream, a);
         hipLaunchKernelGGL(sb_tasks, dim3(a.G), dim3(BLK), 0, g->stream, a);
         hipLaunchKernelGGL(sb_relax, dim3(a.G), dim3(BLK), 0, g->stream, a);
         hipLaunchKernelGGL(sb_fin_a, dim3(a.G), dim3(BLK), 0, g->stream, a);
@@ -328,9 +447,8 @@ extern "C" int mg_sssp_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_
         MG_HIP(hipGetLastError());
         return MG_OK;
     };
-    const uint64_t key = mix64((uint64_t)k * 2 + (uint64_t)o.directed);
-    const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n) + 3;
-    if ((rc = drive(g, step, max_steps, KIND_BATCH_SSSP, key))) return rc;
+    const uint64_t key = mix64(mix64((uint64_t)k * 2 + (uint64_t)o.directed) ^ (uint64_t)a.delta);
+    if ((rc = drive(g, step, (int64_t)o.iter_num + 3, KIND_BATCH_SSSP, key))) return rc;
     const int tb = grid_blocks((int64_t)((g->n + 63) / 64) * ((k + 63) / 64) * BLK, BLK);
     hipLaunchKernelGGL(sb_transpose, dim3(tb), dim3(BLK), 0, g->stream, a);
     hipLaunchKernelGGL(sb_stats, dim3(1), dim3(SMALL_BLK), 0, g->stream, a, tb);

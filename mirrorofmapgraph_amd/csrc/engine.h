@@ -75,7 +75,9 @@ struct Ctl {
     long long edges_examined;
     double dsum;              // scratch accumulator (PageRank L1 change)
     long long mf_cur;         // out-edges of the current frontier
-    long long pad1[3];
+    long long thr;            // bucketed SSSP: distances below thr are in the current bucket
+    long long far_hi;         // bucketed SSSP: highest bucket holding a waiting distance
+    long long pad1[1];
 };
 
 // Pooled per-search device state, grown on demand and reused across searches.
@@ -121,6 +123,8 @@ struct mg_graph {
     int32_t *row_off = nullptr, *col = nullptr, *w = nullptr;
     int32_t *col_off = nullptr, *row_idx = nullptr, *csc_eid = nullptr;
     int32_t min_weight = 0;
+    int32_t max_weight = 0;
+    double mean_weight = 0;
     int64_t topo_bytes = 0;
     std::mutex mu;
     mg::Workspace ws;
@@ -174,6 +178,8 @@ int gas_cc(mg_graph *g, int *out, const mg_options &o, mg_stats *st, const CallT
 int gas_pagerank(mg_graph *g, float *out, const mg_options &o, mg_stats *st, const CallTimer &t);
 void gas_release(mg_graph *g);
 void batch_release(mg_graph *g);
+// SSSP bucket width for options o on graph g (a huge value when unbucketed).
+long long sssp_delta(const mg_graph *g, const mg_options &o);
 
 }  // namespace mg
 
@@ -188,6 +194,8 @@ struct BatchState {
     int32_t *D = nullptr;
     mg::ChunkRec *tasks = nullptr;
     int64_t task_cap = 0;
+    void *FAR = nullptr;       // per-vertex 256-bit mask of pairs waiting in a later bucket
+    uint32_t *fb = nullptr;    // ring of bucket bitmaps (vertices with a waiting pair)
 };
 
 struct mg_graph_steps {

@@ -238,9 +238,16 @@ int mg_graph_create(const int32_t *row_offsets, const int32_t *col_idx, const in
         if ((rc = dalloc(&g->w, (size_t)m))) return fail(rc);
         if (m && hipMemcpy(g->w, edge_values, 4 * (size_t)m, hipMemcpyHostToDevice) != hipSuccess)
             return fail(set_error(MG_ERR_DEVICE, "host-to-device copy of the edge values failed"));
-        int32_t mn = INT32_MAX;
-        for (int64_t e = 0; e < m; ++e) mn = edge_values[e] < mn ? edge_values[e] : mn;
+        int32_t mn = INT32_MAX, mx = INT32_MIN;
+        double sum = 0;
+        for (int64_t e = 0; e < m; ++e) {
+            mn = edge_values[e] < mn ? edge_values[e] : mn;
+            mx = edge_values[e] > mx ? edge_values[e] : mx;
+            sum += edge_values[e];
+        }
         g->min_weight = m ? mn : 0;
+        g->max_weight = m ? mx : 0;
+        g->mean_weight = m ? sum / (double)m : 0;
         g->topo_bytes += 4 * m;
     }
     if (flags & MG_GRAPH_BUILD_CSC) {

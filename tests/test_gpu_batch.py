@@ -156,11 +156,14 @@ def test_sssp_batch_rmat(mg, oracle, device, k):
     _check_sssp_stats(g, exp, st)
 
 
-def test_sssp_batch_directed_and_grid(mg, oracle, device):
+@pytest.mark.parametrize("delta", [0, -1, 1, 5, 300])
+def test_sssp_batch_directed_and_grid(mg, oracle, device, delta):
+    # delta: bucket width (0 = automatic, -1 = unbucketed Bellman-Ford)
+    opt = mg.options(sssp_delta=delta)
     g = mg.rmat_graph(12, 8, seed=3, symmetric=False, weight_max=1000)
     srcs = mg.pick_sources(g, 100, seed=9)
     with mg.Graph(g) as dg:
-        got, st = dg.sssp_batch(srcs, stats=True)
+        got, st = dg.sssp_batch(srcs, opt, stats=True)
     exp = _oracle_sssp_rows(oracle, g, srcs)
     assert np.array_equal(got, exp)
     _check_sssp_stats(g, exp, st)
@@ -168,7 +171,7 @@ def test_sssp_batch_directed_and_grid(mg, oracle, device):
     srcs = np.random.default_rng(1).integers(0, g.n, 45).astype(np.int32)
     srcs[1] = srcs[0]
     with mg.Graph(g, symmetric=True) as dg:
-        got = dg.sssp_batch(srcs)
+        got = dg.sssp_batch(srcs, opt)
     assert np.array_equal(got, _oracle_sssp_rows(oracle, g, srcs))
 
 
