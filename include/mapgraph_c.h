@@ -45,6 +45,7 @@ extern "C" {
 #define MG_ERR_FORMAT         6   /* malformed Matrix Market input                      */
 #define MG_ERR_UNSUPPORTED    7   /* the graph lacks what the call needs (e.g. weights) */
 #define MG_ERR_NO_DEVICE      8   /* no HIP device is visible                           */
+#define MG_ERR_TIMEOUT        9   /* mg_server_wait: the query did not finish in time   */
 
 /* Copies the last error message of the calling thread (NUL-terminated, truncated to
  * len) and returns its full length. */
@@ -120,7 +121,8 @@ typedef struct mg_options {
     int32_t sssp_delta;     /* SSSP bucket width: relax only distances below the current
                                bucket bound, raised by sssp_delta when no such distance is
                                left; 0 = chosen from the mean weight, -1 = unbucketed     */
-    int32_t reserved[5];
+    int32_t ppr_iters;      /* personalized PageRank iterations (0 = 20)                   */
+    int32_t reserved[4];
 } mg_options;
 
 /* Fills *o with the reference defaults. */
@@ -183,6 +185,81 @@ int mg_bfs_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t *depth_out
  * edges_traversed / vertices_reached sum the per-source counts. */
 int mg_sssp_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t *dist_out,
                   const mg_options *opt, mg_stats *stats);
+
+#define MG_PPR_MAX 64  /* sources per batched personalized PageRank call */
+
+/* Personalized PageRank from k <= MG_PPR_MAX sources, FP32 (new here; the
+ * reference has only global PageRank, Algorithms/PageRank/PageRank.h:233-283,
+ * whose conventions it keeps): x_0 = e_s and
+ *   x_{i+1}[u] = (1 - t) * sum_{v -> u} x_i[v] / outdeg(v)  (+ t at u = s),
+ * t = opt->pr_teleport (default 0.15), opt->ppr_iters iterations (default 20),
+ * in-edges of a symmetric graph or the CSC.  rank_out is k x n.  Each row's
+ * floating-point operations are fixed by the graph, so a row is bitwise the same
+ * whatever other sources share the call. */
+int mg_ppr_batch(mg_graph *g, const int32_t *srcs, int32_t k, float *rank_out,
+                 const mg_options *opt, mg_stats *stats);
+
+/* ---------------------------------------------------------------- serving */
+/* A request loop over resident graphs (new here: the reference runs one search at a
+ * time, e.g. Algorithms/BFS/bfs.cu:402-417).  Queries are served with continuous
+ * batching: each (graph, kind) engine keeps one batched search running and seeds
+ * newly admitted queries into free slots between chunks of levels, so queries that
+ * arrive while others are on the GPU join them; engines of different graphs run on
+ * their own streams and overlap.  Engine state is allocated within hbm_budget; a
+ * query waits while no slot or memory is free.  A query's result equals the
+ * single-query result: BFS depths (mg_bfs), SSSP distances (mg_sssp), personalized
+ * PageRank ranks (mg_ppr_batch, bitwise).  Graphs must outlive the server. */
+typedef struct mg_server mg_server;
+
+#define MG_Q_BFS  0
+#define MG_Q_SSSP 1
+#define MG_Q_PPR  2
+
+typedef struct mg_server_config {
+    int64_t hbm_budget;       /* device bytes the engines may hold (default 64 GiB)      */
+    int32_t max_slots;        /* BFS / SSSP slots per engine, <= MG_BATCH_MAX (256)      */
+    int32_t ppr_iters;        /* personalized PageRank iterations (default 20)           */
+    float   ppr_teleport;     /* personalized PageRank teleport t (default 0.15)         */
+    int32_t steps_per_round;  /* levels per engine round between admissions (default 8)  */
+    int32_t sssp_delta;       /* SSSP bucket width, as mg_options.sssp_delta (0 = auto)  */
+    int32_t reserved[7];
+} mg_server_config;
+
+typedef struct mg_query_info {
+    int32_t status;           /* MG_OK or the error that failed the query                */
+    int32_t kind, graph, src;
+    int64_t edges_traversed;  /* TEPS count (PageRank: m x iterations)                   */
+    int64_t vertices_reached;
+    double  t_submit_ms, t_admit_ms, t_done_ms;  /* server clock (mg_server_now_ms)     */
+    int32_t reserved[4];
+} mg_query_info;
+
+typedef struct mg_server_stats {
+    int64_t submitted, completed, rounds, edges_traversed, device_bytes;
+    double  latency_ms_sum;
+    int32_t reserved[8];
+} mg_server_stats;
+
+void mg_server_config_default(mg_server_config *c);
+int mg_server_create(const mg_server_config *cfg, mg_server **out);
+/* Registers a graph; queries name it by *graph_id. */
+int mg_server_add_graph(mg_server *s, mg_graph *g, int32_t *graph_id);
+/* Queues a query.  out receives n int32 (BFS depths / SSSP distances) or n float
+ * (PageRank) when the query finishes; host memory from mg_host_alloc keeps the copy
+ * asynchronous (other host memory works but blocks the dispatcher while it copies);
+ * out_on_device = 1 for a device pointer; NULL discards the result. */
+int mg_server_submit(mg_server *s, int32_t graph_id, int32_t kind, int32_t src, void *out,
+                     int32_t out_on_device, int64_t *query_id);
+/* Waits for a query (timeout_ms < 0: forever), fills *info and forgets the query. */
+int mg_server_wait(mg_server *s, int64_t query_id, int32_t timeout_ms, mg_query_info *info);
+int mg_server_now_ms(mg_server *s, double *ms);
+int mg_server_get_stats(mg_server *s, mg_server_stats *st);
+/* Stops the dispatcher after the round in flight and frees all engine state;
+ * unfinished queries are dropped. */
+int mg_server_destroy(mg_server *s);
+/* Pinned host memory for results. */
+void *mg_host_alloc(size_t bytes);
+void mg_host_free(void *p);
 
 /* ---------------------------------------------------------------- host graph tools */
 /* These run on the host only and need no device. */

@@ -24,7 +24,10 @@ MG_ERR_IO = 5
 MG_ERR_FORMAT = 6
 MG_ERR_UNSUPPORTED = 7
 MG_ERR_NO_DEVICE = 8
+MG_ERR_TIMEOUT = 9
+MG_Q_BFS, MG_Q_SSSP, MG_Q_PPR = 0, 1, 2
 MG_BATCH_MAX = 256
+MG_PPR_MAX = 64
 
 MG_GRAPH_WEIGHTED = 0x1
 MG_GRAPH_BUILD_CSC = 0x2
@@ -46,10 +49,12 @@ INT32_MAX = 2**31 - 1
 EXPORTED = [
     "mg_last_error", "mg_version", "mg_device_init", "mg_device_count", "mg_device_synchronize",
     "mg_graph_create", "mg_graph_destroy", "mg_graph_info", "mg_graph_device_bytes", "mg_graph_get_csc",
-    "mg_options_default", "mg_bfs", "mg_sssp", "mg_cc", "mg_pagerank", "mg_bfs_batch", "mg_sssp_batch",
+    "mg_options_default", "mg_bfs", "mg_sssp", "mg_cc", "mg_pagerank", "mg_bfs_batch", "mg_sssp_batch", "mg_ppr_batch",
     "mg_rmat_num_edges", "mg_gen_rmat_edges", "mg_grid_num_edges", "mg_gen_grid_edges",
     "mg_csr_from_edges", "mg_edge_weights_hash", "mg_csc_from_csr", "mg_pick_sources",
     "mg_mtx_read_header", "mg_mtx_read",
+    "mg_server_config_default", "mg_server_create", "mg_server_add_graph", "mg_server_submit", "mg_server_wait",
+    "mg_server_now_ms", "mg_server_get_stats", "mg_server_destroy", "mg_host_alloc", "mg_host_free",
 ]
 
 
@@ -66,7 +71,7 @@ class Options(C.Structure):
         ("pr_mode", C.c_int32), ("pr_teleport", C.c_float), ("pr_tol", C.c_float),
         ("pr_active_eps", C.c_float), ("max_queue_sizing", C.c_double),
         ("alpha", C.c_int32), ("beta", C.c_int32), ("batch_alpha", C.c_int32), ("batch_beta", C.c_int32),
-        ("sssp_delta", C.c_int32), ("reserved", C.c_int32 * 5),
+        ("sssp_delta", C.c_int32), ("ppr_iters", C.c_int32), ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -83,6 +88,30 @@ class Stats(C.Structure):
 
 
 _lib = None
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("max_slots", C.c_int32), ("ppr_iters", C.c_int32),
+                ("ppr_teleport", C.c_float), ("steps_per_round", C.c_int32), ("sssp_delta", C.c_int32),
+                ("reserved", C.c_int32 * 7)]
+
+
+class QueryInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("kind", C.c_int32), ("graph", C.c_int32), ("src", C.c_int32),
+                ("edges_traversed", C.c_int64), ("vertices_reached", C.c_int64), ("t_submit_ms", C.c_double),
+                ("t_admit_ms", C.c_double), ("t_done_ms", C.c_double), ("reserved", C.c_int32 * 4)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rounds", C.c_int64),
+                ("edges_traversed", C.c_int64), ("device_bytes", C.c_int64), ("latency_ms_sum", C.c_double),
+                ("reserved", C.c_int32 * 8)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 I32P = C.POINTER(C.c_int32)
 F32P = C.POINTER(C.c_float)
 
@@ -134,11 +163,30 @@ def lib() -> C.CDLL:
         "mg_pagerank": [C.c_void_p, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
         "mg_bfs_batch": [C.c_void_p, I32P, C.c_int32, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
         "mg_sssp_batch": [C.c_void_p, I32P, C.c_int32, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
+        "mg_ppr_batch": [C.c_void_p, I32P, C.c_int32, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
     }
+    search.update({
+        "mg_server_config_default": [C.POINTER(ServerConfig)],
+        "mg_server_create": [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)],
+        "mg_server_add_graph": [C.c_void_p, C.c_void_p, I32P],
+        "mg_server_submit": [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_int32,
+                             C.POINTER(C.c_int64)],
+        "mg_server_wait": [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(QueryInfo)],
+        "mg_server_now_ms": [C.c_void_p, C.POINTER(C.c_double)],
+        "mg_server_get_stats": [C.c_void_p, C.POINTER(ServerStats)],
+        "mg_server_destroy": [C.c_void_p],
+        "mg_host_alloc": [C.c_size_t],
+        "mg_host_free": [C.c_void_p],
+    })
     for name, argtypes in search.items():
         fn = getattr(L, name, None)  # a missing export is reported by the export test
         if fn is not None:
             fn.argtypes = argtypes
+    for name in ("mg_server_config_default", "mg_host_free"):
+        if getattr(L, name, None) is not None:
+            getattr(L, name).restype = None
+    if getattr(L, "mg_host_alloc", None) is not None:
+        L.mg_host_alloc.restype = C.c_void_p
     _lib = L
     return L
 
@@ -376,3 +424,96 @@ class Graph:
         s = np.ascontiguousarray(srcs, dtype=np.int32)
         out = np.empty((len(s), self.n), np.int32) if copy else None
         return self._run(lib().mg_sssp_batch, _p(s), len(s), opt=opt, out=out, stats=stats)
+
+    def ppr_batch(self, srcs, opt: Options | None = None, stats: bool = False, copy: bool = True):
+        """Personalized PageRank from every source in srcs (at most MG_PPR_MAX) as a float32 len(srcs) x n array."""
+        s = np.ascontiguousarray(srcs, dtype=np.int32)
+        out = np.empty((len(s), self.n), np.float32) if copy else None
+        return self._run(lib().mg_ppr_batch, _p(s), len(s), opt=opt, out=out, stats=stats)
+
+
+class PinnedArray:
+    """A numpy array in pinned host memory (mg_host_alloc), for asynchronous result copies."""
+
+    def __init__(self, n: int, dtype=np.int32):
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(n) * self.dtype.itemsize
+        self.ptr = lib().mg_host_alloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise MemoryError("mg_host_alloc failed")
+        buf = (C.c_char * max(self.nbytes, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=self.dtype, count=int(n))
+
+    def close(self) -> None:
+        if self.ptr:
+            self.array = None
+            lib().mg_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop over resident graphs with continuous batching (mg_server)."""
+
+    def __init__(self, **cfg):
+        L = lib()
+        c = ServerConfig()
+        L.mg_server_config_default(C.byref(c))
+        for k, v in cfg.items():
+            if not hasattr(c, k):
+                raise KeyError(f"unknown server option {k}")
+            setattr(c, k, v)
+        h = C.c_void_p()
+        check(L.mg_server_create(C.byref(c), C.byref(h)))
+        self._h = h
+        self.config = c
+        self.graphs = []
+
+    def add_graph(self, g: Graph) -> int:
+        gid = C.c_int32()
+        check(lib().mg_server_add_graph(self._h, g.handle, C.byref(gid)))
+        self.graphs.append(g)
+        return gid.value
+
+    def submit(self, graph_id: int, kind: int, src: int, out: "PinnedArray | None" = None) -> int:
+        qid = C.c_int64()
+        ptr = C.c_void_p(out.ptr) if out is not None else None
+        check(lib().mg_server_submit(self._h, graph_id, kind, src, ptr, 0, C.byref(qid)))
+        return qid.value
+
+    def wait(self, qid: int, timeout_ms: int = -1) -> dict:
+        info = QueryInfo()
+        check(lib().mg_server_wait(self._h, qid, timeout_ms, C.byref(info)))
+        return info.as_dict()
+
+    def now_ms(self) -> float:
+        t = C.c_double()
+        check(lib().mg_server_now_ms(self._h, C.byref(t)))
+        return t.value
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib().mg_server_get_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().mg_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

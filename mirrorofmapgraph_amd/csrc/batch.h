@@ -77,7 +77,112 @@ __device__ void for_each_bit(const uint32_t *bm, int nwords, Fn fn) {
     }
 }
 
-int ensure_batch(mg_graph *g, bool sssp);
+// Per-slot state of a batched search (device memory).  A slot holds one query
+// (source); slots are seeded and retired while the search keeps running.
+struct SlotCtl {
+    uint32_t assigned[8];                     // slots holding a query
+    uint32_t live[8];                         // slots with frontier pairs after the last level
+    int start[MG_BATCH_MAX];                  // BFS: level at which the slot was seeded
+    long long off[MG_BATCH_MAX];              // SSSP: bucket base when the slot was seeded
+    unsigned long long far_hi[MG_BATCH_MAX];  // SSSP: highest bucket the slot has pairs waiting in
+    long long te[MG_BATCH_MAX];               // row_stats: TEPS count of the slot's row
+    long long nr[MG_BATCH_MAX];               // row_stats: vertices reached
+};
+
+// Device state of one batched search context: the masks, the slot rows and the
+// level bookkeeping.  mg_bfs_batch / mg_sssp_batch keep one per graph; every
+// serving engine owns one.
+struct BatchDev {
+    bool sssp = false;
+    int S = 0;                 // slots (rows)
+    int KS = 0;                // SSSP: distance row stride (S rounded up to 16)
+    int G = 0;                 // workgroups of the level kernels
+    int64_t bytes = 0;         // device bytes held
+    Ctl *ctl = nullptr, *h_ctl = nullptr;        // control block, pinned host copy
+    SlotCtl *sc = nullptr, *h_sc = nullptr;      // slot state, pinned host copy
+    LevelPart *part = nullptr;                   // 3 * MAX_GRID level partials
+    uint4 *slot_or = nullptr;                    // MAX_GRID per-workgroup OR of new frontier masks
+    uint32_t *act[2] = {nullptr, nullptr};       // active-vertex bitmaps
+    uint32_t *ring = nullptr;                    // SSSP bucket ring (SB_RING bitmaps)
+    int *hard = nullptr;                         // BFS pull: long in-lists
+    ChunkRec *tasks = nullptr;
+    int64_t task_cap = 0;
+    uint4 *V = nullptr, *F = nullptr, *N = nullptr, *FAR = nullptr;
+    int *D = nullptr;                            // SSSP distances, n x KS
+    int *rows = nullptr;                         // S x n results
+    int nwords = 0;
+};
+
+// (slot, source) pairs to seed, passed by value as a kernel argument.
+struct SeedList {
+    int n;
+    int slot[MG_BATCH_MAX];
+    int src[MG_BATCH_MAX];
+};
+
+// Slot ids, passed by value as a kernel argument.
+struct SlotList {
+    int n;
+    int slot[MG_BATCH_MAX];
+};
+
+int batch_dev_alloc(BatchDev &b, mg_graph *g, bool sssp, int slots);
+void batch_dev_free(BatchDev &b);
+// Device bytes batch_dev_alloc would take.
+int64_t batch_dev_bytes(const mg_graph *g, bool sssp, int slots);
+
+// Host launchers shared by the batched entry points and the serving engines; all
+// enqueue on `st`.
+// Clears the context (no slot assigned, all rows of the first `rows_to_clear`
+// slots unreached) and sets the level limit and direction parameters.
+int bfs_batch_reset(BatchDev &b, mg_graph *g, hipStream_t st, int iter_limit, int alpha, int beta,
+                    int rows_to_clear);
+// Seeds the listed slots: their old visited bits and rows are cleared first.
+int bfs_batch_seed(BatchDev &b, mg_graph *g, hipStream_t st, const SeedList &sl);
+int bfs_batch_step(BatchDev &b, mg_graph *g, hipStream_t st);
+int sssp_batch_reset(BatchDev &b, mg_graph *g, hipStream_t st, int iter_limit, long long delta);
+// fresh: the context was just reset (distance columns already clear).
+int sssp_batch_seed(BatchDev &b, mg_graph *g, hipStream_t st, const SeedList &sl, long long delta, bool fresh);
+int sssp_batch_step(BatchDev &b, mg_graph *g, hipStream_t st, long long delta);
+// SSSP: the distance columns of the listed slots into their rows.
+int sssp_batch_extract(BatchDev &b, mg_graph *g, hipStream_t st, const SlotList &sl);
+// Per-slot OR of the current frontier masks into sc->live.
+int batch_live(BatchDev &b, hipStream_t st);
+// TEPS count and reached vertices of the listed rows into sc->te / sc->nr (a vertex
+// is reached when its entry differs from `unreached`).
+int batch_row_stats(BatchDev &b, mg_graph *g, hipStream_t st, const SlotList &sl, int unreached);
+// Clears the assigned bits of `mask`.
+int batch_release_slots(BatchDev &b, hipStream_t st, const uint32_t mask[8]);
+
+// ---- batched personalized PageRank (ppr.hip)
+constexpr int PPR_SLOTS = 64;  // slots per context: one per lane
+
+struct PprDev {
+    int n = 0, ntasks = 0, nbig = 0, npart = 0;
+    int cur = 0;                          // X[cur] holds the current ranks
+    int64_t bytes = 0;
+    float *X[2] = {nullptr, nullptr};     // ranks, n x PPR_SLOTS
+    float *C = nullptr;                   // contributions
+    float *part = nullptr;                // partial sums of long in-lists
+    float *rows = nullptr;                // PPR_SLOTS x n results
+    ChunkRec *tasks = nullptr;
+    int *big = nullptr, *src = nullptr;   // long in-lists; per-slot sources (-1 = idle)
+};
+
+int ppr_dev_alloc(PprDev &p, mg_graph *g);
+void ppr_dev_free(PprDev &p);
+int64_t ppr_dev_bytes(const mg_graph *g);
+// Seeds the listed slots (rank column = e_src) after retiring the `retire` slots.
+int ppr_seed(PprDev &p, hipStream_t st, const SeedList &sl, const SlotList &retire);
+int ppr_iterate(PprDev &p, mg_graph *g, hipStream_t st, float teleport, int iters);
+int ppr_extract_rows(PprDev &p, hipStream_t st, const SlotList &sl);
+int ppr_iters_of(const mg_options &o);
+
 int check_batch_sources(const char *fn, const mg_graph *g, const int32_t *srcs, int32_t k);
+// The graph's own batch context (allocated once at MG_BATCH_MAX slots).
+int graph_batch_dev(mg_graph *g, bool sssp, BatchDev **out);
+// Ends a batched call: device time, overflow check, stats, result delivery.
+int finish_batch(mg_graph *g, BatchDev &b, mg_stats *stats, void *out, size_t bytes, bool out_on_device,
+                 const CallTimer &timer);
 
 }  // namespace mg

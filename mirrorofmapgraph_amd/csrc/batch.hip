@@ -1,39 +1,65 @@
-// Batched multi-source BFS: up to 256 sources share every edge read.
+// Batched multi-source BFS: up to 256 sources (slots) share every edge read.
 //
 // The reference runs one search per source, re-initialising between them
 // (Algorithms/BFS/bfs.cu:402-417).  Here each vertex carries three 256-bit masks
-// (one 32-byte sector each; bit s = source s): V (visited), F (frontier), N (next).
+// (one 32-byte sector each; bit s = slot s): V (visited), F (frontier), N (next).
 // One level is
 //   * push (sparse levels): every active vertex v (F[v] != 0, listed in the active
 //     bitmap A[L%2]) ORs F[v] & ~V[u] into N[u] for each out-neighbour u (atomicOr on
 //     the words that gain bits) and marks u in A[(L+1)%2]; lists longer than LANE_T
 //     become TASK-edge tasks walked by one wave;
-//   * pull (dense levels, needs in-edges): every vertex u whose mask is not yet full
-//     ORs the frontier masks of its in-neighbours, stopping once every source has
-//     reached it, and stores the new bits in N[u] (no atomics);
+//   * pull (dense levels, needs in-edges): every vertex u not yet reached by every
+//     assigned slot ORs the frontier masks of its in-neighbours, stopping once every
+//     slot has reached it, and stores the new bits in N[u] (no atomics);
 //   * finalize: F[v] = 0 for the old frontier, then for every marked u: new = N[u],
-//     V[u] |= new, F[u] = new, N[u] = 0, and depth[s][u] = L + 1 for each new bit s.
-// Depths are unique per source, so row s equals a single-source search from
-// srcs[s].  Level close, direction choice and termination run on the device in the
-// next step's single-workgroup kernel; steps are replayed from a captured HIP graph.
+//     V[u] |= new, F[u] = new, N[u] = 0, and row[s][u] = L + 1 - start[s] for each
+//     new bit s (start[s] = the level the slot was seeded at).
+// Depths are unique per source, so row s equals a single-source search from the
+// slot's source, whatever other slots run beside it and whenever it was seeded.
+// Level close, direction choice and termination run on the device in the next
+// step's single-workgroup kernel; steps are replayed from a captured HIP graph.
+// Slots can be seeded between steps (serving): a new slot joins the next level's
+// frontier; a slot whose bit left every frontier mask is finished.
 #include "batch.h"
 
 namespace mg {
 
 struct BatchArgs {
     Ctl *ctl;
+    SlotCtl *sc;
     const int *off, *adj;        // push view (CSR)
     const int *in_off, *in_adj;  // pull view (in-edges), nullptr when none
     uint4 *V, *F, *N;            // 2 uint4 per vertex
     uint32_t *act[2];            // active-vertex bitmaps
-    int *out;                    // k x n depths
+    int *out;                    // S x n depths
     int *hard;                   // pull: vertices handed to a wave
     ChunkRec *tasks;
     int64_t task_cap;
     LevelPart *part;
-    M8 full;                     // bits of the k sources
-    int n, nwords, k, G;
+    uint4 *slot_or;              // per-workgroup OR of new frontier masks (2 uint4 each)
+    int n, nwords, G;
 };
+
+__device__ __forceinline__ M8 zero8() { return M8{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)}; }
+
+// Block-wide OR of a mask; the result is valid in thread 0.
+template <int BLOCK>
+__device__ M8 block_or8(M8 x) {
+    __shared__ uint4 s_or[2 * (BLOCK / WAVE)];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = or8(x, shfl_xor8(x, o));
+    const int wv = (int)threadIdx.x / WAVE;
+    if (lane_id() == 0) {
+        s_or[2 * wv] = x.a;
+        s_or[2 * wv + 1] = x.b;
+    }
+    __syncthreads();
+    M8 r = zero8();
+    if (threadIdx.x == 0)
+        for (int i = 0; i < BLOCK / WAVE; ++i) r = or8(r, M8{s_or[2 * i], s_or[2 * i + 1]});
+    __syncthreads();
+    return r;
+}
 
 // push: v's frontier bits that u has not seen yet go to N[u]
 __device__ __forceinline__ void push_edge(const BatchArgs &a, const M8 &fv, int u, uint32_t *act_next) {
@@ -49,23 +75,26 @@ __device__ __forceinline__ void push_edge(const BatchArgs &a, const M8 &fv, int 
     if (!(act_next[u >> 5] & bit)) atomicOr(&act_next[u >> 5], bit);
 }
 
-__global__ void batch_init(BatchArgs a, int iter_limit, int can_pull, int alpha, int beta, long long m) {
-    const int64_t kn = (int64_t)a.k * a.n;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < kn; i += (int64_t)gridDim.x * blockDim.x)
-        a.out[i] = -1;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 2 * (int64_t)a.n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+__global__ void batch_init(BatchArgs a, int iter_limit, int can_pull, int alpha, int beta, long long m,
+                           int rows_to_clear) {
+    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, nt = (int64_t)gridDim.x * blockDim.x;
+    const int64_t kn = (int64_t)rows_to_clear * a.n;
+    for (int64_t i = tid; i < kn; i += nt) a.out[i] = -1;
+    for (int64_t i = tid; i < 2 * (int64_t)a.n; i += nt) {
         a.V[i] = make_uint4(0, 0, 0, 0);
         a.F[i] = make_uint4(0, 0, 0, 0);
         a.N[i] = make_uint4(0, 0, 0, 0);
     }
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.nwords; i += gridDim.x * blockDim.x) {
+    for (int64_t i = tid; i < a.nwords; i += nt) {
         a.act[0][i] = 0;
         a.act[1][i] = 0;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    int *sci = reinterpret_cast<int *>(a.sc);
+    for (int64_t i = tid; i < (int64_t)(sizeof(SlotCtl) / 4); i += nt) sci[i] = 0;
+    for (int64_t i = tid; i < 2 * MAX_GRID; i += nt) a.slot_or[i] = make_uint4(0, 0, 0, 0);
+    if (tid == 0) {
         Ctl *c = a.ctl;
-        ctl_reset(c, a.n, iter_limit, a.k);
+        ctl_reset(c, a.n, iter_limit, 0);  // idle until a slot is seeded
         c->can_bu = can_pull;
         c->m_unexplored = m;  // pull once the frontier's out-edges exceed m / alpha
         c->alpha = alpha;
@@ -73,20 +102,44 @@ __global__ void batch_init(BatchArgs a, int iter_limit, int can_pull, int alpha,
     }
 }
 
-// One thread per source: visited/frontier bit, depth 0, and the source's out-degree
-// toward the traversed-edge count.
-__global__ void batch_seed(BatchArgs a, const int *srcs) {
-    const int s = threadIdx.x;
-    if (s >= a.k) return;
-    const int v = srcs[s];
-    uint32_t *vv = reinterpret_cast<uint32_t *>(a.V + 2 * v);
-    uint32_t *fv = reinterpret_cast<uint32_t *>(a.F + 2 * v);
-    atomicOr(&vv[s >> 5], 1u << (s & 31));
-    atomicOr(&fv[s >> 5], 1u << (s & 31));
-    atomicOr(&a.act[0][v >> 5], 1u << (v & 31));
-    a.out[(int64_t)s * a.n + v] = 0;
-    atomicAdd((unsigned long long *)&a.ctl->edges_traversed, (unsigned long long)(a.off[v + 1] - a.off[v]));
-    atomicAdd((unsigned long long *)&a.ctl->reached, 1ULL);
+// Clears the visited bits and rows of the slots about to be seeded.
+__global__ void batch_prepare(BatchArgs a, M8 newmask, SlotList sl) {
+    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, nt = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t u = tid; u < a.n; u += nt) {
+        const M8 v = ld8(a.V, (int)u);
+        const M8 k = andnot8(v, newmask);
+        if (any8(andnot8(v, k))) st8(a.V, (int)u, k);
+        for (int j = 0; j < sl.n; ++j) a.out[(int64_t)sl.slot[j] * a.n + u] = -1;
+    }
+}
+
+// One thread per (slot, source) pair.  A slot seeded while a level is pending
+// joins the next level's frontier; seeded into an idle context it starts the search.
+__global__ void batch_seed(BatchArgs a, SeedList sl) {
+    Ctl *c = a.ctl;
+    const int i = threadIdx.x;
+    const int pending = c->pending;
+    const int Lf = pending ? c->level + 1 : c->level;
+    if (i < sl.n) {
+        const int s = sl.slot[i], v = sl.src[i];
+        const uint32_t sb = 1u << (s & 31);
+        atomicOr(reinterpret_cast<uint32_t *>(a.V + 2 * v) + (s >> 5), sb);
+        atomicOr(reinterpret_cast<uint32_t *>(a.F + 2 * v) + (s >> 5), sb);
+        atomicOr(&a.act[Lf & 1][v >> 5], 1u << (v & 31));
+        atomicOr(&a.sc->assigned[s >> 5], sb);
+        atomicOr(&a.sc->live[s >> 5], sb);
+        a.sc->start[s] = Lf;
+        a.sc->te[s] = 0;
+        a.sc->nr[s] = 0;
+        a.out[(int64_t)s * a.n + v] = 0;
+        atomicAdd((unsigned long long *)&c->edges_traversed, (unsigned long long)(a.off[v + 1] - a.off[v]));
+        atomicAdd((unsigned long long *)&c->reached, 1ULL);
+    }
+    __syncthreads();
+    if (i == 0 && sl.n > 0) {
+        if (pending) atomicAdd((unsigned long long *)&a.part[0].nf, (unsigned long long)sl.n);
+        else c->done = c->iter_limit <= 0;
+    }
 }
 
 // One workgroup: closes the pending level (partials of batch_fin_b).
@@ -163,26 +216,27 @@ __global__ __launch_bounds__(BLK) void batch_a(BatchArgs a) {
         __shared__ int s_w[BLK / WAVE];
         __shared__ int s_hb;
         const int lane = lane_id();
+        const M8 full = ld8(reinterpret_cast<const uint4 *>(a.sc->assigned), 0);
         for (int base = blockIdx.x * BLK; base < a.n; base += gridDim.x * BLK) {
             const int u = base + (int)threadIdx.x;
             bool found = false, hard = false;
             if (u < a.n) {
                 const M8 vu = ld8(a.V, u);
-                if (any8(andnot8(a.full, vu))) {
-                    M8 acc{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
+                if (any8(andnot8(full, vu))) {
+                    M8 acc = zero8();
                     const int beg = a.in_off[u], end = a.in_off[u + 1];
                     const int lim = min(end, beg + BU_INLINE);
                     int e = beg;
                     for (; e < lim; ++e) {
                         acc = or8(acc, ld8(a.F, a.in_adj[e]));
-                        if (!any8(andnot8(a.full, or8(acc, vu)))) break;
+                        if (!any8(andnot8(full, or8(acc, vu)))) break;
                     }
                     const M8 nw = andnot8(acc, vu);
                     if (any8(nw)) {
                         st8(a.N, u, nw);
                         found = true;
                     }
-                    hard = e == lim && lim < end && any8(andnot8(a.full, or8(acc, vu)));
+                    hard = e == lim && lim < end && any8(andnot8(full, or8(acc, vu)));
                 }
             }
             const uint64_t fm = wave_ballot(found);
@@ -220,7 +274,7 @@ __global__ __launch_bounds__(BLK) void batch_b(BatchArgs a) {
             const int u = a.hard[h];
             const M8 vu = ld8(a.V, u);
             const M8 prev = ld8(a.N, u);  // bits found by the lane scan
-            M8 acc{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
+            M8 acc = zero8();
             const int end = a.in_off[u + 1];
             for (int e = a.in_off[u] + BU_INLINE + lane; e < end; e += WAVE) acc = or8(acc, ld8(a.F, a.in_adj[e]));
 #pragma unroll
@@ -243,7 +297,7 @@ __global__ __launch_bounds__(BLK) void batch_fin_a(BatchArgs a) {
     if (c->done) return;
     const int L = c->level;
     for_each_bit(a.act[L & 1], a.nwords, [&](int v) {
-        if (v >= 0) st8(a.F, v, M8{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)});
+        if (v >= 0) st8(a.F, v, zero8());
     });
 }
 
@@ -253,15 +307,17 @@ __global__ __launch_bounds__(BLK) void batch_fin_b(BatchArgs a) {
     if (c->done) return;
     const int L = c->level;
     long long nf = 0, mf = 0, te = 0, nr = 0;
+    M8 lv = zero8();
     // the old active bitmap becomes the next level's (empty) target
     uint32_t *old = a.act[L & 1];
     for (int i = blockIdx.x * BLK + (int)threadIdx.x; i < a.nwords; i += gridDim.x * BLK) old[i] = 0;
     for_each_bit(a.act[(L + 1) & 1], a.nwords, [&](int u) {
         if (u < 0) return;
         const M8 nw = ld8(a.N, u);
-        st8(a.N, u, M8{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)});
+        st8(a.N, u, zero8());
         st8(a.V, u, or8(ld8(a.V, u), nw));
         st8(a.F, u, nw);
+        lv = or8(lv, nw);
         const int deg = a.off[u + 1] - a.off[u];
         const int cnt = popc8(nw);
         ++nf;
@@ -272,7 +328,7 @@ __global__ __launch_bounds__(BLK) void batch_fin_b(BatchArgs a) {
             uint32_t b = word8(nw, w);
             while (b) {
                 const int s = w * 32 + __ffs(b) - 1;
-                a.out[(int64_t)s * a.n + u] = L + 1;
+                a.out[(int64_t)s * a.n + u] = L + 1 - a.sc->start[s];
                 b &= b - 1;
             }
         }
@@ -281,6 +337,7 @@ __global__ __launch_bounds__(BLK) void batch_fin_b(BatchArgs a) {
     mf = block_sum<BLK>(mf);
     te = block_sum<BLK>(te);
     nr = block_sum<BLK>(nr);
+    lv = block_or8<BLK>(lv);
     if (threadIdx.x == 0) {
         LevelPart p;
         p.nf = nf;
@@ -288,61 +345,250 @@ __global__ __launch_bounds__(BLK) void batch_fin_b(BatchArgs a) {
         p.ex = te;
         p.pad = nr;  // (source, vertex) pairs reached
         a.part[blockIdx.x] = p;
+        st8(a.slot_or, blockIdx.x, lv);
         if (blockIdx.x == 0) c->pending = 1;
     }
 }
 
-void batch_release(mg_graph *g) {
-    BatchState &bs = g->steps->batch;
-    free_dev(bs.V);
-    free_dev(bs.F);
-    free_dev(bs.N);
-    free_dev(bs.out);
-    free_dev(bs.srcs);
-    free_dev(bs.D);
-    free_dev(bs.tasks);
-    free_dev(bs.FAR);
-    free_dev(bs.fb);
-    bs = BatchState{};
+// ---------------------------------------------------------------- slot utilities
+
+// One workgroup: ORs the per-workgroup frontier masks written by the last
+// finalize into sc->live.  Run right after a run of steps, before any seeding.
+__global__ __launch_bounds__(SMALL_BLK) void k_batch_live(Ctl *c, SlotCtl *sc, const uint4 *slot_or, int G) {
+    (void)c;
+    M8 x = zero8();
+    for (int i = threadIdx.x; i < G; i += SMALL_BLK) x = or8(x, ld8(slot_or, i));
+    x = block_or8<SMALL_BLK>(x);
+    if (threadIdx.x == 0) st8(reinterpret_cast<uint4 *>(sc->live), 0, x);
 }
 
-// Allocates the batch state of graph g once, at the maximum batch size, so that
-// captured step graphs keep valid pointers for the graph's lifetime.
-int ensure_batch(mg_graph *g, bool sssp) {
-    BatchState &bs = g->steps->batch;
+// Grid (ns, B): workgroup (j, b) counts a stripe of row slot[j]: vertices reached
+// and their out-degrees, one atomic per workgroup.
+__global__ __launch_bounds__(BLK) void k_row_stats(const int *rows, const int *off, int n, SlotList sl, int unreached,
+                                                    SlotCtl *sc) {
+    const int s = sl.slot[blockIdx.y];
+    const int *row = rows + (int64_t)s * n;
+    long long te = 0, nr = 0;
+    for (int u = blockIdx.x * BLK + (int)threadIdx.x; u < n; u += gridDim.x * BLK)
+        if (row[u] != unreached) {
+            ++nr;
+            te += off[u + 1] - off[u];
+        }
+    te = block_sum<BLK>(te);
+    nr = block_sum<BLK>(nr);
+    if (threadIdx.x == 0) {
+        atomicAdd((unsigned long long *)&sc->te[s], (unsigned long long)te);
+        atomicAdd((unsigned long long *)&sc->nr[s], (unsigned long long)nr);
+    }
+}
+
+__global__ void k_release_slots(SlotCtl *sc, M8 mask) {
+    const int w = threadIdx.x;
+    if (w < 8) {
+        sc->assigned[w] &= ~word8(mask, w);
+        sc->live[w] &= ~word8(mask, w);
+    }
+}
+
+static M8 host_mask(const uint32_t m[8]) {
+    return M8{make_uint4(m[0], m[1], m[2], m[3]), make_uint4(m[4], m[5], m[6], m[7])};
+}
+
+static BatchArgs bfs_args(BatchDev &b, mg_graph *g) {
+    BatchArgs a{};
+    a.ctl = b.ctl;
+    a.sc = b.sc;
+    a.off = g->row_off;
+    a.adj = g->col;
+    if (g->flags & MG_GRAPH_SYMMETRIC) {
+        a.in_off = g->row_off;
+        a.in_adj = g->col;
+    } else if (g->col_off) {
+        a.in_off = g->col_off;
+        a.in_adj = g->row_idx;
+    }
+    a.V = b.V;
+    a.F = b.F;
+    a.N = b.N;
+    a.act[0] = b.act[0];
+    a.act[1] = b.act[1];
+    a.out = b.rows;
+    a.hard = b.hard;
+    a.tasks = b.tasks;
+    a.task_cap = b.task_cap;
+    a.part = b.part;
+    a.slot_or = b.slot_or;
+    a.n = g->n;
+    a.nwords = b.nwords;
+    a.G = b.G;
+    return a;
+}
+
+int bfs_batch_reset(BatchDev &b, mg_graph *g, hipStream_t st, int iter_limit, int alpha, int beta,
+                    int rows_to_clear) {
+    BatchArgs a = bfs_args(b, g);
+    const int can_pull = a.in_off != nullptr;
+    hipLaunchKernelGGL(batch_init, dim3(grid_blocks(std::max<int64_t>((int64_t)rows_to_clear * g->n, 2 * g->n), BLK)),
+                       dim3(BLK), 0, st, a, iter_limit, can_pull, alpha, beta, (long long)g->m, rows_to_clear);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+int bfs_batch_seed(BatchDev &b, mg_graph *g, hipStream_t st, const SeedList &sl) {
+    if (sl.n <= 0) return MG_OK;
+    BatchArgs a = bfs_args(b, g);
+    uint32_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    SlotList ls{};
+    ls.n = sl.n;
+    for (int i = 0; i < sl.n; ++i) {
+        m[sl.slot[i] >> 5] |= 1u << (sl.slot[i] & 31);
+        ls.slot[i] = sl.slot[i];
+    }
+    hipLaunchKernelGGL(batch_prepare, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, st, a, host_mask(m), ls);
+    hipLaunchKernelGGL(batch_seed, dim3(1), dim3(MG_BATCH_MAX), 0, st, a, sl);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+int bfs_batch_step(BatchDev &b, mg_graph *g, hipStream_t st) {
+    BatchArgs a = bfs_args(b, g);
+    hipLaunchKernelGGL(batch_close, dim3(1), dim3(SMALL_BLK), 0, st, a);
+    hipLaunchKernelGGL(batch_a, dim3(a.G), dim3(BLK), 0, st, a);
+    hipLaunchKernelGGL(batch_b, dim3(a.G), dim3(BLK), 0, st, a);
+    hipLaunchKernelGGL(batch_fin_a, dim3(a.G), dim3(BLK), 0, st, a);
+    hipLaunchKernelGGL(batch_fin_b, dim3(a.G), dim3(BLK), 0, st, a);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+int batch_live(BatchDev &b, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch_live, dim3(1), dim3(SMALL_BLK), 0, st, b.ctl, b.sc, (const uint4 *)b.slot_or, b.G);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+int batch_row_stats(BatchDev &b, mg_graph *g, hipStream_t st, const SlotList &sl, int unreached) {
+    if (sl.n <= 0) return MG_OK;
+    const int bx = std::max(1, std::min(64, grid_blocks(g->n, BLK)));
+    hipLaunchKernelGGL(k_row_stats, dim3(bx, sl.n), dim3(BLK), 0, st, (const int *)b.rows, (const int *)g->row_off,
+                       g->n, sl, unreached, b.sc);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+int batch_release_slots(BatchDev &b, hipStream_t st, const uint32_t mask[8]) {
+    hipLaunchKernelGGL(k_release_slots, dim3(1), dim3(64), 0, st, b.sc, host_mask(mask));
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+// ---------------------------------------------------------------- device state
+
+int64_t batch_dev_bytes(const mg_graph *g, bool sssp, int slots) {
+    const int64_t n = g->n, nwords = ((n + 63) / 64) * 2;
+    const int KS = (slots + 15) / 16 * 16;
+    int64_t b = (int64_t)sizeof(Ctl) + sizeof(SlotCtl) + 3LL * MAX_GRID * sizeof(LevelPart) + 32LL * MAX_GRID;
+    b += 2 * 4 * nwords + 3 * (32 * n + 32) + 4LL * slots * n + 4;
+    if (sssp) {
+        b += (32 * n + 32) + 4LL * SB_RING * nwords + 4LL * KS * n + 64;
+        b += (int64_t)sizeof(ChunkRec) * (g->m / SB_TASK + n + 64);
+    } else {
+        b += 4 * n + 4 + (int64_t)sizeof(ChunkRec) * (2 * (g->m / TASK) + 64);
+    }
+    return b;
+}
+
+void batch_dev_free(BatchDev &b) {
+    free_dev(b.ctl);
+    free_dev(b.sc);
+    free_dev(b.part);
+    free_dev(b.slot_or);
+    free_dev(b.act[0]);
+    free_dev(b.act[1]);
+    free_dev(b.ring);
+    free_dev(b.hard);
+    free_dev(b.tasks);
+    free_dev(b.V);
+    free_dev(b.F);
+    free_dev(b.N);
+    free_dev(b.FAR);
+    free_dev(b.D);
+    free_dev(b.rows);
+    if (b.h_ctl) (void)hipHostFree(b.h_ctl);
+    if (b.h_sc) (void)hipHostFree(b.h_sc);
+    b = BatchDev{};
+}
+
+int batch_dev_alloc(BatchDev &b, mg_graph *g, bool sssp, int slots) {
+    if (slots < 1 || slots > MG_BATCH_MAX) return set_error(MG_ERR_INVALID_ARG, "batch slots must be in [1, %d]", MG_BATCH_MAX);
+    batch_dev_free(b);
     const size_t n = (size_t)g->n;
+    b.sssp = sssp;
+    b.S = slots;
+    b.KS = (slots + 15) / 16 * 16;
+    b.nwords = (int)(((n + 63) / 64) * 2);
+    b.G = sssp ? grid_blocks(std::max<int64_t>(g->n / 4, g->m / 16), BLK) : grid_blocks(std::max<int64_t>(g->n / 4, g->m / 64), BLK);
+    b.task_cap = sssp ? g->m / SB_TASK + (int64_t)n + 64 : 2 * (g->m / TASK) + 64;
     hipError_t e = hipSuccess;
-    if (!bs.V) {
-        e = hipMalloc(&bs.V, 32 * n + 32);
-        if (e == hipSuccess) e = hipMalloc(&bs.F, 32 * n + 32);
-        if (e == hipSuccess) e = hipMalloc(&bs.N, 32 * n + 32);
-        if (e == hipSuccess) e = hipMalloc((void **)&bs.out, 4 * (size_t)MG_BATCH_MAX * n + 4);
-        if (e == hipSuccess) e = hipMalloc((void **)&bs.srcs, 4 * MG_BATCH_MAX);
-        if (e != hipSuccess) {
-            batch_release(g);
-            return set_error(MG_ERR_OOM, "batch state allocation failed: %s", hipGetErrorString(e));
-        }
-        bs.n = g->n;
-        bs.out_cap = (size_t)MG_BATCH_MAX * n;
+    auto A = [&](void **p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, bytes ? bytes : 16);
+    };
+    A((void **)&b.ctl, sizeof(Ctl));
+    A((void **)&b.sc, sizeof(SlotCtl));
+    A((void **)&b.part, 3 * MAX_GRID * sizeof(LevelPart));
+    A((void **)&b.slot_or, 32 * MAX_GRID);
+    A((void **)&b.act[0], 4 * (size_t)b.nwords);
+    A((void **)&b.act[1], 4 * (size_t)b.nwords);
+    A((void **)&b.tasks, sizeof(ChunkRec) * (size_t)b.task_cap);
+    A((void **)&b.F, 32 * n + 32);
+    A((void **)&b.N, 32 * n + 32);
+    A((void **)&b.rows, 4 * (size_t)slots * n + 4);
+    if (sssp) {
+        A((void **)&b.FAR, 32 * n + 32);
+        A((void **)&b.ring, 4 * (size_t)SB_RING * b.nwords);
+        A((void **)&b.D, 4 * (size_t)b.KS * n + 64);
+        A((void **)&b.V, 32);  // unused by SSSP
+    } else {
+        A((void **)&b.V, 32 * n + 32);
+        A((void **)&b.hard, 4 * n + 4);
     }
-    if (sssp && !bs.D) {
-        bs.task_cap = g->m / SB_TASK + (int64_t)n + 64;
-        e = hipMalloc((void **)&bs.D, 4 * (size_t)MG_BATCH_MAX * n + 64);
-        if (e == hipSuccess) e = hipMalloc((void **)&bs.tasks, sizeof(ChunkRec) * (size_t)bs.task_cap);
-        if (e == hipSuccess) e = hipMalloc(&bs.FAR, 32 * n + 32);
-        if (e == hipSuccess) e = hipMalloc((void **)&bs.fb, 4 * (size_t)SB_RING * g->ws.nwords);
-        if (e != hipSuccess) {
-            free_dev(bs.D);
-            free_dev(bs.tasks);
-            free_dev(bs.FAR);
-            free_dev(bs.fb);
-            bs.D = nullptr;
-            bs.tasks = nullptr;
-            bs.FAR = nullptr;
-            bs.fb = nullptr;
-            return set_error(MG_ERR_OOM, "batched SSSP state allocation failed: %s", hipGetErrorString(e));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&b.h_ctl, sizeof(Ctl), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&b.h_sc, sizeof(SlotCtl), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        batch_dev_free(b);
+        return set_error(MG_ERR_OOM, "batch state allocation (%d slots) failed: %s", slots, hipGetErrorString(e));
+    }
+    b.bytes = batch_dev_bytes(g, sssp, slots);
+    return MG_OK;
+}
+
+int64_t batch_dev_size(const BatchDev *b) { return b->bytes; }
+
+void batch_release(mg_graph *g) {
+    for (BatchDev *&p : g->steps->bdev) {
+        if (p) {
+            batch_dev_free(*p);
+            delete p;
+            p = nullptr;
         }
     }
+}
+
+// The graph's own batch context for kind 0 (BFS) or 1 (SSSP), allocated once at
+// MG_BATCH_MAX slots so captured step graphs keep valid pointers.
+int graph_batch_dev(mg_graph *g, bool sssp, BatchDev **out) {
+    BatchDev *&p = g->steps->bdev[sssp ? 1 : 0];
+    if (!p) {
+        p = new (std::nothrow) BatchDev();
+        if (!p) return set_error(MG_ERR_OOM, "host allocation failed");
+        int rc = batch_dev_alloc(*p, g, sssp, MG_BATCH_MAX);
+        if (rc) {
+            delete p;
+            p = nullptr;
+            return rc;
+        }
+    }
+    *out = p;
     return MG_OK;
 }
 
@@ -352,6 +598,30 @@ int check_batch_sources(const char *fn, const mg_graph *g, const int32_t *srcs, 
     for (int i = 0; i < k; ++i)
         if (srcs[i] < 0 || srcs[i] >= g->n)
             return set_error(MG_ERR_INVALID_ARG, "%s: source %d = %d outside [0,%d)", fn, i, srcs[i], g->n);
+    return MG_OK;
+}
+
+int finish_batch(mg_graph *g, BatchDev &b, mg_stats *stats, void *out, size_t bytes, bool out_on_device,
+                 const CallTimer &timer) {
+    MG_HIP(hipEventRecord(g->ev1, g->stream));
+    MG_HIP(hipEventSynchronize(g->ev1));
+    float ms = 0;
+    MG_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+    MG_HIP(hipMemcpy(b.h_ctl, b.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    if (b.h_ctl->changed & (1 << 30))
+        return set_error(MG_ERR_OVERFLOW, "batched task queue overflow (capacity %lld)", (long long)b.task_cap);
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->iterations = b.h_ctl->level;
+        stats->bottom_up_levels = b.h_ctl->bu_levels;
+        stats->edges_examined = b.h_ctl->edges_examined;
+        stats->edges_traversed = b.h_ctl->edges_traversed;
+        stats->vertices_reached = b.h_ctl->reached;
+        stats->device_ms = ms;
+    }
+    int rc = deliver(g, out, b.rows, bytes, out_on_device);
+    if (rc) return rc;
+    if (stats) stats->host_ms = timer.ms();
     return MG_OK;
 }
 
@@ -372,64 +642,20 @@ extern "C" int mg_bfs_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t
     if ((rc = check_batch_sources("mg_bfs_batch", g, srcs, k))) return rc;
     if (o.directed == 0 && !(g->flags & MG_GRAPH_SYMMETRIC))
         return set_error(MG_ERR_UNSUPPORTED, "mg_bfs_batch: directed=0 needs a symmetric graph");
-    if ((rc = ensure_batch(g, false))) return rc;
-    Workspace &ws = g->ws;
-    const int n = g->n;
-    BatchState &bs = g->steps->batch;
-    BatchArgs a{};
-    a.ctl = ws.ctl;
-    a.off = g->row_off;
-    a.adj = g->col;
-    if (g->flags & MG_GRAPH_SYMMETRIC) {
-        a.in_off = g->row_off;
-        a.in_adj = g->col;
-    } else if (g->col_off) {
-        a.in_off = g->col_off;
-        a.in_adj = g->row_idx;
+    BatchDev *b = nullptr;
+    if ((rc = graph_batch_dev(g, false, &b))) return rc;
+    SeedList sl{};
+    sl.n = k;
+    for (int s = 0; s < k; ++s) {
+        sl.slot[s] = s;
+        sl.src[s] = srcs[s];
     }
-    a.V = reinterpret_cast<uint4 *>(bs.V);
-    a.F = reinterpret_cast<uint4 *>(bs.F);
-    a.N = reinterpret_cast<uint4 *>(bs.N);
-    a.act[0] = ws.bm[0];
-    a.act[1] = ws.bm[1];
-    a.out = bs.out;
-    a.hard = ws.hard;
-    a.tasks = ws.chunks;
-    a.task_cap = ws.chunk_cap;
-    a.part = ws.part;
-    uint32_t fw[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int s = 0; s < k; ++s) fw[s >> 5] |= 1u << (s & 31);
-    a.full = M8{make_uint4(fw[0], fw[1], fw[2], fw[3]), make_uint4(fw[4], fw[5], fw[6], fw[7])};
-    a.n = n;
-    a.nwords = ws.nwords;
-    a.k = k;
-    a.G = grid_blocks(std::max<int64_t>(n / 4, g->m / 64), BLK);
-    const int can_pull = a.in_off != nullptr;
-
-    MG_HIP(hipMemcpyAsync(bs.srcs, srcs, 4 * (size_t)k, hipMemcpyHostToDevice, g->stream));
     MG_HIP(hipEventRecord(g->ev0, g->stream));
-    hipLaunchKernelGGL(batch_init, dim3(grid_blocks((int64_t)k * n, BLK)), dim3(BLK), 0, g->stream, a, o.iter_num,
-                       can_pull, batch_alpha(o), batch_beta(o), (long long)g->m);
-    hipLaunchKernelGGL(batch_seed, dim3(1), dim3(MG_BATCH_MAX), 0, g->stream, a, (const int *)bs.srcs);
-    MG_HIP(hipGetLastError());
-    auto step = [&]() -> int {
-        hipLaunchKernelGGL(batch_close, dim3(1), dim3(SMALL_BLK), 0, g->stream, a);
-        hipLaunchKernelGGL(batch_a, dim3(a.G), dim3(BLK), 0, g->stream, a);
-        hipLaunchKernelGGL(batch_b, dim3(a.G), dim3(BLK), 0, g->stream, a);
-        hipLaunchKernelGGL(batch_fin_a, dim3(a.G), dim3(BLK), 0, g->stream, a);
-        hipLaunchKernelGGL(batch_fin_b, dim3(a.G), dim3(BLK), 0, g->stream, a);
-        MG_HIP(hipGetLastError());
-        return MG_OK;
-    };
-    // the captured steps bake in k and the full mask
-    const uint64_t key = mix64((uint64_t)k * 2 + (uint64_t)o.directed);
-    const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)n) + 3;
-    if ((rc = drive(g, step, max_steps, KIND_BATCH, key))) return rc;
-    rc = finish_search(g, stats, nullptr, 0, depth_out, bs.out, 4 * (size_t)k * n, o.out_on_device, timer);
-    if (rc) return rc;
-    if (stats) {
-        stats->edges_traversed = ws.h_ctl->edges_traversed;
-        stats->vertices_reached = ws.h_ctl->reached;
-    }
-    return MG_OK;
+    if ((rc = bfs_batch_reset(*b, g, g->stream, o.iter_num, batch_alpha(o), batch_beta(o), 0))) return rc;
+    if ((rc = bfs_batch_seed(*b, g, g->stream, sl))) return rc;
+    auto step = [&]() -> int { return bfs_batch_step(*b, g, g->stream); };
+    // the step graph reads everything from device memory: one capture per graph
+    const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n) + 3;
+    if ((rc = drive(g, step, max_steps, KIND_BATCH, 0, b->ctl, b->h_ctl))) return rc;
+    return finish_batch(g, *b, stats, depth_out, 4 * (size_t)k * g->n, o.out_on_device, timer);
 }

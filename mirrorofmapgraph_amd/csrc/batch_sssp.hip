@@ -1,28 +1,32 @@
-// Batched multi-source SSSP: up to 256 sources relax together over shared edge reads.
+// Batched multi-source SSSP: up to 256 sources (slots) relax together over shared
+// edge reads.
 //
 // The reference runs one search per source (Algorithms/SSSP/sssp.cu:310-428).  Here the
-// distances are vertex-major, D[v * KS + s] (KS = k rounded up to 16, one 64-byte
-// line per 16 sources), so one wave relaxes one edge for all sources at once: lane l
-// owns sources 4l..4l+3 and moves 16 contiguous bytes of D[u].  Each vertex carries
+// distances are vertex-major, D[v * KS + s] (KS = slots rounded up to 16, one 64-byte
+// line per 16 slots), so one wave relaxes one edge for all slots at once: lane l
+// owns slots 4l..4l+3 and moves 16 contiguous bytes of D[u].  Each vertex carries
 // 256-bit masks F (pairs (v, s) to expand this iteration), N (pairs improved this
 // iteration) and FAR (pairs waiting in a later bucket).
 //
-// Buckets (delta-stepping, width delta shared by all sources): only distances below
-// the bound T = (b + 1) * delta of the current bucket b are expanded.  One iteration:
+// Buckets (delta-stepping, width delta): slot s compares the virtual distance
+// D + off[s] (off = the bucket base when the slot was seeded) with the bound
+// T = (b + 1) * delta of the current bucket b, so every slot runs its own bucket
+// sequence on a shared clock.  One iteration:
 //   * tasks: every active vertex (bitmap A[L%2]) is cut into SB_TASK-edge tasks;
 //   * relax: one wave per task; for each edge (v, u, w) and each s in F[v]:
-//     c = D[v][s] + w; if c < D[u][s]: atomicMin(D[u][s], c), then if c < T the pair
-//     joins N[u] (expanded next iteration) else FAR[u], with u marked in the ring
-//     bitmap of bucket c / delta;
+//     c = D[v][s] + w; if c < D[u][s]: atomicMin(D[u][s], c), then if c + off[s] < T
+//     the pair joins N[u] (expanded next iteration) else FAR[u], with u marked in
+//     the ring bitmap of bucket (c + off[s]) / delta;
 //   * finalize: F = 0 on the old frontier, F[u] = N[u] on the new one.
 // When an iteration leaves no pair in N, the next bucket's ring bitmap is scanned
 // and its waiting pairs move from FAR to F.  Every improvement is at most
 // max_weight above T, so a ring of SB_RING bitmaps never wraps onto a live bucket
 // when delta >= max_weight / (SB_RING - 2).  Any such schedule is label-correcting:
-// every source converges to its unique shortest distances, so row s equals mg_sssp
-// from srcs[s].  A distance read while another wave lowers it is only stale-high:
-// the later drop marks the pair again.  At the end D is transposed into the k x n
-// result.
+// every slot converges to its unique shortest distances, so row s equals mg_sssp
+// from the slot's source whatever runs beside it.  A distance read while another
+// wave lowers it is only stale-high: the later drop marks the pair again.  Rows
+// are transposed out of D when a slot finishes (no frontier pair and no bucket
+// above the current one holding its pairs).
 #include "batch.h"
 
 #include <climits>
@@ -32,6 +36,7 @@ namespace mg {
 
 struct SBatchArgs {
     Ctl *ctl;
+    SlotCtl *sc;
     const int *off, *adj, *w;
     int *D;                // n x KS
     uint4 *F, *N, *FAR;    // pair masks, 2 uint4 per vertex
@@ -40,10 +45,13 @@ struct SBatchArgs {
     ChunkRec *tasks;
     int64_t task_cap;
     LevelPart *part;       // [0, G): finalize counts, [G, 2G): relax counts
-    int *out;              // k x n
-    int n, nwords, k, KS, G;
+    uint4 *slot_or;        // per-workgroup OR of new frontier masks
+    int *out;              // S x n rows
+    int n, nwords, KS, G;
     long long delta;       // bucket width (LLONG_MAX / 4: one bucket, plain Bellman-Ford)
 };
+
+__device__ __forceinline__ M8 zero8s() { return M8{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)}; }
 
 __global__ void sb_init(SBatchArgs a, int iter_limit) {
     const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, nt = (int64_t)gridDim.x * blockDim.x;
@@ -60,21 +68,56 @@ __global__ void sb_init(SBatchArgs a, int iter_limit) {
         a.act[1][i] = 0;
     }
     for (int64_t i = tid; i < (int64_t)SB_RING * a.nwords; i += nt) a.ring[i] = 0;
+    int *sci = reinterpret_cast<int *>(a.sc);
+    for (int64_t i = tid; i < (int64_t)(sizeof(SlotCtl) / 4); i += nt) sci[i] = 0;
+    for (int64_t i = tid; i < 2 * MAX_GRID; i += nt) a.slot_or[i] = make_uint4(0, 0, 0, 0);
     if (tid == 0) {
-        ctl_reset(a.ctl, a.n, iter_limit, a.k);
+        ctl_reset(a.ctl, a.n, iter_limit, 0);  // idle until a slot is seeded
         a.ctl->thr = a.delta;
         a.ctl->far_hi = 0;
     }
 }
 
-__global__ void sb_seed(SBatchArgs a, const int *srcs) {
-    const int s = threadIdx.x;
-    if (s >= a.k) return;
-    const int v = srcs[s];
-    a.D[(int64_t)v * a.KS + s] = 0;
-    uint32_t *fv = reinterpret_cast<uint32_t *>(a.F + 2 * v);
-    atomicOr(&fv[s >> 5], 1u << (s & 31));
-    atomicOr(&a.act[0][v >> 5], 1u << (v & 31));
+// Resets the distance columns (and stray waiting pairs) of the slots about to be seeded.
+__global__ void sb_prepare(SBatchArgs a, M8 newmask, SlotList sl) {
+    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, nt = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t u = tid; u < a.n; u += nt) {
+        for (int j = 0; j < sl.n; ++j) a.D[u * a.KS + sl.slot[j]] = INT_MAX;
+        const M8 f = ld8(a.FAR, (int)u);
+        const M8 kept = andnot8(f, newmask);
+        if (any8(andnot8(f, kept))) st8(a.FAR, (int)u, kept);
+    }
+}
+
+// One thread per (slot, source) pair; see batch_seed for the level bookkeeping.
+__global__ void sb_seed(SBatchArgs a, SeedList sl) {
+    Ctl *c = a.ctl;
+    const int i = threadIdx.x;
+    const int pending = c->pending;
+    const int Lf = pending ? c->level + 1 : c->level;
+    const long long base = c->thr - a.delta;  // the current bucket's lower bound
+    if (i < sl.n) {
+        const int s = sl.slot[i], v = sl.src[i];
+        const uint32_t sb = 1u << (s & 31);
+        a.D[(int64_t)v * a.KS + s] = 0;
+        atomicOr(reinterpret_cast<uint32_t *>(a.F + 2 * v) + (s >> 5), sb);
+        atomicOr(&a.act[Lf & 1][v >> 5], 1u << (v & 31));
+        atomicOr(&a.sc->assigned[s >> 5], sb);
+        atomicOr(&a.sc->live[s >> 5], sb);
+        a.sc->off[s] = base;
+        a.sc->far_hi[s] = (unsigned long long)(base / a.delta);
+        a.sc->te[s] = 0;
+        a.sc->nr[s] = 0;
+    }
+    __syncthreads();
+    if (i == 0 && sl.n > 0) {
+        if (pending) {
+            atomicAdd((unsigned long long *)&a.part[0].nf, (unsigned long long)sl.n);
+        } else {
+            c->done = c->iter_limit <= 0;
+            c->aux = 0;
+        }
+    }
 }
 
 // One workgroup: closes the previous iteration; moves to the next bucket when the
@@ -113,8 +156,8 @@ __global__ __launch_bounds__(SMALL_BLK) void sb_close(SBatchArgs a) {
     }
 }
 
-// Calls fn(v) for each set bit of bm, one vertex per wave (all lanes), and clears
-// the words it reads when `clear` is set.
+// Calls fn(v, lane) for each set bit of bm, one vertex per wave, and clears the
+// words it reads when `clear` is set.
 template <class Fn>
 __device__ void for_each_bit_wave(uint32_t *bm, int nwords, bool clear, Fn fn) {
     constexpr int WPR = BLK / 2;
@@ -140,7 +183,7 @@ __device__ void for_each_bit_wave(uint32_t *bm, int nwords, bool clear, Fn fn) {
     }
 }
 
-// Combines the lanes' source nibbles into the 32-bit words of a vertex mask.
+// Combines the lanes' slot nibbles into the 32-bit words of a vertex mask.
 __device__ __forceinline__ uint32_t nibble_word(uint32_t nib, int lane) {
     uint32_t word = nib << (4 * (lane & 7));
     word |= __shfl_xor(word, 1);
@@ -160,8 +203,15 @@ __device__ __forceinline__ void mark_active(uint32_t *bm, int v) {
     if (!(bm[v >> 5] & bit)) atomicOr(&bm[v >> 5], bit);
 }
 
-// New bucket: pairs of the ring bitmap's vertices whose distance lies in the bucket
-// move from FAR to F; pairs already below it were expanded when they dropped.
+// The bucket offsets of lane's four slots.
+__device__ __forceinline__ void lane_offsets(const SBatchArgs &a, int lane, long long o[4]) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) o[b] = 4 * lane + b < a.KS ? a.sc->off[4 * lane + b] : 0;
+}
+
+// New bucket: pairs of the ring bitmap's vertices whose virtual distance lies in
+// the bucket move from FAR to F; pairs already below it were expanded when they
+// dropped.
 __global__ __launch_bounds__(BLK) void sb_activate(SBatchArgs a) {
     Ctl *c = a.ctl;
     if (c->done || !c->aux) return;
@@ -169,6 +219,8 @@ __global__ __launch_bounds__(BLK) void sb_activate(SBatchArgs a) {
     const int L = c->level;
     uint32_t *bm = a.ring + (int64_t)((T0 / a.delta) % SB_RING) * a.nwords;
     uint32_t *act = a.act[L & 1];
+    long long off4[4];
+    lane_offsets(a, lane_id(), off4);
     for_each_bit_wave(bm, a.nwords, true, [&](int u, int lane) {
         const uint32_t fw = reinterpret_cast<const uint32_t *>(a.FAR + 2 * u)[lane >> 3];
         const uint32_t nib = 4 * lane < a.KS ? (fw >> (4 * (lane & 7))) & 0xFu : 0u;
@@ -179,8 +231,9 @@ __global__ __launch_bounds__(BLK) void sb_activate(SBatchArgs a) {
 #pragma unroll
             for (int b = 0; b < 4; ++b)
                 if (nib & (1u << b)) {
-                    if (dd[b] < T0) drop |= 1u << b;
-                    else if (dd[b] < T) go |= 1u << b;
+                    const long long vd = (long long)dd[b] + off4[b];
+                    if (vd < T0) drop |= 1u << b;
+                    else if (vd < T) go |= 1u << b;
                 }
         }
         const uint32_t gw = nibble_word(go, lane), kw = nibble_word(nib & ~(go | drop), lane);
@@ -222,7 +275,7 @@ __global__ __launch_bounds__(BLK) void sb_tasks(SBatchArgs a) {
     });
 }
 
-// One wave per task; lanes are sources.
+// One wave per task; lanes are slots.
 __global__ __launch_bounds__(BLK) void sb_relax(SBatchArgs a) {
     Ctl *c = a.ctl;
     if (c->done) return;
@@ -233,6 +286,8 @@ __global__ __launch_bounds__(BLK) void sb_relax(SBatchArgs a) {
     const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
     const int nt = (int)min((int64_t)c->nchunks, a.task_cap);
     const bool has_slice = 4 * lane < a.KS;
+    long long off4[4], hi4[4] = {-1, -1, -1, -1};
+    lane_offsets(a, lane, off4);
     long long ex = 0, hi = 0;
     for (int t = wave; t < nt; t += nwaves) {
         const ChunkRec r = a.tasks[t];
@@ -267,13 +322,15 @@ __global__ __launch_bounds__(BLK) void sb_relax(SBatchArgs a) {
                     for (int b = 0; b < 4; ++b)
                         if ((nib & (1u << b)) && cs[b] < ds[b]) {
                             atomicMin(dp + b, (int)cs[b]);
-                            if (cs[b] < T) {
+                            const long long vd = cs[b] + off4[b];
+                            if (vd < T) {
                                 near |= 1u << b;
                             } else {
                                 far |= 1u << b;
-                                const long long bk = cs[b] / delta;
+                                const long long bk = vd / delta;
                                 slots |= 1u << (int)(bk % SB_RING);
                                 hi = max(hi, bk);
+                                hi4[b] = max(hi4[b], bk);
                             }
                         }
                 }
@@ -291,7 +348,12 @@ __global__ __launch_bounds__(BLK) void sb_relax(SBatchArgs a) {
             }
         }
     }
-    ex = block_sum<BLK>(ex);  // each lane counted its own sources
+    // per-slot highest waiting bucket (read first: most waves add nothing new)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        if (hi4[b] >= 0 && (unsigned long long)hi4[b] > a.sc->far_hi[4 * lane + b])
+            atomicMax(&a.sc->far_hi[4 * lane + b], (unsigned long long)hi4[b]);
+    ex = block_sum<BLK>(ex);  // each lane counted its own slots
     hi = -block_min<BLK>(-hi);
     if (threadIdx.x == 0) {
         LevelPart p{0, hi, ex, 0};
@@ -303,7 +365,7 @@ __global__ __launch_bounds__(BLK) void sb_fin_a(SBatchArgs a) {
     Ctl *c = a.ctl;
     if (c->done) return;
     for_each_bit(a.act[c->level & 1], a.nwords, [&](int v) {
-        if (v >= 0) st8(a.F, v, M8{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)});
+        if (v >= 0) st8(a.F, v, zero8s());
     });
 }
 
@@ -314,68 +376,148 @@ __global__ __launch_bounds__(BLK) void sb_fin_b(SBatchArgs a) {
     uint32_t *old = a.act[L & 1];
     for (int i = blockIdx.x * BLK + (int)threadIdx.x; i < a.nwords; i += gridDim.x * BLK) old[i] = 0;
     long long nf = 0;
+    M8 lv = zero8s();
     for_each_bit(a.act[(L + 1) & 1], a.nwords, [&](int u) {
         if (u < 0) return;
-        st8(a.F, u, ld8(a.N, u));
-        st8(a.N, u, M8{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)});
+        const M8 nw = ld8(a.N, u);
+        st8(a.F, u, nw);
+        st8(a.N, u, zero8s());
+        lv = or8(lv, nw);
         ++nf;
     });
     nf = block_sum<BLK>(nf);
+    // block OR of the new masks
+    __shared__ uint4 s_or[2 * (BLK / WAVE)];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) lv = or8(lv, shfl_xor8(lv, o));
+    if (lane_id() == 0) {
+        s_or[2 * (threadIdx.x / WAVE)] = lv.a;
+        s_or[2 * (threadIdx.x / WAVE) + 1] = lv.b;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        M8 r = zero8s();
+        for (int i = 0; i < BLK / WAVE; ++i) r = or8(r, M8{s_or[2 * i], s_or[2 * i + 1]});
+        st8(a.slot_or, blockIdx.x, r);
         LevelPart p{nf, 0, 0, 0};
         a.part[blockIdx.x] = p;
         if (blockIdx.x == 0) c->pending = 1;
     }
 }
 
-// D (n x KS) -> out (k x n) through a 64 x 64 LDS tile; counts reached
-// (source, vertex) pairs and their out-degrees into part[blockIdx].
-__global__ __launch_bounds__(BLK) void sb_transpose(SBatchArgs a) {
+// Distance columns of the listed slots -> their rows, through a 64-vertex x 64-slot
+// LDS tile (reads along a vertex's row of D, writes along the slot's row).
+__global__ __launch_bounds__(BLK) void sb_extract(SBatchArgs a, SlotList sl) {
     __shared__ int tile[64][65];
-    const int tv = (a.n + 63) / 64, ts = (a.k + 63) / 64;
-    long long te = 0, nr = 0;
+    const int tv = (a.n + 63) / 64, ts = (sl.n + 63) / 64;
     for (int t = blockIdx.x; t < tv * ts; t += gridDim.x) {
-        const int v0 = (t % tv) * 64, s0 = (t / tv) * 64;
+        const int v0 = (t % tv) * 64, j0 = (t / tv) * 64;
         for (int i = threadIdx.x; i < 64 * 64; i += BLK) {
             const int r = i / 64, cc = i % 64;
-            const int v = v0 + r, s = s0 + cc;
-            tile[r][cc] = (v < a.n && s < a.k) ? a.D[(int64_t)v * a.KS + s] : INT_MAX;
+            const int v = v0 + r, j = j0 + cc;
+            tile[r][cc] = (v < a.n && j < sl.n) ? a.D[(int64_t)v * a.KS + sl.slot[j]] : INT_MAX;
         }
         __syncthreads();
         for (int i = threadIdx.x; i < 64 * 64; i += BLK) {
             const int cc = i / 64, r = i % 64;
-            const int v = v0 + r, s = s0 + cc;
-            if (v < a.n && s < a.k) {
-                const int d = tile[r][cc];
-                a.out[(int64_t)s * a.n + v] = d;
-                if (d != INT_MAX) {
-                    ++nr;
-                    te += a.off[v + 1] - a.off[v];
-                }
-            }
+            const int v = v0 + r, j = j0 + cc;
+            if (v < a.n && j < sl.n) a.out[(int64_t)sl.slot[j] * a.n + v] = tile[r][cc];
         }
         __syncthreads();
     }
-    te = block_sum<BLK>(te);
-    nr = block_sum<BLK>(nr);
-    if (threadIdx.x == 0) {
-        LevelPart p{0, 0, te, nr};
-        a.part[blockIdx.x] = p;
-    }
 }
 
-__global__ __launch_bounds__(SMALL_BLK) void sb_stats(SBatchArgs a, int nblocks) {
+// Sums the per-slot counts of slots [0, k) into the control block.
+__global__ __launch_bounds__(SMALL_BLK) void sb_sum_stats(Ctl *c, const SlotCtl *sc, int k) {
     long long te = 0, nr = 0;
-    for (int i = threadIdx.x; i < nblocks; i += SMALL_BLK) {
-        te += a.part[i].ex;
-        nr += a.part[i].pad;
+    for (int s = threadIdx.x; s < k; s += SMALL_BLK) {
+        te += sc->te[s];
+        nr += sc->nr[s];
     }
     te = block_sum<SMALL_BLK>(te);
     nr = block_sum<SMALL_BLK>(nr);
     if (threadIdx.x == 0) {
-        a.ctl->edges_traversed = te;
-        a.ctl->reached = nr;
+        c->edges_traversed = te;
+        c->reached = nr;
     }
+}
+
+static M8 host_mask8(const uint32_t m[8]) {
+    return M8{make_uint4(m[0], m[1], m[2], m[3]), make_uint4(m[4], m[5], m[6], m[7])};
+}
+
+static SBatchArgs sssp_args(BatchDev &b, mg_graph *g, long long delta) {
+    SBatchArgs a{};
+    a.ctl = b.ctl;
+    a.sc = b.sc;
+    a.off = g->row_off;
+    a.adj = g->col;
+    a.w = g->w;
+    a.D = b.D;
+    a.F = b.F;
+    a.N = b.N;
+    a.FAR = b.FAR;
+    a.act[0] = b.act[0];
+    a.act[1] = b.act[1];
+    a.ring = b.ring;
+    a.tasks = b.tasks;
+    a.task_cap = b.task_cap;
+    a.part = b.part;
+    a.slot_or = b.slot_or;
+    a.out = b.rows;
+    a.n = g->n;
+    a.nwords = b.nwords;
+    a.KS = b.KS;
+    a.G = b.G;
+    a.delta = delta;
+    return a;
+}
+
+int sssp_batch_reset(BatchDev &b, mg_graph *g, hipStream_t st, int iter_limit, long long delta) {
+    SBatchArgs a = sssp_args(b, g, delta);
+    hipLaunchKernelGGL(sb_init, dim3(grid_blocks((int64_t)g->n * b.KS / 4 + 2 * (int64_t)g->n, BLK)), dim3(BLK), 0, st, a,
+                       iter_limit);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+int sssp_batch_seed(BatchDev &b, mg_graph *g, hipStream_t st, const SeedList &sl, long long delta, bool fresh) {
+    if (sl.n <= 0) return MG_OK;
+    SBatchArgs a = sssp_args(b, g, delta);
+    if (!fresh) {
+        uint32_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        SlotList ls{};
+        ls.n = sl.n;
+        for (int i = 0; i < sl.n; ++i) {
+            m[sl.slot[i] >> 5] |= 1u << (sl.slot[i] & 31);
+            ls.slot[i] = sl.slot[i];
+        }
+        hipLaunchKernelGGL(sb_prepare, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, st, a, host_mask8(m), ls);
+    }
+    hipLaunchKernelGGL(sb_seed, dim3(1), dim3(MG_BATCH_MAX), 0, st, a, sl);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+int sssp_batch_step(BatchDev &b, mg_graph *g, hipStream_t st, long long delta) {
+    SBatchArgs a = sssp_args(b, g, delta);
+    hipLaunchKernelGGL(sb_close, dim3(1), dim3(SMALL_BLK), 0, st, a);
+    hipLaunchKernelGGL(sb_activate, dim3(a.G), dim3(BLK), 0, st, a);
+    hipLaunchKernelGGL(sb_tasks, dim3(a.G), dim3(BLK), 0, st, a);
+    hipLaunchKernelGGL(sb_relax, dim3(a.G), dim3(BLK), 0, st, a);
+    hipLaunchKernelGGL(sb_fin_a, dim3(a.G), dim3(BLK), 0, st, a);
+    hipLaunchKernelGGL(sb_fin_b, dim3(a.G), dim3(BLK), 0, st, a);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+int sssp_batch_extract(BatchDev &b, mg_graph *g, hipStream_t st, const SlotList &sl) {
+    if (sl.n <= 0) return MG_OK;
+    SBatchArgs a = sssp_args(b, g, 1);
+    const int tb = grid_blocks((int64_t)((g->n + 63) / 64) * ((sl.n + 63) / 64) * BLK, BLK);
+    hipLaunchKernelGGL(sb_extract, dim3(tb), dim3(BLK), 0, st, a, sl);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
 }
 
 // Bucket width: the option, or about four mean edge weights, raised to
@@ -405,59 +547,25 @@ extern "C" int mg_sssp_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_
     if (g->min_weight < 0) return set_error(MG_ERR_UNSUPPORTED, "mg_sssp_batch: negative edge weights");
     if (o.directed == 0 && !(g->flags & MG_GRAPH_SYMMETRIC))
         return set_error(MG_ERR_UNSUPPORTED, "mg_sssp_batch: directed=0 needs a symmetric graph");
-    if ((rc = ensure_batch(g, true))) return rc;
-    Workspace &ws = g->ws;
-    BatchState &bs = g->steps->batch;
-    SBatchArgs a{};
-    a.ctl = ws.ctl;
-    a.off = g->row_off;
-    a.adj = g->col;
-    a.w = g->w;
-    a.D = bs.D;
-    a.F = reinterpret_cast<uint4 *>(bs.F);
-    a.N = reinterpret_cast<uint4 *>(bs.N);
-    a.FAR = reinterpret_cast<uint4 *>(bs.FAR);
-    a.ring = bs.fb;
-    a.act[0] = ws.bm[0];
-    a.act[1] = ws.bm[1];
-    a.tasks = bs.tasks;
-    a.task_cap = bs.task_cap;
-    a.part = ws.part;
-    a.out = bs.out;
-    a.n = g->n;
-    a.nwords = ws.nwords;
-    a.k = k;
-    a.KS = (k + 15) / 16 * 16;
-    a.G = grid_blocks(std::max<int64_t>(g->n / 4, g->m / 16), BLK);
-    a.delta = sssp_delta(g, o);
-
-    MG_HIP(hipMemcpyAsync(bs.srcs, srcs, 4 * (size_t)k, hipMemcpyHostToDevice, g->stream));
-    MG_HIP(hipEventRecord(g->ev0, g->stream));
-    hipLaunchKernelGGL(sb_init, dim3(grid_blocks((int64_t)g->n * a.KS / 4, BLK)), dim3(BLK), 0, g->stream, a,
-                       o.iter_num);
-    hipLaunchKernelGGL(sb_seed, dim3(1), dim3(MG_BATCH_MAX), 0, g->stream, a, (const int *)bs.srcs);
-    MG_HIP(hipGetLastError());
-    auto step = [&]() -> int {
-        hipLaunchKernelGGL(sb_close, dim3(1), dim3(SMALL_BLK), 0, g->stream, a);
-        hipLaunchKernelGGL(sb_activate, dim3(a.G), dim3(BLK), 0, g->stream, a);
-        hipLaunchKernelGGL(sb_tasks, dim3(a.G), dim3(BLK), 0, g->stream, a);
-        hipLaunchKernelGGL(sb_relax, dim3(a.G), dim3(BLK), 0, g->stream, a);
-        hipLaunchKernelGGL(sb_fin_a, dim3(a.G), dim3(BLK), 0, g->stream, a);
-        hipLaunchKernelGGL(sb_fin_b, dim3(a.G), dim3(BLK), 0, g->stream, a);
-        MG_HIP(hipGetLastError());
-        return MG_OK;
-    };
-    const uint64_t key = mix64(mix64((uint64_t)k * 2 + (uint64_t)o.directed) ^ (uint64_t)a.delta);
-    if ((rc = drive(g, step, (int64_t)o.iter_num + 3, KIND_BATCH_SSSP, key))) return rc;
-    const int tb = grid_blocks((int64_t)((g->n + 63) / 64) * ((k + 63) / 64) * BLK, BLK);
-    hipLaunchKernelGGL(sb_transpose, dim3(tb), dim3(BLK), 0, g->stream, a);
-    hipLaunchKernelGGL(sb_stats, dim3(1), dim3(SMALL_BLK), 0, g->stream, a, tb);
-    MG_HIP(hipGetLastError());
-    rc = finish_search(g, stats, nullptr, 0, dist_out, bs.out, 4 * (size_t)k * g->n, o.out_on_device, timer);
-    if (rc) return rc;
-    if (stats) {
-        stats->edges_traversed = ws.h_ctl->edges_traversed;
-        stats->vertices_reached = ws.h_ctl->reached;
+    BatchDev *b = nullptr;
+    if ((rc = graph_batch_dev(g, true, &b))) return rc;
+    const long long delta = sssp_delta(g, o);
+    SeedList sl{};
+    SlotList all{};
+    sl.n = all.n = k;
+    for (int s = 0; s < k; ++s) {
+        sl.slot[s] = all.slot[s] = s;
+        sl.src[s] = srcs[s];
     }
-    return MG_OK;
+    MG_HIP(hipEventRecord(g->ev0, g->stream));
+    if ((rc = sssp_batch_reset(*b, g, g->stream, o.iter_num, delta))) return rc;
+    if ((rc = sssp_batch_seed(*b, g, g->stream, sl, delta, true))) return rc;
+    auto step = [&]() -> int { return sssp_batch_step(*b, g, g->stream, delta); };
+    if ((rc = drive(g, step, (int64_t)o.iter_num + 3, KIND_BATCH_SSSP, mix64((uint64_t)delta), b->ctl, b->h_ctl)))
+        return rc;
+    if ((rc = sssp_batch_extract(*b, g, g->stream, all))) return rc;
+    if ((rc = batch_row_stats(*b, g, g->stream, all, INT_MAX))) return rc;
+    hipLaunchKernelGGL(sb_sum_stats, dim3(1), dim3(SMALL_BLK), 0, g->stream, b->ctl, (const SlotCtl *)b->sc, k);
+    MG_HIP(hipGetLastError());
+    return finish_batch(g, *b, stats, dist_out, 4 * (size_t)k * g->n, o.out_on_device, timer);
 }

@@ -167,49 +167,45 @@ struct StepGraph {
 };
 
 template <class Step>
-int drive(mg_graph *g, Step step, int64_t max_steps, int kind = -1, uint64_t key = 0);
+int drive(mg_graph *g, Step step, int64_t max_steps, int kind = -1, uint64_t key = 0, Ctl *ctl = nullptr,
+          Ctl *h_ctl = nullptr);
 bool graphs_enabled();
 
 // Generic GAS enactor over the ported programs (engine = MG_ENGINE_GAS).
 struct GasStateBase;
+struct BatchDev;
+struct PprDev;
 int gas_bfs(mg_graph *g, int src, int *out, const mg_options &o, mg_stats *st, const CallTimer &t);
 int gas_sssp(mg_graph *g, int src, int *out, const mg_options &o, mg_stats *st, const CallTimer &t);
 int gas_cc(mg_graph *g, int *out, const mg_options &o, mg_stats *st, const CallTimer &t);
 int gas_pagerank(mg_graph *g, float *out, const mg_options &o, mg_stats *st, const CallTimer &t);
 void gas_release(mg_graph *g);
 void batch_release(mg_graph *g);
+void ppr_release(mg_graph *g);
+int64_t batch_dev_size(const BatchDev *b);  // device bytes held by a batch context
+int64_t ppr_dev_size(const PprDev *p);
 // SSSP bucket width for options o on graph g (a huge value when unbucketed).
 long long sssp_delta(const mg_graph *g, const mg_options &o);
 
 }  // namespace mg
 
-// Device state of batched searches: three 256-bit masks per vertex and the
-// MG_BATCH_MAX x n result, allocated on the first batched call.
-struct BatchState {
-    void *V = nullptr, *F = nullptr, *N = nullptr;
-    int32_t *out = nullptr, *srcs = nullptr;
-    int32_t n = 0;
-    size_t out_cap = 0;
-    // batched SSSP: n x MG_BATCH_MAX distances (vertex-major) and edge tasks
-    int32_t *D = nullptr;
-    mg::ChunkRec *tasks = nullptr;
-    int64_t task_cap = 0;
-    void *FAR = nullptr;       // per-vertex 256-bit mask of pairs waiting in a later bucket
-    uint32_t *fb = nullptr;    // ring of bucket bitmaps (vertices with a waiting pair)
-};
-
 struct mg_graph_steps {
     mg::StepGraph sg[mg::NKINDS];
     mg::GasStateBase *gas[4] = {nullptr, nullptr, nullptr, nullptr};
-    BatchState batch;
+    mg::BatchDev *bdev[2] = {nullptr, nullptr};  // batched BFS / SSSP contexts
+    mg::PprDev *pdev = nullptr;                  // batched personalized PageRank context
 };
 
 // Queues steps until the device reports `done`.  Steps run from a captured HIP graph
 // (kind >= 0) or as direct launches; the host reads the control block only between
 // batches of steps (1, 2, 4, ... graphs), never once per level.
 template <class Step>
-int mg::drive(mg_graph *g, Step step, int64_t max_steps, int kind, uint64_t key) {
+int mg::drive(mg_graph *g, Step step, int64_t max_steps, int kind, uint64_t key, Ctl *ctl, Ctl *h_ctl) {
     Workspace &ws = g->ws;
+    if (!ctl) {
+        ctl = ws.ctl;
+        h_ctl = ws.h_ctl;
+    }
     if (kind >= 0 && kind < NKINDS && graphs_enabled()) {
         StepGraph &sg = g->steps->sg[kind];
         if (!sg.exec || sg.key != key) {
@@ -237,9 +233,9 @@ int mg::drive(mg_graph *g, Step step, int64_t max_steps, int kind, uint64_t key)
         while (queued < max_steps) {
             for (int i = 0; i < k && queued < max_steps; ++i, queued += STEPS_PER_GRAPH)
                 MG_HIP(hipGraphLaunch(sg.exec, g->stream));
-            MG_HIP(hipMemcpyAsync(ws.h_ctl, ws.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
+            MG_HIP(hipMemcpyAsync(h_ctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
             MG_HIP(hipStreamSynchronize(g->stream));
-            if (ws.h_ctl->done) return MG_OK;
+            if (h_ctl->done) return MG_OK;
             if (k < 16) k *= 2;
         }
         return MG_OK;
@@ -250,9 +246,9 @@ int mg::drive(mg_graph *g, Step step, int64_t max_steps, int kind, uint64_t key)
             int rc = step();
             if (rc) return rc;
         }
-        MG_HIP(hipMemcpyAsync(ws.h_ctl, ws.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
+        MG_HIP(hipMemcpyAsync(h_ctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
         MG_HIP(hipStreamSynchronize(g->stream));
-        if (ws.h_ctl->done) return MG_OK;
+        if (h_ctl->done) return MG_OK;
         if (k < 64) k *= 2;
     }
     return MG_OK;

@@ -275,6 +275,7 @@ int mg_graph_destroy(mg_graph *g) {
     if (g->steps) {
         gas_release(g);
         batch_release(g);
+        ppr_release(g);
         for (auto &s : g->steps->sg)
             if (s.exec) (void)hipGraphExecDestroy(s.exec);
         delete g->steps;
@@ -310,7 +311,10 @@ int mg_graph_device_bytes(const mg_graph *g, int64_t *bytes) {
     int64_t b = g->topo_bytes;
     if (ws.ctl) b += 4LL * ws.n * 5 + 4LL * ws.nwords * 4 + (int64_t)sizeof(ChunkRec) * ws.chunk_cap;
     if (ws.rank) b += 12LL * ws.n;
-    if (g->steps && g->steps->batch.V) b += 96LL * g->steps->batch.n + 4LL * (int64_t)g->steps->batch.out_cap;
+    if (g->steps)
+        for (const BatchDev *p : g->steps->bdev)
+            if (p) b += batch_dev_size(p);
+    if (g->steps && g->steps->pdev) b += ppr_dev_size(g->steps->pdev);
     *bytes = b;
     return MG_OK;
 }

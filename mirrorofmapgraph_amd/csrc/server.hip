@@ -1,0 +1,509 @@
+// Serving: traversal queries on resident graphs, with continuous batching.
+//
+// Not in the reference (each of its drivers runs one search at a time, e.g.
+// Algorithms/BFS/bfs.cu:402-417).  Queries (BFS, SSSP, personalized PageRank) on
+// graphs added to the server are served by one engine per (graph, kind).  An
+// engine owns a batch context with up to 256 slots (64 for PageRank) and its own
+// HIP stream, so engines of different graphs and kinds overlap on the device.
+// Engines advance in rounds driven by one dispatcher thread; a round
+//   1. retires finished slots: their rows are copied out (SSSP: transposed out of
+//      the vertex-major distances first) with per-slot TEPS counts;
+//   2. admits waiting queries into free slots: seeded into the running search, which
+//      does not stop (a BFS slot joins the next level, an SSSP slot the current
+//      bucket with its own offset, a PageRank slot the next iteration);
+//   3. runs a chunk of steps (a captured HIP graph of STEPS_PER_GRAPH levels; a few
+//      PageRank iterations) and copies the slot status back;
+// and ends with an event the dispatcher polls, so no round waits for another
+// engine.  An engine's device state is allocated when it first has work, within
+// the server's HBM budget (idle engines give theirs back when another needs it;
+// an engine that cannot fit even one slot leaves its queries waiting).  Results do
+// not depend on what else runs: BFS depths and SSSP distances are unique, and a
+// PageRank slot's arithmetic is fixed by the graph.
+#include "batch.h"
+
+#include <climits>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace mg {
+
+struct Query {
+    int64_t id = 0;
+    int graph = 0, kind = 0, src = 0;
+    void *out = nullptr;
+    bool out_on_device = false;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    int status = MG_OK;
+    bool done = false;
+    int slot = -1;
+    int iters_left = 0;  // PageRank
+    int64_t te = 0, nr = 0;
+};
+
+struct Server;
+
+struct Engine {
+    Server *srv = nullptr;
+    mg_graph *g = nullptr;
+    int kind = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    bool allocated = false, inflight = false;
+    int S = 0;                              // slots
+    BatchDev b;                             // BFS / SSSP
+    PprDev p;                               // PageRank
+    hipGraphExec_t chunk = nullptr;         // captured steps
+    long long delta = 1;                    // SSSP bucket width
+    std::deque<Query *> waiting;
+    std::vector<Query *> slot_q;            // slot -> running query
+    std::vector<Query *> retiring;          // rows copied in the round in flight
+    int running = 0;
+    double idle_since = 0;
+    int64_t bytes() const { return allocated ? (kind == MG_Q_PPR ? p.bytes : b.bytes) : 0; }
+};
+
+struct Server {
+    mg_server_config cfg{};
+    std::vector<mg_graph *> graphs;
+    std::vector<std::unique_ptr<Engine>> engines;  // graph * 3 + kind
+    std::unordered_map<int64_t, std::unique_ptr<Query>> queries;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::thread th;
+    bool stop = false;
+    int device = 0;
+    int64_t next_id = 1;
+    int64_t bytes_used = 0;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    mg_server_stats stats{};
+    int error = MG_OK;
+    double now() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+static int engine_steps(Engine &e) {
+    return e.srv->cfg.steps_per_round > 0 ? e.srv->cfg.steps_per_round : STEPS_PER_GRAPH;
+}
+
+static void engine_free(Server &s, Engine &e) {
+    if (!e.allocated) return;
+    if (e.chunk) (void)hipGraphExecDestroy(e.chunk);
+    e.chunk = nullptr;
+    s.bytes_used -= e.bytes();
+    if (e.kind == MG_Q_PPR) ppr_dev_free(e.p);
+    else batch_dev_free(e.b);
+    e.allocated = false;
+    e.S = 0;
+}
+
+// Allocates the engine's device state within the budget: idle engines are freed
+// first if needed, and the slot count is halved until it fits.
+static int engine_alloc(Server &s, Engine &e) {
+    const int want = e.kind == MG_Q_PPR ? PPR_SLOTS : std::min(MG_BATCH_MAX, std::max(1, s.cfg.max_slots));
+    auto need = [&](int S) {
+        return e.kind == MG_Q_PPR ? ppr_dev_bytes(e.g) : batch_dev_bytes(e.g, e.kind == MG_Q_SSSP, S);
+    };
+    int S = want;
+    if (s.bytes_used + need(S) > s.cfg.hbm_budget) {
+        for (auto &o : s.engines)
+            if (o && o.get() != &e && o->allocated && !o->inflight && o->running == 0 && o->waiting.empty())
+                engine_free(s, *o);
+    }
+    while (S > 1 && s.bytes_used + need(S) > s.cfg.hbm_budget) S = e.kind == MG_Q_PPR ? 0 : S / 2;
+    if (S < 1 || s.bytes_used + need(S) > s.cfg.hbm_budget) return MG_ERR_OOM;  // wait for memory
+    int rc;
+    if (e.kind == MG_Q_PPR) {
+        rc = ppr_dev_alloc(e.p, e.g);
+    } else {
+        rc = batch_dev_alloc(e.b, e.g, e.kind == MG_Q_SSSP, S);
+    }
+    if (rc) return rc;
+    e.S = S;
+    e.allocated = true;
+    s.bytes_used += e.bytes();
+    e.slot_q.assign(S, nullptr);
+    if (e.kind == MG_Q_BFS) {
+        if ((rc = bfs_batch_reset(e.b, e.g, e.st, INT_MAX, 4, 16, 0))) return rc;
+    } else if (e.kind == MG_Q_SSSP) {
+        mg_options o;
+        mg_options_default(&o);
+        o.sssp_delta = s.cfg.sssp_delta;
+        e.delta = sssp_delta(e.g, o);
+        if ((rc = sssp_batch_reset(e.b, e.g, e.st, INT_MAX, e.delta))) return rc;
+    }
+    if (e.kind != MG_Q_PPR) {
+        MG_HIP(hipStreamBeginCapture(e.st, hipStreamCaptureModeThreadLocal));
+        int src = MG_OK;
+        for (int i = 0; i < engine_steps(e) && src == MG_OK; ++i)
+            src = e.kind == MG_Q_BFS ? bfs_batch_step(e.b, e.g, e.st) : sssp_batch_step(e.b, e.g, e.st, e.delta);
+        if (src == MG_OK) src = batch_live(e.b, e.st);
+        hipGraph_t graph = nullptr;
+        hipError_t he = hipStreamEndCapture(e.st, &graph);
+        if (src) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return src;
+        }
+        if (he != hipSuccess) return set_error(MG_ERR_DEVICE, "step capture failed: %s", hipGetErrorString(he));
+        he = hipGraphInstantiate(&e.chunk, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (he != hipSuccess) return set_error(MG_ERR_DEVICE, "step graph instantiation failed: %s", hipGetErrorString(he));
+    }
+    return MG_OK;
+}
+
+static void finish_query(Server &s, Query *q, double t) {
+    q->t_done = t;
+    q->done = true;
+    s.stats.completed++;
+    s.stats.edges_traversed += q->te;
+    s.stats.latency_ms_sum += q->t_done - q->t_submit;
+}
+
+// One round of an engine; returns true if it queued work.  Called with s.mu held.
+static int engine_round(Server &s, Engine &e, bool *queued) {
+    *queued = false;
+    std::vector<int> finished;
+    if (e.inflight) {
+        const hipError_t q = hipEventQuery(e.ev);
+        if (q == hipErrorNotReady) return MG_OK;
+        if (q != hipSuccess) return set_error(MG_ERR_DEVICE, "engine event: %s", hipGetErrorString(q));
+        e.inflight = false;
+        const double t = s.now();
+        for (Query *r : e.retiring) {
+            if (e.kind != MG_Q_PPR) {
+                r->te = e.b.h_sc->te[r->slot];
+                r->nr = e.b.h_sc->nr[r->slot];
+            }
+            e.slot_q[r->slot] = nullptr;
+            r->slot = -1;
+            finish_query(s, r, t);
+        }
+        if (!e.retiring.empty()) s.cv_done.notify_all();
+        e.retiring.clear();
+        if (e.kind != MG_Q_PPR && e.b.h_ctl->changed & (1 << 30))
+            return set_error(MG_ERR_OVERFLOW, "serving engine task queue overflow");
+        // slots whose search ended
+        for (int sl = 0; sl < e.S; ++sl) {
+            Query *r = e.slot_q[sl];
+            if (!r) continue;
+            bool fin;
+            if (e.kind == MG_Q_PPR) {
+                fin = r->iters_left <= 0;
+            } else {
+                const bool live = (e.b.h_sc->live[sl >> 5] >> (sl & 31)) & 1;
+                fin = !live;
+                if (fin && e.kind == MG_Q_SSSP) {
+                    const long long cb = e.b.h_ctl->thr / e.delta - 1;
+                    fin = (long long)e.b.h_sc->far_hi[sl] <= cb || e.b.h_ctl->done;
+                }
+            }
+            if (fin) finished.push_back(sl);
+        }
+    }
+    const bool has_waiting = !e.waiting.empty();
+    if (finished.empty() && !has_waiting && e.running == 0) {
+        if (e.idle_since == 0) e.idle_since = s.now();
+        return MG_OK;
+    }
+    e.idle_since = 0;
+    int rc;
+    if (!e.allocated) {
+        rc = engine_alloc(s, e);
+        if (rc == MG_ERR_OOM && !e.allocated) {
+            clear_error();
+            return MG_OK;  // queries wait for memory
+        }
+        if (rc) return rc;
+    }
+    // 1. retire finished slots
+    SlotList fl{};
+    uint32_t fmask[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int sl : finished) {
+        fl.slot[fl.n++] = sl;
+        fmask[sl >> 5] |= 1u << (sl & 31);
+    }
+    if (fl.n) {
+        if (e.kind == MG_Q_SSSP && (rc = sssp_batch_extract(e.b, e.g, e.st, fl))) return rc;
+        if (e.kind == MG_Q_PPR) {
+            if ((rc = ppr_extract_rows(e.p, e.st, fl))) return rc;
+        } else if ((rc = batch_row_stats(e.b, e.g, e.st, fl, e.kind == MG_Q_BFS ? -1 : INT_MAX))) {
+            return rc;
+        }
+        for (int sl : finished) {
+            Query *r = e.slot_q[sl];
+            const size_t bytes = 4 * (size_t)e.g->n;
+            const void *row = e.kind == MG_Q_PPR ? (const void *)(e.p.rows + (size_t)sl * e.g->n)
+                                                 : (const void *)(e.b.rows + (size_t)sl * e.g->n);
+            if (r->out)
+                MG_HIP(hipMemcpyAsync(r->out, row, bytes,
+                                      r->out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, e.st));
+            if (e.kind == MG_Q_PPR) {
+                r->te = e.g->m * (int64_t)(s.cfg.ppr_iters);
+                r->nr = e.g->n;
+            }
+            e.retiring.push_back(r);
+            e.running--;
+        }
+        if (e.kind != MG_Q_PPR && (rc = batch_release_slots(e.b, e.st, fmask))) return rc;
+    }
+    // 2. admit waiting queries into free slots
+    SeedList seeds{};
+    std::vector<bool> busy(e.S, false);
+    for (int sl = 0; sl < e.S; ++sl) busy[sl] = e.slot_q[sl] != nullptr;
+    for (int sl = 0; sl < e.S && !e.waiting.empty(); ++sl) {
+        if (busy[sl]) continue;
+        Query *r = e.waiting.front();
+        e.waiting.pop_front();
+        r->slot = sl;
+        r->t_admit = s.now();
+        r->iters_left = s.cfg.ppr_iters;
+        e.slot_q[sl] = r;
+        busy[sl] = true;
+        seeds.slot[seeds.n] = sl;
+        seeds.src[seeds.n++] = r->src;
+        e.running++;
+    }
+    if (e.kind == MG_Q_BFS) {
+        if ((rc = bfs_batch_seed(e.b, e.g, e.st, seeds))) return rc;
+    } else if (e.kind == MG_Q_SSSP) {
+        if ((rc = sssp_batch_seed(e.b, e.g, e.st, seeds, e.delta, false))) return rc;
+    } else if (seeds.n || fl.n) {
+        if ((rc = ppr_seed(e.p, e.st, seeds, fl))) return rc;
+    }
+    // 3. a chunk of steps, then the status copy
+    if (e.running > 0) {
+        if (e.kind == MG_Q_PPR) {
+            int it = 4;
+            for (Query *r : e.slot_q)
+                if (r) it = std::min(it, r->iters_left);
+            it = std::max(it, 1);
+            if ((rc = ppr_iterate(e.p, e.g, e.st, s.cfg.ppr_teleport, it))) return rc;
+            for (Query *r : e.slot_q)
+                if (r) r->iters_left -= it;
+        } else {
+            MG_HIP(hipGraphLaunch(e.chunk, e.st));
+        }
+    }
+    if (e.kind != MG_Q_PPR) {
+        MG_HIP(hipMemcpyAsync(e.b.h_ctl, e.b.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e.st));
+        MG_HIP(hipMemcpyAsync(e.b.h_sc, e.b.sc, sizeof(SlotCtl), hipMemcpyDeviceToHost, e.st));
+    }
+    MG_HIP(hipEventRecord(e.ev, e.st));
+    e.inflight = true;
+    *queued = true;
+    s.stats.rounds++;
+    return MG_OK;
+}
+
+static void dispatcher(Server *sp) {
+    Server &s = *sp;
+    (void)hipSetDevice(s.device);
+    std::unique_lock<std::mutex> lk(s.mu);
+    while (!s.stop) {
+        bool any = false, pending = false;
+        for (auto &ep : s.engines) {
+            if (!ep) continue;
+            bool q = false;
+            const int rc = engine_round(s, *ep, &q);
+            if (rc) {
+                // a device error fails the engine's queries; the server keeps serving others
+                s.error = rc;
+                for (Query *r : ep->waiting) {
+                    r->status = rc;
+                    finish_query(s, r, s.now());
+                }
+                ep->waiting.clear();
+                for (Query *&r : ep->slot_q)
+                    if (r) {
+                        r->status = rc;
+                        finish_query(s, r, s.now());
+                        r = nullptr;
+                    }
+                ep->running = 0;
+                s.cv_done.notify_all();
+            }
+            any |= q;
+            pending |= ep->inflight || ep->running > 0 || !ep->waiting.empty();
+        }
+        if (!any) {
+            // nothing queued this pass: wait briefly for events or new queries
+            s.cv_work.wait_for(lk, std::chrono::microseconds(pending ? 20 : 2000));
+        }
+    }
+}
+
+}  // namespace mg
+
+using namespace mg;
+
+struct mg_server : mg::Server {};
+
+extern "C" {
+
+void mg_server_config_default(mg_server_config *c) {
+    std::memset(c, 0, sizeof *c);
+    c->hbm_budget = 64LL << 30;
+    c->max_slots = MG_BATCH_MAX;
+    c->ppr_iters = 20;
+    c->ppr_teleport = 0.15f;
+    c->steps_per_round = STEPS_PER_GRAPH;
+    c->sssp_delta = 0;
+}
+
+int mg_server_create(const mg_server_config *cfg, mg_server **out) {
+    clear_error();
+    if (!out) return set_error(MG_ERR_INVALID_ARG, "null output pointer");
+    *out = nullptr;
+    mg_server *s = new (std::nothrow) mg_server();
+    if (!s) return set_error(MG_ERR_OOM, "host allocation failed");
+    if (cfg) s->cfg = *cfg;
+    else mg_server_config_default(&s->cfg);
+    if (s->cfg.hbm_budget <= 0 || s->cfg.max_slots < 1 || s->cfg.max_slots > MG_BATCH_MAX || s->cfg.ppr_iters < 1 ||
+        s->cfg.steps_per_round < 1 || s->cfg.steps_per_round > 64) {
+        delete s;
+        return set_error(MG_ERR_INVALID_ARG, "invalid server configuration");
+    }
+    if (hipGetDevice(&s->device) != hipSuccess) {
+        delete s;
+        return set_error(MG_ERR_NO_DEVICE, "no HIP device");
+    }
+    s->th = std::thread(dispatcher, static_cast<Server *>(s));
+    *out = s;
+    return MG_OK;
+}
+
+int mg_server_add_graph(mg_server *s, mg_graph *g, int32_t *graph_id) {
+    clear_error();
+    if (!s || !g || !graph_id) return set_error(MG_ERR_INVALID_ARG, "null argument");
+    if (g->device != s->device) return set_error(MG_ERR_INVALID_ARG, "graph lives on another device");
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int id = (int)s->graphs.size();
+    s->graphs.push_back(g);
+    for (int k = 0; k < 3; ++k) {
+        auto e = std::make_unique<Engine>();
+        e->srv = s;
+        e->g = g;
+        e->kind = k;
+        if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&e->ev, hipEventDisableTiming) != hipSuccess)
+            return set_error(MG_ERR_DEVICE, "stream/event creation failed");
+        s->engines.push_back(std::move(e));
+    }
+    *graph_id = id;
+    return MG_OK;
+}
+
+int mg_server_submit(mg_server *s, int32_t graph_id, int32_t kind, int32_t src, void *out, int32_t out_on_device,
+                     int64_t *query_id) {
+    clear_error();
+    if (!s || !query_id) return set_error(MG_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (graph_id < 0 || graph_id >= (int)s->graphs.size()) return set_error(MG_ERR_INVALID_ARG, "unknown graph %d", graph_id);
+    if (kind < MG_Q_BFS || kind > MG_Q_PPR) return set_error(MG_ERR_INVALID_ARG, "unknown query kind %d", kind);
+    mg_graph *g = s->graphs[graph_id];
+    if (src < 0 || src >= g->n) return set_error(MG_ERR_INVALID_ARG, "source %d outside [0,%d)", src, g->n);
+    if (kind == MG_Q_SSSP && (!g->w || g->min_weight < 0))
+        return set_error(MG_ERR_UNSUPPORTED, "SSSP needs non-negative edge weights");
+    if (kind == MG_Q_PPR && !(g->flags & MG_GRAPH_SYMMETRIC) && !g->col_off)
+        return set_error(MG_ERR_UNSUPPORTED, "personalized PageRank needs MG_GRAPH_SYMMETRIC or MG_GRAPH_BUILD_CSC");
+    auto q = std::make_unique<Query>();
+    q->id = s->next_id++;
+    q->graph = graph_id;
+    q->kind = kind;
+    q->src = src;
+    q->out = out;
+    q->out_on_device = out_on_device != 0;
+    q->t_submit = s->now();
+    s->engines[(size_t)graph_id * 3 + kind]->waiting.push_back(q.get());
+    *query_id = q->id;
+    s->queries.emplace(q->id, std::move(q));
+    s->stats.submitted++;
+    s->cv_work.notify_one();
+    return MG_OK;
+}
+
+static void fill_info(const Query &q, mg_query_info *info) {
+    if (!info) return;
+    std::memset(info, 0, sizeof *info);
+    info->status = q.status;
+    info->kind = q.kind;
+    info->graph = q.graph;
+    info->src = q.src;
+    info->edges_traversed = q.te;
+    info->vertices_reached = q.nr;
+    info->t_submit_ms = q.t_submit;
+    info->t_admit_ms = q.t_admit;
+    info->t_done_ms = q.t_done;
+}
+
+int mg_server_wait(mg_server *s, int64_t query_id, int32_t timeout_ms, mg_query_info *info) {
+    clear_error();
+    if (!s) return set_error(MG_ERR_INVALID_ARG, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->queries.find(query_id);
+    if (it == s->queries.end()) return set_error(MG_ERR_INVALID_ARG, "unknown query %lld", (long long)query_id);
+    Query *q = it->second.get();
+    auto ready = [&] { return q->done; };
+    if (timeout_ms < 0) s->cv_done.wait(lk, ready);
+    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+        return set_error(MG_ERR_TIMEOUT, "query %lld not done after %d ms", (long long)query_id, timeout_ms);
+    fill_info(*q, info);
+    const int st = q->status;
+    s->queries.erase(it);
+    if (st) return set_error(st, "query %lld failed", (long long)query_id);
+    return MG_OK;
+}
+
+int mg_server_now_ms(mg_server *s, double *ms) {
+    clear_error();
+    if (!s || !ms) return set_error(MG_ERR_INVALID_ARG, "null argument");
+    *ms = s->now();
+    return MG_OK;
+}
+
+int mg_server_get_stats(mg_server *s, mg_server_stats *st) {
+    clear_error();
+    if (!s || !st) return set_error(MG_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *st = s->stats;
+    st->device_bytes = s->bytes_used;
+    return MG_OK;
+}
+
+int mg_server_destroy(mg_server *s) {
+    clear_error();
+    if (!s) return MG_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_work.notify_all();
+    }
+    if (s->th.joinable()) s->th.join();
+    (void)hipSetDevice(s->device);
+    for (auto &e : s->engines) {
+        if (!e) continue;
+        if (e->st) (void)hipStreamSynchronize(e->st);
+        engine_free(*s, *e);
+        if (e->ev) (void)hipEventDestroy(e->ev);
+        if (e->st) (void)hipStreamDestroy(e->st);
+    }
+    delete s;
+    return MG_OK;
+}
+
+void *mg_host_alloc(size_t bytes) {
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+    return p;
+}
+
+void mg_host_free(void *p) {
+    if (p) (void)hipHostFree(p);
+}
+
+}  // extern "C"

@@ -140,3 +140,20 @@ def pr_jacobi_f64(row_offsets, col_idx, iters=50, teleport=0.15, tol=0.0):
 def teps_edges(row_offsets, labels, unreached=-1):
     ro, lb = _i32(row_offsets), _i32(labels)
     return int(lib().oracle_teps_edges(len(ro) - 1, _p(ro), _p(lb), unreached))
+
+
+def ppr(row_offsets, col_idx, src, iters=20, teleport=0.15):
+    """Personalized PageRank in float64: x_0 = e_src, x_{i+1} = t e_src + (1 - t) A^T D^-1 x_i over the
+    CSR edges v -> u (rank of out-degree-0 vertices is dropped, as in PageRank.h:233-283)."""
+    ro, ci = np.asarray(row_offsets, np.int64), np.asarray(col_idx, np.int64)
+    n = len(ro) - 1
+    deg = np.diff(ro)
+    rows = np.repeat(np.arange(n), deg)
+    x = np.zeros(n)
+    x[src] = 1.0
+    inv = np.where(deg > 0, 1.0 / np.maximum(deg, 1), 0.0)
+    for _ in range(iters):
+        y = np.bincount(ci, weights=(x * inv)[rows], minlength=n) * (1.0 - teleport)
+        y[src] += teleport
+        x = y
+    return x

@@ -219,3 +219,39 @@ def test_sssp_batch_rmat20_256_sources(mg, device):
         reached[srcs[s]] = False
         assert np.all(tight[reached])
     _check_sssp_stats(g, got, st)
+
+
+# ---------------------------------------------------------------- batched personalized PageRank
+PPR_TOL = 1e-6  # max-abs against the float64 oracle (ranks sum to at most 1)
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_ppr_batch_against_float64(mg, oracle, device, symmetric):
+    g = mg.rmat_graph(12, 16, seed=5, symmetric=symmetric)
+    srcs = mg.pick_sources(g, 64, seed=8)
+    with mg.Graph(g, symmetric=symmetric, build_csc=not symmetric) as dg:
+        got, st = dg.ppr_batch(srcs, stats=True)
+    assert st.iterations == 20
+    for j in range(0, 64, 7):
+        exp = oracle.ppr(g.row_offsets, g.col_idx, int(srcs[j]), iters=20, teleport=0.15)
+        assert np.max(np.abs(got[j].astype(np.float64) - exp)) <= PPR_TOL, j
+
+
+def test_ppr_batch_rows_do_not_depend_on_companions(mg, device):
+    # the same source gives the same bits alone, in a batch, and in another slot
+    g = mg.rmat_graph(13, 16, seed=2)
+    srcs = mg.pick_sources(g, 40, seed=4)
+    with mg.Graph(g, symmetric=True) as dg:
+        full = dg.ppr_batch(srcs)
+        for j in (0, 17, 39):
+            alone = dg.ppr_batch(srcs[j:j + 1])
+            assert np.array_equal(full[j].view(np.int32), alone[0].view(np.int32)), j
+        rev = dg.ppr_batch(srcs[::-1].copy())
+        assert np.array_equal(full.view(np.int32), rev[::-1].view(np.int32))
+        # other iteration counts and teleport values
+        o = mg.options(ppr_iters=3, pr_teleport=0.3)
+        x = dg.ppr_batch(srcs[:5], o)
+    import oracle as orc
+    for j in range(5):
+        exp = orc.ppr(g.row_offsets, g.col_idx, int(srcs[j]), iters=3, teleport=0.3)
+        assert np.max(np.abs(x[j] - exp)) <= PPR_TOL

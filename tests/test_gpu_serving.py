@@ -1,0 +1,147 @@
+"""Serving on the GPU: queries on several resident graphs, submitted together or as a
+timed stream, are served with continuous batching; every result must equal the
+single-query result (BFS depths and SSSP distances bit-exact, personalized PageRank
+bitwise equal to a one-source mg_ppr_batch call), and later queries must join
+batches already running."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def graphs(mg):
+    hosts = [mg.rmat_graph(12, 16, seed=21, weight_max=64), mg.grid_graph(48, seed=22, weight_max=16),
+             mg.rmat_graph(10, 8, seed=23, weight_max=8)]
+    dgs = [mg.Graph(h, symmetric=True) for h in hosts]
+    yield hosts, dgs
+    for d in dgs:
+        d.close()
+
+
+def _alone(mg, dg, kind, src):
+    if kind == mg.MG_Q_BFS:
+        return dg.bfs(src)
+    if kind == mg.MG_Q_SSSP:
+        return dg.sssp(src)
+    return dg.ppr_batch([src])[0]
+
+
+def _same(mg, kind, got, exp):
+    if kind == mg.MG_Q_PPR:
+        return np.array_equal(got.view(np.int32), exp.view(np.int32))
+    return np.array_equal(got, exp)
+
+
+def _queries(mg, hosts, count, seed):
+    rng = np.random.default_rng(seed)
+    qs = []
+    for _ in range(count):
+        gi = int(rng.integers(0, len(hosts)))
+        kind = int(rng.choice([mg.MG_Q_BFS, mg.MG_Q_SSSP, mg.MG_Q_PPR], p=[0.6, 0.3, 0.1]))
+        src = int(mg.pick_sources(hosts[gi], 1, seed=int(rng.integers(1 << 30)))[0])
+        qs.append((gi, kind, src))
+    return qs
+
+
+def test_serving_mixed_equals_alone(mg, device, graphs):
+    hosts, dgs = graphs
+    qs = _queries(mg, hosts, 150, seed=1)
+    with mg.Server(max_slots=16, steps_per_round=2) as srv:
+        ids = [srv.add_graph(d) for d in dgs]
+        outs, qids = [], []
+        for gi, kind, src in qs:
+            out = mg.PinnedArray(hosts[gi].n, np.float32 if kind == mg.MG_Q_PPR else np.int32)
+            outs.append(out)
+            qids.append(srv.submit(ids[gi], kind, src, out))
+        infos = [srv.wait(q, 60000) for q in qids]
+        st = srv.stats()
+    assert st["completed"] == len(qs)
+    for (gi, kind, src), out, info in zip(qs, outs, infos):
+        exp = _alone(mg, dgs[gi], kind, src)
+        assert _same(mg, kind, out.array, exp), (gi, kind, src)
+        assert info["status"] == 0 and info["t_submit_ms"] <= info["t_admit_ms"] <= info["t_done_ms"]
+        if kind != mg.MG_Q_PPR:
+            reached = exp >= 0 if kind == mg.MG_Q_BFS else exp != 2**31 - 1
+            assert info["vertices_reached"] == int(reached.sum())
+            assert info["edges_traversed"] == int(np.diff(hosts[gi].row_offsets)[reached].sum())
+    # continuous batching: with 16 slots per engine, some query was admitted while
+    # an earlier query of the same engine was still running
+    joined = 0
+    for a, (qa, ia) in enumerate(zip(qs, infos)):
+        for qb, ib in zip(qs[a + 1:], infos[a + 1:]):
+            if qa[:2] == qb[:2] and ia["t_admit_ms"] < ib["t_admit_ms"] < ia["t_done_ms"]:
+                joined += 1
+                break
+    assert joined > 0
+
+
+def test_serving_timed_stream(mg, device, graphs):
+    hosts, dgs = graphs
+    qs = _queries(mg, hosts, 200, seed=2)
+    rng = np.random.default_rng(3)
+    gaps = rng.exponential(1.0 / 400.0, len(qs))  # Poisson arrivals, 400 queries/s
+    with mg.Server() as srv:
+        ids = [srv.add_graph(d) for d in dgs]
+        qids = [None] * len(qs)
+        outs = [mg.PinnedArray(hosts[gi].n, np.float32 if k == mg.MG_Q_PPR else np.int32) for gi, k, _ in qs]
+
+        def client():
+            t = time.perf_counter()
+            for i, (gi, kind, src) in enumerate(qs):
+                t += gaps[i]
+                d = t - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                qids[i] = srv.submit(ids[gi], kind, src, outs[i])
+
+        th = threading.Thread(target=client)
+        th.start()
+        th.join()
+        infos = [srv.wait(q, 60000) for q in qids]
+    lat = np.array([i["t_done_ms"] - i["t_submit_ms"] for i in infos])
+    assert np.all(lat > 0)
+    for i in range(0, len(qs), 10):
+        gi, kind, src = qs[i]
+        assert _same(mg, kind, outs[i].array, _alone(mg, dgs[gi], kind, src)), qs[i]
+
+
+def test_serving_small_hbm_budget(mg, device):
+    hosts = [mg.rmat_graph(10, 8, seed=31, weight_max=16), mg.rmat_graph(10, 8, seed=32, weight_max=16)]
+    dgs = [mg.Graph(h, symmetric=True) for h in hosts]
+    budget = 3 << 20
+    try:
+        with mg.Server(hbm_budget=budget) as srv:
+            ids = [srv.add_graph(d) for d in dgs]
+            qs = _queries(mg, hosts, 60, seed=4)
+            outs = [mg.PinnedArray(hosts[gi].n, np.float32 if k == mg.MG_Q_PPR else np.int32) for gi, k, _ in qs]
+            qids = [srv.submit(ids[gi], k, s, o) for (gi, k, s), o in zip(qs, outs)]
+            for q in qids:
+                srv.wait(q, 60000)
+            assert srv.stats()["device_bytes"] <= budget
+        for (gi, kind, src), o in zip(qs, outs):
+            assert _same(mg, kind, o.array, _alone(mg, dgs[gi], kind, src))
+    finally:
+        for d in dgs:
+            d.close()
+
+
+def test_serving_rejects_bad_queries(mg, device, graphs):
+    hosts, dgs = graphs
+    unweighted = mg.Graph(mg.rmat_graph(8, 4, seed=1), symmetric=True)
+    try:
+        with mg.Server() as srv:
+            gid = srv.add_graph(dgs[0])
+            uid = srv.add_graph(unweighted)
+            for args in [(gid + 7, mg.MG_Q_BFS, 0), (gid, 5, 0), (gid, mg.MG_Q_BFS, hosts[0].n), (uid, mg.MG_Q_SSSP, 0)]:
+                with pytest.raises(mg.MgError):
+                    srv.submit(*args)
+            with pytest.raises(mg.MgError):
+                srv.wait(12345, 10)
+            q = srv.submit(uid, mg.MG_Q_BFS, 0)  # no output buffer: result discarded
+            assert srv.wait(q, 60000)["status"] == 0
+    finally:
+        unweighted.close()

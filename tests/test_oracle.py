@@ -219,3 +219,35 @@ def test_pr_gas_semantics(mg, oracle):
     r1, it1 = oracle.pr_gas(g.row_offsets, g.col_idx, iter_num=1)
     j1, _, _ = oracle.pr_jacobi_f32(g.row_offsets, g.col_idx, iters=1)
     assert it1 == 1 and np.array_equal(r1, j1)
+
+
+# ---------------------------------------------------------------- personalized PageRank
+def test_ppr_two_cycle_closed_form(oracle):
+    # 0 <-> 1 from source 0: a' = t + (1-t) b, b' = (1-t) a
+    ro, ci = np.array([0, 1, 2]), np.array([1, 0])
+    t = 0.15
+    a, b = 1.0, 0.0
+    for k in range(1, 8):
+        a, b = t + (1 - t) * b, (1 - t) * a
+        x = oracle.ppr(ro, ci, 0, iters=k, teleport=t)
+        assert abs(x[0] - a) < 1e-15 and abs(x[1] - b) < 1e-15
+
+
+def test_ppr_star_closed_form(oracle):
+    # centre 0, L leaves of degree 1, from the centre: c' = t + (1-t) L l, l' = (1-t) c / L
+    L, t = 5, 0.2
+    ro = np.concatenate([[0, L], L + np.arange(1, L + 1)])
+    ci = np.concatenate([np.arange(1, L + 1), np.zeros(L, np.int64)])
+    c, leaf = 1.0, 0.0
+    for k in range(1, 6):
+        c, leaf = t + (1 - t) * L * leaf, (1 - t) * c / L
+        x = oracle.ppr(ro, ci, 0, iters=k, teleport=t)
+        assert abs(x[0] - c) < 1e-14 and np.allclose(x[1:], leaf, atol=1e-14)
+
+
+def test_ppr_mass_is_conserved_without_dangling_vertices(mg, oracle):
+    g = mg.rmat_graph(9, 8, seed=3)
+    src = int(np.argmax(g.degrees()))
+    for k in (1, 7, 20):
+        x = oracle.ppr(g.row_offsets, g.col_idx, src, iters=k)
+        assert abs(x.sum() - 1.0) < 1e-12 and x.min() >= 0.0
