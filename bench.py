@@ -70,6 +70,98 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+def batched_part(mg, seed: int, log) -> dict:
+    """256 BFS and 256 SSSP sources (seeded draws among non-isolated vertices) on the R-MAT scale-20 graph
+    and the 2048x2048 grid: one batched call each against the same sources run one by one (plain)."""
+    out = {}
+    for name, host in [("rmat20", mg.rmat_graph(20, 16, seed=seed, weight_max=64)),
+                       ("grid2048", mg.grid_graph(2048, seed=seed, weight_max=16))]:
+        srcs = mg.pick_sources(host, 256, seed=seed)
+        with mg.Graph(host, symmetric=True) as dg:
+            res = {"n": host.n, "m": host.m, "sources": 256}
+            for kind, batch, plain in [("bfs", dg.bfs_batch, dg.bfs), ("sssp", dg.sssp_batch, dg.sssp)]:
+                batch(srcs, copy=False)
+                ts = []
+                for _ in range(3):
+                    _, st = batch(srcs, stats=True, copy=False)
+                    ts.append(st.device_ms)
+                tb = float(np.median(ts))
+                te = st.edges_traversed
+                # plain: every source on the R-MAT graph; a 32-source sample on the grid (seconds per source)
+                sample = srcs if name == "rmat20" else srcs[:32]
+                plain(int(sample[0]), copy=False)
+                tp = sum(plain(int(v), stats=True, copy=False)[1].device_ms for v in sample) * (256 / len(sample))
+                res[kind] = {"batched_ms": tb, "batched_gteps": te / tb / 1e6, "plain_ms": tp,
+                             "plain_gteps": te / tp / 1e6, "speedup": tp / tb, "levels": st.iterations,
+                             "plain_sources_timed": len(sample)}
+                log(f"batched {name} {kind}: {tb:.2f} ms vs plain {tp:.1f} ms ({tp / tb:.1f}x)")
+        out[name] = res
+    return out
+
+
+def serving_part(mg, seed: int, seconds: float, rate: float, log) -> dict:
+    """Poisson query stream (seeded) over six resident graphs -- R-MAT scales 16, 18, 20, 22 (symmetrized,
+    weights 1..64) and 1024^2 / 2048^2 grids (weights 1..16) -- with 60% BFS, 30% SSSP and 10% personalized
+    PageRank (FP32, 20 iterations) from sources drawn among non-isolated vertices.  Latency = result copied
+    to pinned host memory minus scheduled arrival."""
+    import threading
+
+    hosts = [mg.rmat_graph(s, 16, seed=seed + s, weight_max=64) for s in (16, 18, 20, 22)]
+    hosts += [mg.grid_graph(w, seed=seed + w, weight_max=16) for w in (1024, 2048)]
+    names = ["rmat16", "rmat18", "rmat20", "rmat22", "grid1024", "grid2048"]
+    dgs = [mg.Graph(h, symmetric=True) for h in hosts]
+    rng = np.random.default_rng(seed)
+    nq = int(rng.poisson(rate * seconds))
+    gaps = rng.exponential(1.0 / rate, nq)
+    gsel = rng.integers(0, len(hosts), nq)
+    kinds = rng.choice([mg.MG_Q_BFS, mg.MG_Q_SSSP, mg.MG_Q_PPR], p=[0.6, 0.3, 0.1], size=nq)
+    pools = [mg.pick_sources(h, 4096, seed=seed + i) for i, h in enumerate(hosts)]
+    srcs = [int(pools[g][i % 4096]) for i, g in enumerate(gsel)]
+    bufs = [[mg.PinnedArray(h.n) for _ in range(4)] for h in hosts]  # result buffers, reused round-robin
+    qids = [0] * nq
+    arrive = np.cumsum(gaps) * 1e3  # ms after the stream starts
+    with mg.Server() as srv:
+        ids = [srv.add_graph(d) for d in dgs]
+        t_start = srv.now_ms()
+
+        def client():
+            t = time.perf_counter()
+            for i in range(nq):
+                t += gaps[i]
+                d = t - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                g = int(gsel[i])
+                qids[i] = srv.submit(ids[g], int(kinds[i]), srcs[i], bufs[g][i % 4])
+
+        th = threading.Thread(target=client)
+        th.start()
+        th.join()
+        infos = [srv.wait(q, 600000) for q in qids]
+        st = srv.stats()
+    for d in dgs:
+        d.close()
+    # latency from the scheduled arrival (a late submit call counts against the server)
+    lat = np.array([i["t_done_ms"] - (t_start + arrive[k]) for k, i in enumerate(infos)])
+    t_end = max(i["t_done_ms"] for i in infos)
+    te = sum(i["edges_traversed"] for i in infos)
+    res = {"queries": nq, "rate_qps": rate, "arrival_window_s": seconds, "p50_ms": float(np.percentile(lat, 50)),
+           "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+           "aggregate_gteps": te / (t_end - t_start) / 1e6, "completed_qps": nq / ((t_end - t_start) / 1e3),
+           "device_bytes": st["device_bytes"], "graphs": names}
+    per = {}
+    for k, kn in [(mg.MG_Q_BFS, "bfs"), (mg.MG_Q_SSSP, "sssp"), (mg.MG_Q_PPR, "ppr")]:
+        for g, gn in enumerate(names):
+            sel = (kinds == k) & (gsel == g)
+            if sel.any():
+                per[f"{kn}_{gn}"] = {"n": int(sel.sum()), "p50_ms": float(np.percentile(lat[sel], 50)),
+                                     "p99_ms": float(np.percentile(lat[sel], 99))}
+    res["per_kind_graph"] = per
+    log(f"serving: {nq} queries, p50 {res['p50_ms']:.2f} ms, p99 {res['p99_ms']:.2f} ms, "
+        f"{res['aggregate_gteps']:.1f} GTEPS")
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +170,11 @@ def main() -> int:
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--parts", default="batch,serve",
+                    help="extra measurements beside the main step: batch (256 sources, batched vs plain) and "
+                         "serve (Poisson query stream over six resident graphs); '' for none")
+    ap.add_argument("--serve-seconds", type=float, default=4.0, help="length of the serving arrival window")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="mean query arrival rate (queries/s)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="for counter runs: one warm-up and one measured SSSP, CC and BFS search, then a "
@@ -187,6 +284,12 @@ def main() -> int:
                "sample": f"the full step on the same graph: SSSP from vertex 0 (binary-heap Dijkstra restatement of "
                          f"CPUSSSP, {t_sssp:.2f}s) + CC (CPUCC restatement, {t_cc:.2f}s), one thread",
                "cpu_model": cpu_model()}
+
+    parts_wanted = [p for p in a.parts.split(",") if p]
+    if "batch" in parts_wanted:
+        parts["batched"] = batched_part(mg, a.seed, log)
+    if "serve" in parts_wanted:
+        parts["serving"] = serving_part(mg, a.seed, a.serve_seconds, a.serve_rate, log)
 
     line = {
         "metric": METRIC, "value": value, "unit": "GTEPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,

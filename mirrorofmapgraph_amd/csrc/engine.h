@@ -109,6 +109,16 @@ struct Workspace {
     ChunkRec *pull_tasks = nullptr;
     int pull_ntasks = 0;
     const int32_t *pull_view = nullptr;
+    // XCD-sliced pull: in-lists cut by source slice (PULL_SLICES slices of slice_len
+    // vertices, slice x gathered by the workgroups on XCD x)
+    int32_t *slice_off = nullptr;      // (PULL_SLICES - 1) x n interior cut points
+    int32_t *slice_min = nullptr;      // PULL_SLICES x n per-slice partial results
+    ChunkRec *slice_tasks = nullptr;   // long (vertex, slice) ranges, grouped by slice
+    int32_t slice_task_off[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int32_t *d_slice_task_off = nullptr;
+    int slice_len = 0;
+    const int32_t *slice_view = nullptr;  // in-edge offsets the slices were built for (nullptr: none)
+    bool slice_ok = false;                // lists sorted, so slicing applies
 };
 
 }  // namespace mg
@@ -159,6 +169,7 @@ int deliver(mg_graph *g, void *dst, const void *src, size_t bytes, bool out_on_d
 // arguments are fixed per (graph, kind, key), so the executable graph is reused by
 // every later search with the same key.
 constexpr int STEPS_PER_GRAPH = 8;
+constexpr int PULL_SLICES = 8;  // one source slice per XCD (workgroup b runs on XCD b % 8)
 enum { KIND_BFS = 0, KIND_SSSP = 1, KIND_PR = 2, KIND_GAS = 3, KIND_BATCH = 4, KIND_BATCH_SSSP = 5, NKINDS = 8 };
 
 struct StepGraph {
@@ -170,6 +181,8 @@ template <class Step>
 int drive(mg_graph *g, Step step, int64_t max_steps, int kind = -1, uint64_t key = 0, Ctl *ctl = nullptr,
           Ctl *h_ctl = nullptr);
 bool graphs_enabled();
+// MG_NO_SLICED_PULL=1 keeps pull levels unsliced (for comparison runs).
+bool pull_slicing_disabled();
 
 // Generic GAS enactor over the ported programs (engine = MG_ENGINE_GAS).
 struct GasStateBase;

@@ -62,6 +62,10 @@ static void free_workspace(Workspace &ws) {
     dfree(ws.pr_pact);
     dfree(ws.pr_bigv);
     dfree(ws.pull_tasks);
+    dfree(ws.slice_off);
+    dfree(ws.slice_min);
+    dfree(ws.slice_tasks);
+    dfree(ws.d_slice_task_off);
     ws = Workspace();
 }
 
@@ -115,6 +119,14 @@ bool graphs_enabled() {
         return !(e && *e && *e != '0');
     }();
     return on;
+}
+
+bool pull_slicing_disabled() {
+    static const bool off = [] {
+        const char *e = getenv("MG_NO_SLICED_PULL");
+        return e && *e && *e != '0';
+    }();
+    return off;
 }
 
 int begin_search(mg_graph *g, const mg_options *opt, mg_options *resolved) {

@@ -31,7 +31,13 @@ struct SsspArgs {
     const int *w;
     const ChunkRec *pull;  // static pull tasks (in-edge lists longer than LANE_T)
     int npull;
-    int G;
+    int G;                 // a multiple of PULL_SLICES
+    // XCD-sliced pull (sliced = 1): workgroup b gathers from source slice b % 8 only
+    int sliced;
+    const int *slice_off;        // (PULL_SLICES - 1) x n cut points in the in-lists
+    int *slice_min;              // PULL_SLICES x n partial minima
+    const ChunkRec *slice_tasks; // long ranges, grouped by slice
+    int sto[PULL_SLICES + 1];    // slice task offsets
 };
 
 __device__ __forceinline__ long long edge_w(const SsspArgs &a, int view, int slot) {
@@ -137,7 +143,8 @@ __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
     __shared__ int s_go, s_L, s_qv, s_fin, s_cnt;
     if (c->pending) {
         const int L = c->level;
-        const int na = (c->dir == DIR_PULL ? 1 : fa.nviews) * a.G;
+        // pull levels: level_a (and level_c when sliced) slots; push: one slot set per view
+        const int na = (c->dir == DIR_PULL ? (a.sliced ? 2 : 1) : fa.nviews) * a.G;
         long long nf = 0, mf = 0, ex = 0;
         for (int i = threadIdx.x; i < na + a.G; i += SMALL_BLK) {
             const LevelPart p = fa.part[i < na ? i : 2 * a.G + (i - na)];
@@ -217,6 +224,33 @@ __global__ __launch_bounds__(BLK) void sssp_level_a(SsspArgs a, int view) {
         write_part<BLK>(fa, view * a.G + blockIdx.x, pol.nf, pol.mf, pol.ex);
         return;
     }
+    if (a.sliced) {
+        // XCD-sliced pull: this workgroup scans, for every vertex, the part of its
+        // in-list whose sources lie in slice x = blockIdx % 8, so the dist[] and
+        // frontier-bit gathers stay inside one 1/8 of the arrays, resident in this
+        // XCD's L2.  Ranges longer than LANE_T are sssp_level_b tasks.
+        const uint32_t *fr = fa.bm[L % 3];
+        const int *off = fa.off[2];
+        const int x = blockIdx.x % PULL_SLICES, nbx = gridDim.x / PULL_SLICES;
+        int *pm = a.slice_min + (int64_t)x * fa.n;
+        const int *c0 = x == 0 ? nullptr : a.slice_off + (int64_t)(x - 1) * fa.n;
+        const int *c1 = x == PULL_SLICES - 1 ? nullptr : a.slice_off + (int64_t)x * fa.n;
+        long long ex = 0;
+        for (int base = (blockIdx.x / PULL_SLICES) * BLK; base < fa.n; base += nbx * BLK) {
+            const int v = base + (int)threadIdx.x;
+            if (v < fa.n) {
+                const int beg = c0 ? c0[v] : off[v], end = c1 ? c1[v] : off[v + 1];
+                long long best = INT_MAX;
+                if (end - beg <= LANE_T) {
+                    ex += end - beg;
+                    best = pull_best(a, fr, beg, end, 1);
+                }
+                pm[v] = (int)best;  // long ranges stay INT_MAX until their tasks
+            }
+        }
+        write_part<BLK>(fa, blockIdx.x, 0, 0, ex);
+        return;
+    }
     // pull, vertices of in-degree <= LANE_T, one lane each in vertex order; each
     // wave owns the two bitmap words of its 64 vertices.  Longer in-edge lists are
     // the static tasks of sssp_level_b.
@@ -260,6 +294,26 @@ __global__ __launch_bounds__(BLK) void sssp_level_b(SsspArgs a) {
         nf = pol.nf;
         mf = pol.mf;
         ex = pol.ex;
+    } else if (a.sliced) {
+        // long (vertex, slice) ranges of this workgroup's slice: wave minimum, one
+        // atomicMin into the slice's partial
+        const uint32_t *fr = fa.bm[L % 3];
+        const int lane = lane_id();
+        const int x = blockIdx.x % PULL_SLICES;
+        const int wave = (blockIdx.x / PULL_SLICES) * (BLK / WAVE) + (int)threadIdx.x / WAVE;
+        const int nwaves = (gridDim.x / PULL_SLICES) * (BLK / WAVE);
+        int *pm = a.slice_min + (int64_t)x * fa.n;
+        for (int t = a.sto[x] + wave; t < a.sto[x + 1]; t += nwaves) {
+            const ChunkRec r = a.slice_tasks[t];
+            ex += (r.end - r.beg + WAVE - 1 - lane) / WAVE;
+            long long best = pull_best(a, fr, r.beg + lane, r.end, WAVE);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const long long y = __shfl_xor(best, o);
+                best = y < best ? y : best;
+            }
+            if (lane == 0 && best < INT_MAX) atomicMin(&pm[r.v], (int)best);
+        }
     } else {
         // static in-edge tasks: wave minimum, one atomicMin per task
         const uint32_t *fr = fa.bm[L % 3];
@@ -287,6 +341,36 @@ __global__ __launch_bounds__(BLK) void sssp_level_b(SsspArgs a) {
     if (blockIdx.x == 0 && threadIdx.x == 0) c->pending = 1;
 }
 
+// Sliced pull levels only: dist[v] = min(dist[v], the slices' partial minima);
+// improved vertices form the next frontier (each wave writes its 64 vertices' words).
+__global__ __launch_bounds__(BLK) void sssp_level_c(SsspArgs a) {
+    Ctl *c = a.fa.ctl;
+    const FrontierArgs &fa = a.fa;
+    if (c->done || c->dir != DIR_PULL || !a.sliced) return;
+    uint32_t *nx = fa.bm[(c->level + 1) % 3];
+    const int lane = lane_id();
+    long long nf = 0, mf = 0;
+    for (int base = blockIdx.x * BLK; base < fa.n; base += gridDim.x * BLK) {
+        const int v = base + (int)threadIdx.x;
+        bool found = false;
+        if (v < fa.n) {
+            int best = a.slice_min[v];
+#pragma unroll
+            for (int x = 1; x < PULL_SLICES; ++x) best = min(best, a.slice_min[(int64_t)x * fa.n + v]);
+            if (best < a.dist[v]) {
+                a.dist[v] = best;
+                found = true;
+                ++nf;
+                mf += view_degree(fa, v);
+            }
+        }
+        const uint64_t fm = wave_ballot(found);
+        const uint32_t mine = (lane < 32) ? (uint32_t)fm : (uint32_t)(fm >> 32);
+        if ((lane & 31) == 0 && mine) nx[v >> 5] = mine;
+    }
+    write_part<BLK>(fa, a.G + blockIdx.x, nf, mf, 0);
+}
+
 }  // namespace mg
 
 using namespace mg;
@@ -309,12 +393,22 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
     a.fa = make_frontier_args(g, o.directed);
     a.dist = ws.vals;
     a.w = g->w;
-    a.G = grid_blocks(std::max<int64_t>(g->n / 4, g->m / 64), BLK);
+    a.G = std::max(PULL_SLICES, grid_blocks(std::max<int64_t>(g->n / 4, g->m / 64), BLK) / PULL_SLICES * PULL_SLICES);
     const int can_pull = a.fa.off[2] != nullptr && a.fa.nviews == 1 && o.engine != MG_ENGINE_GAS;
     if (can_pull) {
         if ((rc = ensure_pull_tasks(g, a.fa.off[2]))) return rc;
         a.pull = ws.pull_tasks;
         a.npull = ws.pull_ntasks;
+        if (!pull_slicing_disabled()) {
+            if ((rc = ensure_pull_slices(g, a.fa.off[2], a.fa.adj[2]))) return rc;
+            if (ws.slice_ok) {
+                a.sliced = 1;
+                a.slice_off = ws.slice_off;
+                a.slice_min = ws.slice_min;
+                a.slice_tasks = ws.slice_tasks;
+                for (int x = 0; x <= PULL_SLICES; ++x) a.sto[x] = ws.slice_task_off[x];
+            }
+        }
     }
 
     MG_HIP(hipEventRecord(g->ev0, g->stream));
@@ -328,11 +422,12 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
         for (int v = 0; v < a.fa.nviews; ++v)
             hipLaunchKernelGGL(sssp_level_a, dim3(a.G), dim3(BLK), 0, g->stream, a, v);
         hipLaunchKernelGGL(sssp_level_b, dim3(a.G), dim3(BLK), 0, g->stream, a);
+        if (a.sliced) hipLaunchKernelGGL(sssp_level_c, dim3(a.G), dim3(BLK), 0, g->stream, a);
         MG_HIP(hipGetLastError());
         return MG_OK;
     };
     // Bellman-Ford with non-negative weights settles within n rounds.
     const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n + 1) + 3;
-    if ((rc = drive(g, step, max_steps, KIND_SSSP, (uint64_t)o.directed))) return rc;
+    if ((rc = drive(g, step, max_steps, KIND_SSSP, (uint64_t)o.directed * 2 + (uint64_t)a.sliced))) return rc;
     return finish_search(g, stats, ws.vals, INT_MAX, dist_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
 }
