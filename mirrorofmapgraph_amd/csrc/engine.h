@@ -113,7 +113,10 @@ struct Workspace {
     // vertices, slice x gathered by the workgroups on XCD x)
     int32_t *slice_off = nullptr;      // (PULL_SLICES - 1) x n interior cut points
     int32_t *slice_min = nullptr;      // PULL_SLICES x n per-slice partial results
-    ChunkRec *slice_tasks = nullptr;   // long (vertex, slice) ranges, grouped by slice
+    ChunkRec *slice_tasks = nullptr;   // TASK-edge pieces of long (vertex, slice) ranges, grouped by slice
+    ChunkRec *slice_segs = nullptr;    // per long range: {vertex, first piece, pieces, slice}
+    float *slice_part = nullptr;       // per piece partial (PageRank)
+    int nslice_segs = 0;
     int32_t slice_task_off[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int32_t *d_slice_task_off = nullptr;
     int slice_len = 0;

@@ -65,6 +65,8 @@ static void free_workspace(Workspace &ws) {
     dfree(ws.slice_off);
     dfree(ws.slice_min);
     dfree(ws.slice_tasks);
+    dfree(ws.slice_segs);
+    dfree(ws.slice_part);
     dfree(ws.d_slice_task_off);
     ws = Workspace();
 }

@@ -147,7 +147,7 @@ int ensure_pull_slices(mg_graph *g, const int *d_in_off, const int *d_in_adj) {
         ws.slice_view = d_in_off;  // unsorted lists: the plain pull stays
         return MG_OK;
     }
-    std::vector<ChunkRec> all;
+    std::vector<ChunkRec> all, segs;
     int32_t sto[PULL_SLICES + 1];
     for (int x = 0; x < PULL_SLICES; ++x) {
         sto[x] = (int32_t)all.size();
@@ -157,18 +157,31 @@ int ensure_pull_slices(mg_graph *g, const int *d_in_off, const int *d_in_adj) {
         }
     }
     sto[PULL_SLICES] = (int32_t)all.size();
+    // the pieces of one (vertex, slice) range are consecutive: one segment each
+    for (size_t i = 0; i < all.size();) {
+        size_t j = i + 1;
+        while (j < all.size() && all[j].v == all[i].v && all[j].pad == all[i].pad) ++j;
+        segs.push_back(ChunkRec{all[i].v, (int)i, (int)(j - i), all[i].pad});
+        i = j;
+    }
     free_dev(ws.slice_off);
     free_dev(ws.slice_min);
     free_dev(ws.slice_tasks);
+    free_dev(ws.slice_segs);
+    free_dev(ws.slice_part);
     ws.slice_off = ws.slice_min = nullptr;
-    ws.slice_tasks = nullptr;
+    ws.slice_tasks = ws.slice_segs = nullptr;
+    ws.slice_part = nullptr;
     ws.slice_view = nullptr;
     int rc;
     if ((rc = alloc_i32(&ws.slice_off, cut.size())) || (rc = alloc_i32(&ws.slice_min, (size_t)PULL_SLICES * n)) ||
-        (rc = alloc_chunks(&ws.slice_tasks, all.size())))
+        (rc = alloc_chunks(&ws.slice_tasks, all.size())) || (rc = alloc_chunks(&ws.slice_segs, segs.size())) ||
+        (rc = alloc_f32(&ws.slice_part, all.size())))
         return rc;
     if (!cut.empty()) MG_HIP(hipMemcpy(ws.slice_off, cut.data(), 4 * cut.size(), hipMemcpyHostToDevice));
     if (!all.empty()) MG_HIP(hipMemcpy(ws.slice_tasks, all.data(), sizeof(ChunkRec) * all.size(), hipMemcpyHostToDevice));
+    if (!segs.empty()) MG_HIP(hipMemcpy(ws.slice_segs, segs.data(), sizeof(ChunkRec) * segs.size(), hipMemcpyHostToDevice));
+    ws.nslice_segs = (int)segs.size();
     for (int x = 0; x <= PULL_SLICES; ++x) ws.slice_task_off[x] = sto[x];
     ws.slice_len = SL;
     ws.slice_view = d_in_off;

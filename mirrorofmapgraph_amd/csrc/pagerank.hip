@@ -23,6 +23,12 @@
 //                        sums added in chunk order.
 // MG_PR_ALL updates every vertex each iteration (all-vertex Jacobi) and can stop on
 // the L1 change, reduced in a fixed order as well.
+// With sorted in-lists the gather is XCD-sliced instead: every in-list is cut into
+// 8 ranges by source slice, workgroup b sums the ranges of slice b % 8 (so its
+// contribution gathers stay within 1/8 of the array, resident in its XCD's L2),
+// and a vertex adds its 8 range sums in slice order.  Ranges up to LANE_T edges
+// are summed by one lane in order; longer ones in TASK-edge pieces (lane-strided,
+// xor-butterfly) added in piece order.  This order is fixed as well.
 #include <vector>
 
 #include "frontier.h"
@@ -49,6 +55,15 @@ struct PrArgs {
     int mode;
     float base, damp, eps;
     double tol_n;
+    // XCD-sliced gather (sliced = 1): workgroup b gathers from source slice b % 8
+    int sliced;
+    const int *slice_off;          // (PULL_SLICES - 1) x n cut points in the in-lists
+    float *sp;                     // PULL_SLICES x n partial sums, sign bit = an in-neighbour changed
+    const ChunkRec *stasks;        // TASK-edge pieces of long ranges, grouped by slice
+    const ChunkRec *segs;          // long ranges: {vertex, first piece, pieces, slice}
+    float *spart;                  // per piece partial (sign bit as in sp)
+    int nsegs;
+    int sto[PULL_SLICES + 1];
 };
 
 struct PrIter {
@@ -193,7 +208,7 @@ __global__ __launch_bounds__(SMALL_BLK) void pr_k3(PrArgs a) {
     const bool all = a.mode == MG_PR_ALL || I.it == 0;
     double delta = 0;
     int nchanged = 0;
-    for (int k = threadIdx.x; k < a.nbig; k += SMALL_BLK) {
+    for (int k = threadIdx.x; k < (a.sliced ? 0 : a.nbig); k += SMALL_BLK) {
         const int c0 = a.bigv[k], c1 = a.bigv[k + 1];
         const int v = a.chunks[c0].v;
         float s = a.part[c0];
@@ -221,6 +236,96 @@ __global__ __launch_bounds__(SMALL_BLK) void pr_k3(PrArgs a) {
         if (a.mode == MG_PR_ALL && a.tol_n > 0 && l1 < a.tol_n) done = true;
         c->done = done;
     }
+}
+
+// ---- XCD-sliced gather.  Partial sums are non-negative (contributions are ranks
+// over positive out-degrees), so the sign bit carries "an in-neighbour changed".
+__device__ __forceinline__ float with_act(float s, bool act) {
+    return __int_as_float(__float_as_int(s) | (act ? (int)0x80000000 : 0));
+}
+
+// Short (vertex, slice) ranges, one lane each in vertex order: workgroup b covers
+// slice b % 8, so contribution and frontier-bit gathers stay in one slice (L2).
+__global__ __launch_bounds__(BLK) void pr_s1(PrArgs a) {
+    Ctl *c = a.ctl;
+    if (c->done) return;
+    const PrIter I = pr_iter(a);
+    uint32_t *clr = a.chg[(I.it + 1) % 3];  // the bitmap filled at iteration it+1
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.nwords; i += gridDim.x * blockDim.x) clr[i] = 0;
+    const int x = blockIdx.x % PULL_SLICES, nbx = gridDim.x / PULL_SLICES;
+    const int *c0 = x == 0 ? nullptr : a.slice_off + (int64_t)(x - 1) * a.n;
+    const int *c1 = x == PULL_SLICES - 1 ? nullptr : a.slice_off + (int64_t)x * a.n;
+    float *sp = a.sp + (int64_t)x * a.n;
+    for (int base = (blockIdx.x / PULL_SLICES) * BLK; base < a.n; base += nbx * BLK) {
+        const int v = base + (int)threadIdx.x;
+        if (v >= a.n) continue;
+        const int beg = c0 ? c0[v] : a.in_off[v], end = c1 ? c1[v] : a.in_off[v + 1];
+        if (end - beg > LANE_T) continue;  // pieces: pr_s2
+        float s = 0.0f;
+        bool act = false;
+        for (int e = beg; e < end; ++e) {
+            const int u = a.in_adj[e];
+            s = s + I.c_old[u];
+            act = act || test_bit(I.prev, u);
+        }
+        sp[v] = with_act(s, act);
+    }
+}
+
+// Pieces of long ranges, one wave each, on the workgroups of their slice.
+__global__ __launch_bounds__(BLK) void pr_s2(PrArgs a) {
+    Ctl *c = a.ctl;
+    if (c->done) return;
+    const PrIter I = pr_iter(a);
+    const int lane = lane_id();
+    const int x = blockIdx.x % PULL_SLICES;
+    const int wave = (blockIdx.x / PULL_SLICES) * (BLK / WAVE) + (int)threadIdx.x / WAVE;
+    const int nwaves = (gridDim.x / PULL_SLICES) * (BLK / WAVE);
+    for (int t = a.sto[x] + wave; t < a.sto[x + 1]; t += nwaves) {
+        const ChunkRec r = a.stasks[t];
+        bool act = false;
+        const float s = wave_gather(a, I, r.beg, r.end, act);
+        if (lane == 0) a.spart[t] = with_act(s, act);
+    }
+}
+
+// Long ranges: pieces added in order.
+__global__ __launch_bounds__(BLK) void pr_s3(PrArgs a) {
+    Ctl *c = a.ctl;
+    if (c->done) return;
+    for (int k = blockIdx.x * BLK + (int)threadIdx.x; k < a.nsegs; k += gridDim.x * BLK) {
+        const ChunkRec sg = a.segs[k];
+        float s = 0.0f;
+        bool act = false;
+        for (int t = sg.beg; t < sg.beg + sg.end; ++t) {
+            const float p = a.spart[t];
+            s = s + fabsf(p);
+            act = act || signbit(p);
+        }
+        a.sp[(int64_t)sg.pad * a.n + sg.v] = with_act(s, act);
+    }
+}
+
+// Every vertex: slices added in order, then the apply.
+__global__ __launch_bounds__(BLK) void pr_s4(PrArgs a) {
+    Ctl *c = a.ctl;
+    if (c->done) return;
+    const PrIter I = pr_iter(a);
+    const bool all = a.mode == MG_PR_ALL || I.it == 0;
+    double delta = 0;
+    int nchanged = 0;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        bool act = all;
+#pragma unroll
+        for (int x = 0; x < PULL_SLICES; ++x) {
+            const float p = a.sp[(int64_t)x * a.n + v];
+            s = s + fabsf(p);
+            act = act || signbit(p);
+        }
+        delta += pr_apply(a, I, v, s, act, nchanged);
+    }
+    pr_write_part<BLK>(a, blockIdx.x, delta, nchanged);
 }
 
 // Builds (once per graph and in-edge view) the medium-vertex list and the chunk
@@ -327,12 +432,36 @@ extern "C" int mg_pagerank(mg_graph *g, float *rank_out, const mg_options *opt, 
     a.eps = o.pr_active_eps;
     a.tol_n = (double)o.pr_tol * (double)g->n;
 
+    int GS = 0;  // sliced gather grid (a multiple of PULL_SLICES)
+    if (!pull_slicing_disabled()) {
+        if ((rc = ensure_pull_slices(g, in_off, in_adj))) return rc;
+        if (ws.slice_ok) {
+            a.sliced = 1;
+            a.slice_off = ws.slice_off;
+            a.sp = reinterpret_cast<float *>(ws.slice_min);
+            a.stasks = ws.slice_tasks;
+            a.segs = ws.slice_segs;
+            a.spart = ws.slice_part;
+            a.nsegs = ws.nslice_segs;
+            for (int x = 0; x <= PULL_SLICES; ++x) a.sto[x] = ws.slice_task_off[x];
+            GS = std::max(PULL_SLICES, grid_blocks(std::max<int64_t>(g->n, g->m / 16), BLK) / PULL_SLICES * PULL_SLICES);
+            a.G2 = 0;  // pr_s4's partials use slots [0, G1)
+        }
+    }
+
     MG_HIP(hipEventRecord(g->ev0, g->stream));
     hipLaunchKernelGGL(pr_init, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, g->stream, a, o.iter_num);
     MG_HIP(hipGetLastError());
     auto step = [&]() -> int {
-        hipLaunchKernelGGL(pr_k1, dim3(a.G1), dim3(BLK), 0, g->stream, a);
-        hipLaunchKernelGGL(pr_k2, dim3(a.G2), dim3(BLK), 0, g->stream, a);
+        if (a.sliced) {
+            hipLaunchKernelGGL(pr_s1, dim3(GS), dim3(BLK), 0, g->stream, a);
+            hipLaunchKernelGGL(pr_s2, dim3(GS), dim3(BLK), 0, g->stream, a);
+            if (a.nsegs) hipLaunchKernelGGL(pr_s3, dim3(grid_blocks(a.nsegs, BLK)), dim3(BLK), 0, g->stream, a);
+            hipLaunchKernelGGL(pr_s4, dim3(a.G1), dim3(BLK), 0, g->stream, a);
+        } else {
+            hipLaunchKernelGGL(pr_k1, dim3(a.G1), dim3(BLK), 0, g->stream, a);
+            hipLaunchKernelGGL(pr_k2, dim3(a.G2), dim3(BLK), 0, g->stream, a);
+        }
         hipLaunchKernelGGL(pr_k3, dim3(1), dim3(SMALL_BLK), 0, g->stream, a);
         MG_HIP(hipGetLastError());
         return MG_OK;
@@ -340,7 +469,8 @@ extern "C" int mg_pagerank(mg_graph *g, float *rank_out, const mg_options *opt, 
     // the captured steps bake in the per-call constants: key on all of them
     uint64_t key = (uint64_t)a.mode;
     for (uint64_t x : {(uint64_t)__builtin_bit_cast(uint32_t, a.base), (uint64_t)__builtin_bit_cast(uint32_t, a.eps),
-                       (uint64_t)__builtin_bit_cast(uint64_t, a.tol_n), (uint64_t)(uintptr_t)in_off})
+                       (uint64_t)__builtin_bit_cast(uint64_t, a.tol_n), (uint64_t)(uintptr_t)in_off,
+                       (uint64_t)a.sliced})
         key = mix64(key ^ x);
     if ((rc = drive(g, step, (int64_t)std::max(o.iter_num, 0) + 1, KIND_PR, key))) return rc;
     rc = finish_search(g, stats, nullptr, 0, rank_out, ws.rank, 4 * (size_t)g->n, o.out_on_device, timer);
