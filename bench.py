@@ -70,6 +70,47 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+def grid_part(mg, seed: int, log) -> dict:
+    """BFS and SSSP from vertex 0 (a corner) of the 2048x2048 4-neighbour grid with 5% of the edges deleted
+    (seeded) and weights 1..16: about 4,100 levels, the high-diameter case."""
+    host = mg.grid_graph(2048, seed=seed, weight_max=16)
+    out = {"n": host.n, "m": host.m}
+    with mg.Graph(host, symmetric=True) as dg:
+        for kind, fn, nb in [("bfs", dg.bfs, bytes_bfs), ("sssp", dg.sssp, bytes_sssp)]:
+            fn(0, copy=False)
+            ms = []
+            for _ in range(3):
+                _, st = fn(0, stats=True, copy=False)
+                ms.append(st.device_ms)
+            t = float(np.median(ms))
+            b = nb(st.edges_traversed, st.vertices_reached)
+            out[kind] = {"device_ms": t, "levels": st.iterations, "gteps": st.edges_traversed / t / 1e6,
+                         "edges_examined": st.edges_examined, "us_per_level": 1e3 * t / max(st.iterations, 1),
+                         "hbm_frac": b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            log(f"grid2048 {kind}: {t:.2f} ms, {st.iterations} levels")
+    return out
+
+
+def pagerank_part(mg, seed: int, log) -> dict:
+    """PageRank on the directed scale-22 R-MAT (dangling vertices kept), all-vertex Jacobi, damping 0.85,
+    FP32, 50 iterations (the L1 stop at 1e-6 never triggers in FP32 at this size)."""
+    host = mg.rmat_graph(22, 16, seed=seed, symmetric=False)
+    with mg.Graph(host, build_csc=True) as dg:
+        o = mg.options(pr_mode=mg.MG_PR_ALL, iter_num=50, pr_tol=1e-6)
+        dg.pagerank(o, copy=False)
+        ms = []
+        for _ in range(3):
+            _, st = dg.pagerank(o, stats=True, copy=False)
+            ms.append(st.device_ms)
+    t = float(np.median(ms))
+    per_it = t / st.iterations
+    b = 8 * host.m + 16 * host.n
+    log(f"pagerank s22d: {t:.2f} ms for {st.iterations} iterations")
+    return {"n": host.n, "m": host.m, "iterations": st.iterations, "device_ms": t, "ms_per_iteration": per_it,
+            "gteps": host.m * st.iterations / t / 1e6, "algorithmic_bytes_per_iteration": b,
+            "hbm_frac": b / (per_it * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
 def batched_part(mg, seed: int, log) -> dict:
     """256 BFS and 256 SSSP sources (seeded draws among non-isolated vertices) on the R-MAT scale-20 graph
     and the 2048x2048 grid: one batched call each against the same sources run one by one (plain)."""
@@ -170,9 +211,11 @@ def main() -> int:
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--parts", default="batch,serve",
-                    help="extra measurements beside the main step: batch (256 sources, batched vs plain) and "
-                         "serve (Poisson query stream over six resident graphs); '' for none")
+    ap.add_argument("--parts", default="grid,pagerank,batch,serve",
+                    help="extra measurements beside the main step: grid (BFS/SSSP from the corner of the "
+                         "2048x2048 grid), pagerank (50 iterations on the directed scale-22 R-MAT), batch (256 "
+                         "sources, batched vs plain) and serve (Poisson query stream over six resident graphs); "
+                         "'' for none")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="length of the serving arrival window")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="mean query arrival rate (queries/s)")
     ap.add_argument("--verbose", action="store_true")
@@ -286,6 +329,10 @@ def main() -> int:
                "cpu_model": cpu_model()}
 
     parts_wanted = [p for p in a.parts.split(",") if p]
+    if "grid" in parts_wanted:
+        parts["grid2048"] = grid_part(mg, a.seed, log)
+    if "pagerank" in parts_wanted:
+        parts["pagerank_s22d"] = pagerank_part(mg, a.seed, log)
     if "batch" in parts_wanted:
         parts["batched"] = batched_part(mg, a.seed, log)
     if "serve" in parts_wanted:

@@ -146,6 +146,14 @@ typedef struct mg_stats {
 int mg_bfs(mg_graph *g, int32_t src, int32_t *depth_out,
            const mg_options *opt, mg_stats *stats);
 
+/* BFS from src followed by predecessor extraction
+ * (Algorithms/BFS/predextract.cuh:246-375, called at bfs.cu:459-468):
+ *   pred[v] = max { u in in(v) : depth[v] - depth[u] == 1 }, or -1 if none,
+ * for every vertex (the source's value is not a parent, as in the reference).
+ * Needs MG_GRAPH_BUILD_CSC or MG_GRAPH_SYMMETRIC.  depth_out may be NULL. */
+int mg_bfs_pred(mg_graph *g, int32_t src, int32_t *depth_out, int32_t *pred_out,
+                const mg_options *opt, mg_stats *stats);
+
 /* SSSP distances from src over int32 edge weights (needs MG_GRAPH_WEIGHTED):
  * dist[src] = 0, unreached = INT32_MAX (sssp.h:42-43). */
 int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out,

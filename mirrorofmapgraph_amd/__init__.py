@@ -49,7 +49,7 @@ INT32_MAX = 2**31 - 1
 EXPORTED = [
     "mg_last_error", "mg_version", "mg_device_init", "mg_device_count", "mg_device_synchronize",
     "mg_graph_create", "mg_graph_destroy", "mg_graph_info", "mg_graph_device_bytes", "mg_graph_get_csc",
-    "mg_options_default", "mg_bfs", "mg_sssp", "mg_cc", "mg_pagerank", "mg_bfs_batch", "mg_sssp_batch", "mg_ppr_batch",
+    "mg_options_default", "mg_bfs", "mg_bfs_pred", "mg_sssp", "mg_cc", "mg_pagerank", "mg_bfs_batch", "mg_sssp_batch", "mg_ppr_batch",
     "mg_rmat_num_edges", "mg_gen_rmat_edges", "mg_grid_num_edges", "mg_gen_grid_edges",
     "mg_csr_from_edges", "mg_edge_weights_hash", "mg_csc_from_csr", "mg_pick_sources",
     "mg_mtx_read_header", "mg_mtx_read",
@@ -158,6 +158,7 @@ def lib() -> C.CDLL:
     L.mg_device_init.argtypes = [C.c_int]
     search = {
         "mg_bfs": [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
+        "mg_bfs_pred": [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
         "mg_sssp": [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
         "mg_cc": [C.c_void_p, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
         "mg_pagerank": [C.c_void_p, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
@@ -403,6 +404,16 @@ class Graph:
 
     def bfs(self, src: int, opt: Options | None = None, stats: bool = False, copy: bool = True):
         return self._run(lib().mg_bfs, src, opt=opt, out=self._out(copy), stats=stats)
+
+    def bfs_pred(self, src: int, opt: Options | None = None, stats: bool = False):
+        """(depths, predecessors) of a BFS from src (predextract.cuh semantics)."""
+        d = np.empty(self.n, np.int32)
+        p = np.empty(self.n, np.int32)
+        st = Stats()
+        o = opt if opt is not None else options()
+        check(lib().mg_bfs_pred(self.handle, src, d.ctypes.data_as(C.c_void_p), p.ctypes.data_as(C.c_void_p),
+                                C.byref(o), C.byref(st)))
+        return (d, p, st) if stats else (d, p)
 
     def sssp(self, src: int, opt: Options | None = None, stats: bool = False, copy: bool = True):
         return self._run(lib().mg_sssp, src, opt=opt, out=self._out(copy), stats=stats)

@@ -72,6 +72,11 @@ def build(jobs: int = 8, force: bool = True, verbose: bool = False) -> Path:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
           "-lpthread"])
     os.replace(tmp, LIB)
+    # the command-line driver, linked against the library next to it
+    drv = PKG / "driver" / "mapgraph_run.cpp"
+    if drv.exists():
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", str(PKG.parent / "include"), str(drv), "-L", str(PKG),
+              "-lmapgraph_hip", "-Wl,-rpath,$ORIGIN", "-o", str(PKG / "mapgraph_run")])
     return LIB
 
 

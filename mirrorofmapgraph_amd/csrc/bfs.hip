@@ -280,9 +280,84 @@ __global__ __launch_bounds__(BLK) void bfs_level_b(BfsArgs a) {
     if (blockIdx.x == 0 && threadIdx.x == 0) c->pending = 1;
 }
 
+// Predecessors (Algorithms/BFS/predextract.cuh:246-375, run by bfs.cu:459-468): the
+// reference gathers, over every vertex's in-edges, the neighbour id when
+// label[v] - label[u] == 1 and -1 otherwise, and reduces them with max.  Lists up to
+// LANE_T in-edges: one lane each; longer lists: TASK-edge pieces, one wave each,
+// folded in with atomicMax.
+__global__ __launch_bounds__(BLK) void bfs_pred_lanes(const int *in_off, const int *in_adj, const int *label,
+                                                     int *pred, int n) {
+    for (int v = blockIdx.x * BLK + (int)threadIdx.x; v < n; v += gridDim.x * BLK) {
+        const int beg = in_off[v], end = in_off[v + 1];
+        int best = -1;
+        if (end - beg <= LANE_T) {
+            const int lv = label[v];
+            for (int e = beg; e < end; ++e) {
+                const int u = in_adj[e];
+                if (lv - label[u] == 1) best = max(best, u);
+            }
+        }
+        pred[v] = best;
+    }
+}
+
+__global__ __launch_bounds__(BLK) void bfs_pred_tasks(const ChunkRec *tasks, int ntasks, const int *in_adj,
+                                                     const int *label, int *pred) {
+    const int lane = lane_id();
+    const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
+    for (int t = wave; t < ntasks; t += nwaves) {
+        const ChunkRec r = tasks[t];
+        const int lv = label[r.v];
+        int best = -1;
+        for (int e = r.beg + lane; e < r.end; e += WAVE) {
+            const int u = in_adj[e];
+            if (lv - label[u] == 1) best = max(best, u);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) best = max(best, __shfl_xor(best, o));
+        if (lane == 0 && best >= 0) atomicMax(&pred[r.v], best);
+    }
+}
+
 }  // namespace mg
 
 using namespace mg;
+
+extern "C" int mg_bfs_pred(mg_graph *g, int32_t src, int32_t *depth_out, int32_t *pred_out, const mg_options *opt,
+                           mg_stats *stats) {
+    CallTimer timer;
+    mg_options o;
+    int rc = begin_search(g, opt, &o);
+    if (rc) return rc;
+    const bool sym = g->flags & MG_GRAPH_SYMMETRIC;
+    if (!sym && !g->col_off)
+        return set_error(MG_ERR_UNSUPPORTED, "mg_bfs_pred: needs the in-edge view (MG_GRAPH_BUILD_CSC or MG_GRAPH_SYMMETRIC)");
+    mg_options inner = o;
+    inner.out_on_device = 1;
+    inner.engine = MG_ENGINE_AUTO;  // the depths must land in the workspace
+    if ((rc = mg_bfs(g, src, nullptr, &inner, stats))) return rc;  // depths stay in the workspace
+    std::lock_guard<std::mutex> lock(g->mu);
+    Workspace &ws = g->ws;
+    const int *in_off = sym ? g->row_off : g->col_off;
+    const int *in_adj = sym ? g->col : g->row_idx;
+    if ((rc = ensure_pull_tasks(g, in_off))) return rc;
+    int *pred = nullptr;
+    if ((rc = alloc_i32(&pred, (size_t)g->n))) return rc;
+    hipLaunchKernelGGL(bfs_pred_lanes, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, g->stream, in_off, in_adj,
+                       (const int *)ws.vals, pred, g->n);
+    if (ws.pull_ntasks)
+        hipLaunchKernelGGL(bfs_pred_tasks, dim3(grid_blocks((int64_t)ws.pull_ntasks * WAVE, BLK)), dim3(BLK), 0,
+                           g->stream, (const ChunkRec *)ws.pull_tasks, ws.pull_ntasks, in_adj, (const int *)ws.vals,
+                           pred);
+    rc = MG_OK;
+    if (hipGetLastError() != hipSuccess) rc = set_error(MG_ERR_DEVICE, "predecessor kernels failed to launch");
+    if (!rc) rc = deliver(g, depth_out, ws.vals, 4 * (size_t)g->n, o.out_on_device);
+    if (!rc) rc = deliver(g, pred_out, pred, 4 * (size_t)g->n, o.out_on_device);
+    (void)hipStreamSynchronize(g->stream);
+    free_dev(pred);
+    if (stats) stats->host_ms = timer.ms();
+    return rc;
+}
 
 extern "C" int mg_bfs(mg_graph *g, int32_t src, int32_t *depth_out, const mg_options *opt, mg_stats *stats) {
     CallTimer timer;

@@ -231,3 +231,39 @@ def test_pagerank_symmetric_graph_uses_csr(mg, oracle, device):
         got = dg.pagerank(mg.options(pr_mode=mg.MG_PR_ALL, iter_num=20))
     exp, _, _ = oracle.pr_jacobi_f32(g.row_offsets, g.col_idx, iters=20)
     assert _pr_err(got, exp, g.n) <= PR_TOL
+
+
+# ---------------------------------------------------------------- BFS predecessors
+def _pred_reference(g, depth, csc=None):
+    """predextract.cuh:131-139 + ReduceFunctor max: over in-edges (CSC), the largest
+    neighbour u with depth[v] - depth[u] == 1, else -1."""
+    co, ri, _ = csc if csc is not None else g.csc()
+    n = g.n
+    dst = np.repeat(np.arange(n), np.diff(co))
+    ok = (depth[dst].astype(np.int64) - depth[ri].astype(np.int64)) == 1
+    pred = np.full(n, -1, np.int64)
+    np.maximum.at(pred, dst[ok], ri[ok].astype(np.int64))
+    return pred.astype(np.int32)
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_bfs_predecessors(mg, oracle, device, symmetric):
+    g = mg.rmat_graph(14, 16, seed=11, symmetric=symmetric)
+    src = int(np.argmax(g.degrees()))
+    with mg.Graph(g, symmetric=symmetric, build_csc=not symmetric) as dg:
+        d, p = dg.bfs_pred(src)
+    assert np.array_equal(d, oracle.bfs(g.row_offsets, g.col_idx, src))
+    assert np.array_equal(p, _pred_reference(g, d))
+    # every reached vertex other than the source has a parent one level up
+    reached = (d > 0)
+    assert np.all(p[reached] >= 0) and np.all(d[p[reached]] == d[reached] - 1)
+    assert np.all(p[d < 0] == -1)
+
+
+def test_bfs_predecessors_fixtures(mg, oracle, device):
+    for name in ["small.mtx", "chesapeake.mtx"]:
+        g = load_mtx(mg, name)
+        with mg.Graph(g, build_csc=True) as dg:
+            for src in range(min(g.n, 4)):
+                d, p = dg.bfs_pred(src)
+                assert np.array_equal(p, _pred_reference(g, d)), (name, src)
