@@ -110,10 +110,25 @@ __global__ void bfs_seed(BfsArgs a, int src, int iter_limit, int can_bu, int alp
 }
 
 // Closes a pending multi-workgroup level, then runs small top-down levels.
+//
+// Narrow levels are latency-bound (a grid has thousands of them), so the loop keeps
+// the chain of dependent memory accesses per level short: the level state lives in
+// LDS (the control block is written only when the loop exits), the frontier queue
+// lives in LDS, each thread keeps its vertices of the next level and their edge
+// offsets in registers (loading them also gives the next frontier's out-edges for
+// the direction test), visits claim the visited bit with one atomic (no pre-test),
+// and a frontier's bitmap words are cleared as its vertices are expanded.  Vertices
+// with more than LANE_T edges go through the LDS-scan tile path.
+constexpr int SMALL_PER = SMALL_Q / SMALL_BLK;           // queue entries per thread
+constexpr int SMALL_BIG = SMALL_E / (LANE_T + 1) + 1;     // lists > LANE_T in a frontier of <= SMALL_E edges
+
 __global__ __launch_bounds__(SMALL_BLK) void bfs_small(BfsArgs a) {
     Ctl *c = a.fa.ctl;
     const FrontierArgs &fa = a.fa;
-    __shared__ int s_go, s_L, s_qv, s_fin, s_cnt;
+    __shared__ int s_q[SMALL_Q];
+    __shared__ int s_big[SMALL_BIG];
+    __shared__ int s_go, s_L, s_cnt, s_nbig, s_levels, s_dir, s_canbu, s_alpha, s_limit;
+    __shared__ long long s_mu;
     if (c->pending) {
         const int L = c->level;
         const bool bu = c->dir == DIR_BU;
@@ -136,48 +151,161 @@ __global__ __launch_bounds__(SMALL_BLK) void bfs_small(BfsArgs a) {
         }
         __syncthreads();
     }
-    for (;;) {
-        if (threadIdx.x == 0) {
-            const int L = c->level;
-            s_go = !c->done && c->dir == DIR_TD && c->qsize[L % 3] <= SMALL_Q && c->mf_cur <= SMALL_E;
-            s_L = L;
-            s_qv = c->qvalid;
-            s_fin = c->qsize[L % 3];
-            s_cnt = 0;
-        }
-        __syncthreads();
-        if (!s_go) break;
+    int n0 = 0, qv = 0;
+    if (threadIdx.x == 0) {
+        const int L = c->level;
+        s_go = !c->done && c->dir == DIR_TD && c->qsize[L % 3] <= SMALL_Q && c->mf_cur <= SMALL_E;
+        s_L = L;
+        s_mu = c->m_unexplored;
+        s_dir = c->dir;
+        s_cnt = c->qsize[L % 3];
+        s_levels = 0;
+        s_canbu = c->can_bu;
+        s_alpha = c->alpha;
+        s_limit = c->iter_limit;
+        s_q[0] = c->qvalid;  // scratch: queue-valid bits
+    }
+    __syncthreads();
+    if (!s_go) return;
+    {
         const int L = s_L, cur = L % 3, prv = (L + 2) % 3;
-        if (!(s_qv & (1 << cur))) {  // entering from multi-workgroup levels: list frontier L
-            queue_from_bitmap<SMALL_BLK>(fa, L, &s_cnt);
-            if (threadIdx.x == 0) {
-                s_fin = s_cnt;
-                s_cnt = 0;
-            }
+        qv = s_q[0];
+        n0 = s_cnt;
+        __syncthreads();
+        // this level's frontier into the LDS queue
+        if (qv & (1 << cur)) {
+            for (int i = threadIdx.x; i < n0; i += SMALL_BLK) s_q[i] = fa.q[cur][i];
+        } else {
+            if (threadIdx.x == 0) s_cnt = 0;
+            __syncthreads();
+            queue_from_bitmap_to<SMALL_BLK>(fa, L, s_q, &s_cnt, SMALL_Q);
+            n0 = min(s_cnt, SMALL_Q);  // equals qsize[cur] <= SMALL_Q
         }
-        if (s_qv & (1 << prv)) {
+        // the previous frontier's bitmap is cleared once here; later levels clear
+        // their own bitmap as they expand it
+        if (qv & (1 << prv)) {
             const int *qp = fa.q[prv];
             for (int i = threadIdx.x; i < c->qsize[prv]; i += SMALL_BLK) fa.bm[prv][qp[i] >> 5] = 0;
         } else {
             clear_prev_sweep(fa, L, threadIdx.x, SMALL_BLK);
         }
         __syncthreads();
-        BfsPolicy pol(a);
-        int *qout = fa.q[(L + 1) % 3];
-        auto emit = [&](int u, bool newly) { wave_append(newly, u, qout, &s_cnt); };
-        const int *qin = fa.q[cur];
-        for (int view = 0; view < fa.nviews; ++view)
-            for (int t0 = 0; t0 < s_fin; t0 += SMALL_BLK) {
-                const int i = t0 + (int)threadIdx.x;
-                expand_tile<SMALL_BLK, false>(fa, pol, view, L, i < s_fin ? qin[i] : -1, emit);
+    }
+    // registers: this thread's frontier vertices and, per view, their edge ranges
+    int my_v[SMALL_PER], my_beg[2][SMALL_PER], my_deg[2][SMALL_PER];
+    int nmine = 0;
+    auto load_mine = [&](int count) -> long long {
+        long long mf = 0;
+        nmine = 0;
+#pragma unroll
+        for (int k = 0; k < SMALL_PER; ++k) {
+            const int i = (int)threadIdx.x + k * SMALL_BLK;
+            my_v[k] = i < count ? s_q[i] : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < SMALL_PER; ++k) {
+            for (int view = 0; view < 2; ++view) {
+                my_beg[view][k] = 0;
+                my_deg[view][k] = 0;
+                if (my_v[k] >= 0 && view < fa.nviews) {
+                    const int *off = fa.off[view];
+                    my_beg[view][k] = off[my_v[k]];
+                    my_deg[view][k] = off[my_v[k] + 1] - my_beg[view][k];
+                    mf += my_deg[view][k];
+                }
             }
-        const long long mf = block_sum<SMALL_BLK>(pol.mf);
+            nmine += my_v[k] >= 0;
+        }
+        return mf;
+    };
+    load_mine(n0);
+    __syncthreads();  // every entry is in registers: the queue is free for the next level
+    BfsPolicy pol(a);
+    long long mf_over = 0;  // out-edges of claimed vertices that did not fit the queue
+    for (;;) {
+        const int L = s_L;
+        uint32_t *nx = fa.bm[(L + 1) % 3];
+        uint32_t *cb = fa.bm[L % 3];
         if (threadIdx.x == 0) {
-            c->small_levels++;
-            bfs_close(a, L, s_cnt, mf);
-            c->qvalid = (1 << cur) | (1 << ((L + 1) % 3));
+            s_cnt = 0;
+            s_nbig = 0;
         }
         __syncthreads();
+        // lane path: a thread walks its vertices' lists, four claims in flight
+        for (int k = 0; k < SMALL_PER; ++k) {
+            const int v = my_v[k];
+            if (v < 0) continue;
+            cb[v >> 5] = 0;
+            if (my_deg[0][k] > LANE_T || my_deg[1][k] > LANE_T) {
+                const int bi = atomicAdd(&s_nbig, 1);  // < SMALL_BIG: the frontier has <= SMALL_E edges
+                if (bi < SMALL_BIG) s_big[bi] = v;
+                else atomicOr(&c->changed, 1 << 30);
+                continue;
+            }
+            for (int view = 0; view < fa.nviews; ++view) {
+                const int *adj = fa.adj[view];
+                const int b = my_beg[view][k], e = b + my_deg[view][k];
+                for (int p = b; p < e; p += 4) {
+                    int u[4];
+                    uint32_t old[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) u[j] = p + j < e ? adj[p + j] : -1;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) old[j] = u[j] >= 0 ? atomicOr(&a.visited[u[j] >> 5], 1u << (u[j] & 31)) : ~0u;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const bool newly = u[j] >= 0 && !((old[j] >> (u[j] & 31)) & 1u);
+                        if (newly) {
+                            a.depth[u[j]] = L + 1;
+                            atomicOr(&nx[u[j] >> 5], 1u << (u[j] & 31));
+                            const int idx = atomicAdd(&s_cnt, 1);
+                            if (idx < SMALL_Q) s_q[idx] = u[j];
+                            else mf_over += view_degree(fa, u[j]);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // tile path for the few long lists
+        const int nbig = min(s_nbig, SMALL_BIG);
+        for (int t0 = 0; t0 < nbig; t0 += SMALL_BLK) {
+            const int i = t0 + (int)threadIdx.x;
+            auto emit = [&](int u, bool newly) {
+                if (!newly) return;
+                const int idx = atomicAdd(&s_cnt, 1);
+                if (idx < SMALL_Q) s_q[idx] = u;
+                else mf_over += view_degree(fa, u);
+            };
+            for (int view = 0; view < fa.nviews; ++view)
+                expand_tile<SMALL_BLK, false>(fa, pol, view, L, i < nbig ? s_big[i] : -1, emit);
+        }
+        __syncthreads();
+        const int nf = s_cnt;
+        long long mf = load_mine(min(nf, SMALL_Q)) + mf_over;
+        mf_over = 0;
+        mf = block_sum<SMALL_BLK>(mf);
+        if (threadIdx.x == 0) {
+            // close level L on the LDS copy of the direction state (as bfs_close)
+            long long mu = s_mu - mf;
+            int dir = s_dir;
+            if (s_canbu && mf > mu / s_alpha) dir = DIR_BU;
+            s_mu = mu;
+            s_dir = dir;
+            s_levels++;
+            const bool go = nf > 0 && nf <= SMALL_Q && dir == DIR_TD && mf <= SMALL_E && L + 1 < s_limit;
+            s_go = go;
+            s_L = L + 1;
+            if (!go) {
+                c->m_unexplored = mu;
+                c->dir = dir;
+                c->small_levels += s_levels;
+                advance_level(c, L, nf, mf);
+                c->qvalid = 0;  // the queue stayed in LDS
+            }
+        }
+        __syncthreads();
+        if (!s_go) break;
     }
 }
 

@@ -184,8 +184,9 @@ template <class Step>
 int drive(mg_graph *g, Step step, int64_t max_steps, int kind = -1, uint64_t key = 0, Ctl *ctl = nullptr,
           Ctl *h_ctl = nullptr);
 bool graphs_enabled();
-// MG_NO_SLICED_PULL=1 keeps pull levels unsliced (for comparison runs).
-bool pull_slicing_disabled();
+// MG_SLICED_PULL=1 turns on the XCD-sliced pull of SSSP and PageRank (off by
+// default: measured slower than the unsliced pull at scale 22).
+bool pull_slicing_enabled();
 
 // Generic GAS enactor over the ported programs (engine = MG_ENGINE_GAS).
 struct GasStateBase;

@@ -123,12 +123,12 @@ bool graphs_enabled() {
     return on;
 }
 
-bool pull_slicing_disabled() {
-    static const bool off = [] {
-        const char *e = getenv("MG_NO_SLICED_PULL");
+bool pull_slicing_enabled() {
+    static const bool on = [] {
+        const char *e = getenv("MG_SLICED_PULL");
         return e && *e && *e != '0';
     }();
-    return off;
+    return on;
 }
 
 int begin_search(mg_graph *g, const mg_options *opt, mg_options *resolved) {

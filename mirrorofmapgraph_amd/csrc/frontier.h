@@ -360,11 +360,10 @@ __device__ void expand_chunks(const FrontierArgs &fa, P &pol, int L) {
 // takes 16 consecutive words (four 16-byte loads in flight) per round, so a 4M-vertex
 // bitmap is 8 rounds of one workgroup rather than 128 load-then-barrier rounds.
 template <int BLOCK>
-__device__ void queue_from_bitmap(const FrontierArgs &fa, int L, int *s_count) {
+__device__ void queue_from_bitmap_to(const FrontierArgs &fa, int L, int *q, int *s_count, int cap = INT_MAX) {
     constexpr int WPT = 16;
     __shared__ int s_wave3[BLOCK / WAVE];
     const uint32_t *bm = fa.bm[L % 3];
-    int *q = fa.q[L % 3];
     for (int w0 = 0; w0 < fa.nwords; w0 += BLOCK * WPT) {
         const int wb = w0 + (int)threadIdx.x * WPT;
         uint32_t words[WPT];
@@ -392,7 +391,8 @@ __device__ void queue_from_bitmap(const FrontierArgs &fa, int L, int *s_count) {
             uint32_t word = words[j];
             while (word) {
                 const int b = __ffs(word) - 1;
-                q[k++] = (wb + j) * 32 + b;
+                if (k < cap) q[k] = (wb + j) * 32 + b;
+                ++k;
                 word &= word - 1;
             }
         }
@@ -400,6 +400,12 @@ __device__ void queue_from_bitmap(const FrontierArgs &fa, int L, int *s_count) {
         if (threadIdx.x == 0) *s_count += total;
         __syncthreads();
     }
+}
+
+// As queue_from_bitmap_to, into the global queue q[L % 3].
+template <int BLOCK>
+__device__ void queue_from_bitmap(const FrontierArgs &fa, int L, int *s_count) {
+    queue_from_bitmap_to<BLOCK>(fa, L, fa.q[L % 3], s_count);
 }
 
 // Fills the search-independent fields of the control block.

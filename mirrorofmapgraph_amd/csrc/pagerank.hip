@@ -433,7 +433,7 @@ extern "C" int mg_pagerank(mg_graph *g, float *rank_out, const mg_options *opt, 
     a.tol_n = (double)o.pr_tol * (double)g->n;
 
     int GS = 0;  // sliced gather grid (a multiple of PULL_SLICES)
-    if (!pull_slicing_disabled()) {
+    if (pull_slicing_enabled()) {
         if ((rc = ensure_pull_slices(g, in_off, in_adj))) return rc;
         if (ws.slice_ok) {
             a.sliced = 1;

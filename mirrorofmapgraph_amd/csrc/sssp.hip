@@ -399,7 +399,7 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
         if ((rc = ensure_pull_tasks(g, a.fa.off[2]))) return rc;
         a.pull = ws.pull_tasks;
         a.npull = ws.pull_ntasks;
-        if (!pull_slicing_disabled()) {
+        if (pull_slicing_enabled()) {
             if ((rc = ensure_pull_slices(g, a.fa.off[2], a.fa.adj[2]))) return rc;
             if (ws.slice_ok) {
                 a.sliced = 1;
