@@ -105,7 +105,12 @@ struct Workspace {
     int32_t *pr_pact = nullptr;                       // per-chunk "an in-neighbour changed" flags
     int pr_nmid = 0, pr_nchunks = 0, pr_nbig = 0;
     const int32_t *pr_view = nullptr;                 // in-edge offsets the buckets were built for
-    // Static pull tasks: TASK-edge pieces of the in-edge lists longer than LANE_T
+    // edge-centric PageRank: in-lists crossing PULL_CHUNK boundaries
+    int32_t *pr_span = nullptr;       // per crossing list: vertex, first chunk, last chunk (3 ints)
+    int pr_nspan_short = 0, pr_nspan = 0;  // short spans (<= PR_SPAN_SHORT chunks) first, then long
+    float *pr_head = nullptr, *pr_tail = nullptr;  // per chunk: partial of its first / last list
+    const int32_t *pr_span_view = nullptr;
+    // Static pull tasks: PULL_TASK-edge pieces of the in-edge lists longer than LANE_T
     ChunkRec *pull_tasks = nullptr;
     int pull_ntasks = 0;
     const int32_t *pull_view = nullptr;
@@ -122,6 +127,9 @@ struct Workspace {
     int slice_len = 0;
     const int32_t *slice_view = nullptr;  // in-edge offsets the slices were built for (nullptr: none)
     bool slice_ok = false;                // lists sorted, so slicing applies
+    // Edge-centric pull: the vertex owning each in-edge slot of the in-edge view
+    int32_t *edge_dst = nullptr;
+    const int32_t *edge_dst_view = nullptr;
 };
 
 }  // namespace mg
@@ -172,7 +180,9 @@ int deliver(mg_graph *g, void *dst, const void *src, size_t bytes, bool out_on_d
 // arguments are fixed per (graph, kind, key), so the executable graph is reused by
 // every later search with the same key.
 constexpr int STEPS_PER_GRAPH = 8;
-constexpr int PULL_SLICES = 8;  // one source slice per XCD (workgroup b runs on XCD b % 8)
+constexpr int PULL_SLICES = 8;
+constexpr int PULL_CHUNK = 256;
+constexpr int PULL_TASK = 512;   // static pull tasks: in-edges per wave (8 per lane in flight)  // edge-centric pull: in-edge slots per wave task (4 per lane)  // one source slice per XCD (workgroup b runs on XCD b % 8)
 enum { KIND_BFS = 0, KIND_SSSP = 1, KIND_PR = 2, KIND_GAS = 3, KIND_BATCH = 4, KIND_BATCH_SSSP = 5, NKINDS = 8 };
 
 struct StepGraph {
@@ -187,6 +197,9 @@ bool graphs_enabled();
 // MG_SLICED_PULL=1 turns on the XCD-sliced pull of SSSP and PageRank (off by
 // default: measured slower than the unsliced pull at scale 22).
 bool pull_slicing_enabled();
+// MG_EDGE_PULL=1 selects edge-centric pull levels (off by default: measured slower
+// than the vertex-centric pull at scale 22).
+bool edge_pull_enabled();
 
 // Generic GAS enactor over the ported programs (engine = MG_ENGINE_GAS).
 struct GasStateBase;

@@ -61,10 +61,14 @@ static void free_workspace(Workspace &ws) {
     dfree(ws.pr_part);
     dfree(ws.pr_pact);
     dfree(ws.pr_bigv);
+    dfree(ws.pr_span);
+    dfree(ws.pr_head);
+    dfree(ws.pr_tail);
     dfree(ws.pull_tasks);
     dfree(ws.slice_off);
     dfree(ws.slice_min);
     dfree(ws.slice_tasks);
+    dfree(ws.edge_dst);
     dfree(ws.slice_segs);
     dfree(ws.slice_part);
     dfree(ws.d_slice_task_off);
@@ -119,6 +123,14 @@ bool graphs_enabled() {
     static const bool on = [] {
         const char *e = getenv("MG_NO_GRAPH");
         return !(e && *e && *e != '0');
+    }();
+    return on;
+}
+
+bool edge_pull_enabled() {
+    static const bool on = [] {
+        const char *e = getenv("MG_EDGE_PULL");
+        return e && *e && *e != '0';
     }();
     return on;
 }

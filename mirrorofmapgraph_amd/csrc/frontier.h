@@ -463,6 +463,8 @@ int launch_fill(mg_graph *g, int *p, int v, int64_t n);
 int launch_fill(mg_graph *g, uint32_t *p, uint32_t v, int64_t n);
 int launch_reached_stats(mg_graph *g, const int *vals, int unreached, mg_stats *st);
 int ensure_pull_tasks(mg_graph *g, const int *d_in_off);
+// Per-slot owner array of the in-edge view (d_in_off) for edge-centric pulls.
+int ensure_edge_dst(mg_graph *g, const int *d_in_off);
 // Builds the XCD-sliced pull structures for the in-edge view (d_in_off, d_in_adj);
 // ws.slice_ok reports whether every in-list is sorted (slicing needs it).
 int ensure_pull_slices(mg_graph *g, const int *d_in_off, const int *d_in_adj);

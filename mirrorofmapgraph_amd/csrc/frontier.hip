@@ -74,6 +74,35 @@ FrontierArgs make_frontier_args(mg_graph *g, int directed) {
 
 // Builds (once per graph and in-edge view) the TASK-edge pieces of every in-edge list
 // longer than LANE_T, for pull steps; the host walks the offsets once.
+// Owner vertex of every in-edge slot (the COO row of the in-edge view), built on
+// the host from the offsets, for pulls that walk edges rather than vertices.
+int ensure_edge_dst(mg_graph *g, const int *d_in_off) {
+    Workspace &ws = g->ws;
+    if (ws.edge_dst_view == d_in_off) return MG_OK;
+    const int n = g->n;
+    const int64_t m = g->m;
+    std::vector<int32_t> off((size_t)n + 1);
+    MG_HIP(hipMemcpy(off.data(), d_in_off, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
+    std::vector<int32_t> dst((size_t)std::max<int64_t>(m, 1));
+    const int nt = std::max(1, std::min(host_threads(), 64));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            const int lo = (int)((int64_t)n * t / nt), hi = (int)((int64_t)n * (t + 1) / nt);
+            for (int v = lo; v < hi; ++v)
+                for (int e = off[v]; e < off[v + 1]; ++e) dst[e] = v;
+        });
+    for (auto &x : th) x.join();
+    free_dev(ws.edge_dst);
+    ws.edge_dst = nullptr;
+    ws.edge_dst_view = nullptr;
+    int rc = alloc_i32(&ws.edge_dst, dst.size());
+    if (rc) return rc;
+    if (m) MG_HIP(hipMemcpy(ws.edge_dst, dst.data(), 4 * (size_t)m, hipMemcpyHostToDevice));
+    ws.edge_dst_view = d_in_off;
+    return MG_OK;
+}
+
 int ensure_pull_tasks(mg_graph *g, const int *d_in_off) {
     Workspace &ws = g->ws;
     if (ws.pull_view == d_in_off) return MG_OK;
@@ -83,7 +112,8 @@ int ensure_pull_tasks(mg_graph *g, const int *d_in_off) {
     std::vector<ChunkRec> t;
     for (int v = 0; v < n; ++v) {
         if (off[v + 1] - off[v] <= LANE_T) continue;
-        for (int b = off[v]; b < off[v + 1]; b += TASK) t.push_back(ChunkRec{v, b, std::min(b + TASK, off[v + 1]), 2});
+        for (int b = off[v]; b < off[v + 1]; b += PULL_TASK)
+            t.push_back(ChunkRec{v, b, std::min(b + PULL_TASK, off[v + 1]), 2});
     }
     free_dev(ws.pull_tasks);
     ws.pull_tasks = nullptr;

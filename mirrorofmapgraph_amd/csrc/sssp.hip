@@ -38,6 +38,10 @@ struct SsspArgs {
     int *slice_min;              // PULL_SLICES x n partial minima
     const ChunkRec *slice_tasks; // long ranges, grouped by slice
     int sto[PULL_SLICES + 1];    // slice task offsets
+    // edge-centric pull (edge_pull = 1): waves walk in-edge slots, whatever vertex owns them
+    int edge_pull;
+    const int *edge_dst;         // owner of each in-edge slot
+    long long m_in;              // in-edge slots
 };
 
 __device__ __forceinline__ long long edge_w(const SsspArgs &a, int view, int slot) {
@@ -88,23 +92,25 @@ struct SsspPush {
 };
 
 // Pull: the best dist[u] + w over in-edge slots [beg, end) from frontier vertices u.
-// Loads go out four at a time: neighbour ids, then their frontier bits, then the
-// distances and weights of the frontier ones.
+// Loads go out PULL_U at a time: neighbour ids, then their frontier bits, then the
+// distances and weights of the frontier ones (three dependent round trips per batch).
+constexpr int PULL_U = 8;
 __device__ __forceinline__ long long pull_best(const SsspArgs &a, const uint32_t *fr, int beg, int end, int step) {
     long long best = INT_MAX;
     const int *adj = a.fa.adj[2];
-    for (int e = beg; e < end; e += 4 * step) {
-        int u[4];
-        load4(adj, e, end, step, u);
-        bool f[4];
+    for (int e = beg; e < end; e += PULL_U * step) {
+        int u[PULL_U];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) f[k] = u[k] >= 0 && test_bit(fr, u[k]);
-        long long cand[4];
+        for (int k = 0; k < PULL_U; ++k) u[k] = e + k * step < end ? adj[e + k * step] : -1;
+        bool f[PULL_U];
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < PULL_U; ++k) f[k] = u[k] >= 0 && test_bit(fr, u[k]);
+        long long cand[PULL_U];
+#pragma unroll
+        for (int k = 0; k < PULL_U; ++k)
             cand[k] = f[k] ? (long long)a.dist[u[k]] + edge_w(a, 2, e + k * step) : (long long)INT_MAX;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) best = cand[k] < best ? cand[k] : best;
+        for (int k = 0; k < PULL_U; ++k) best = cand[k] < best ? cand[k] : best;
     }
     return best;
 }
@@ -224,6 +230,72 @@ __global__ __launch_bounds__(BLK) void sssp_level_a(SsspArgs a, int view) {
         write_part<BLK>(fa, view * a.G + blockIdx.x, pol.nf, pol.mf, pol.ex);
         return;
     }
+    if (a.edge_pull) {
+        // Edge-centric pull: one wave per PULL_CHUNK consecutive in-edge slots, four per
+        // lane (all loads in flight), whatever vertices own them.  Candidates dist[u] + w
+        // from frontier in-neighbours u are staged in LDS; the first slot of each run of
+        // one owner v takes the run's minimum and, if it beats dist[v], lowers it with
+        // one atomicMin (min is order-free, so the result is exact) and marks v.
+        __shared__ int s_d[BLK / WAVE][PULL_CHUNK];
+        __shared__ int s_c[BLK / WAVE][PULL_CHUNK];
+        const uint32_t *fr = fa.bm[L % 3];
+        uint32_t *nx = fa.bm[(L + 1) % 3];
+        const int *adj = fa.adj[2];
+        const int lane = lane_id(), wv = (int)threadIdx.x / WAVE;
+        const long long nch = (a.m_in + PULL_CHUNK - 1) / PULL_CHUNK;
+        const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
+        long long nf = 0, mf = 0, ex = 0;
+        for (long long ch = wave; ch < nch; ch += nwaves) {
+            const long long e0 = ch * PULL_CHUNK + 4 * lane;
+            int u[4], d[4], c[4];
+            if (e0 + 3 < a.m_in) {
+                const int4 u4 = *reinterpret_cast<const int4 *>(adj + e0);
+                const int4 d4 = *reinterpret_cast<const int4 *>(a.edge_dst + e0);
+                u[0] = u4.x; u[1] = u4.y; u[2] = u4.z; u[3] = u4.w;
+                d[0] = d4.x; d[1] = d4.y; d[2] = d4.z; d[3] = d4.w;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    u[j] = e0 + j < a.m_in ? adj[e0 + j] : -1;
+                    d[j] = e0 + j < a.m_in ? a.edge_dst[e0 + j] : -1;
+                }
+            }
+            bool f[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) f[j] = u[j] >= 0 && test_bit(fr, u[j]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                long long cand = INT_MAX;
+                if (f[j]) cand = (long long)a.dist[u[j]] + edge_w(a, 2, (int)(e0 + j));
+                c[j] = cand < INT_MAX ? (int)cand : INT_MAX;
+                s_d[wv][4 * lane + j] = d[j];
+                s_c[wv][4 * lane + j] = c[j];
+            }
+            ex += (e0 < a.m_in) ? min(4LL, a.m_in - e0) : 0;
+            // the wave's LDS writes before any lane reads its neighbours' slots
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int i = 4 * lane + j;
+                const int v = d[j];
+                if (v < 0 || (i > 0 && s_d[wv][i - 1] == v)) continue;  // not the first slot of its run
+                int best = INT_MAX;
+                for (int k = i; k < PULL_CHUNK && s_d[wv][k] == v; ++k) best = min(best, s_c[wv][k]);
+                if (best < a.dist[v]) {
+                    const int old = atomicMin(&a.dist[v], best);
+                    if (best < old && set_bit_first(nx, v)) {
+                        ++nf;
+                        mf += view_degree(fa, v);
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+        write_part<BLK>(fa, blockIdx.x, nf, mf, ex);
+        return;
+    }
     if (a.sliced) {
         // XCD-sliced pull: this workgroup scans, for every vertex, the part of its
         // in-list whose sources lie in slice x = blockIdx % 8, so the dist[] and
@@ -294,6 +366,8 @@ __global__ __launch_bounds__(BLK) void sssp_level_b(SsspArgs a) {
         nf = pol.nf;
         mf = pol.mf;
         ex = pol.ex;
+    } else if (a.edge_pull) {
+        // the edge-centric pull did the whole level in sssp_level_a
     } else if (a.sliced) {
         // long (vertex, slice) ranges of this workgroup's slice: wave minimum, one
         // atomicMin into the slice's partial
@@ -399,7 +473,12 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
         if ((rc = ensure_pull_tasks(g, a.fa.off[2]))) return rc;
         a.pull = ws.pull_tasks;
         a.npull = ws.pull_ntasks;
-        if (pull_slicing_enabled()) {
+        if (edge_pull_enabled()) {
+            if ((rc = ensure_edge_dst(g, a.fa.off[2]))) return rc;
+            a.edge_pull = 1;
+            a.edge_dst = ws.edge_dst;
+            a.m_in = g->m;
+        } else if (pull_slicing_enabled()) {
             if ((rc = ensure_pull_slices(g, a.fa.off[2], a.fa.adj[2]))) return rc;
             if (ws.slice_ok) {
                 a.sliced = 1;
@@ -428,6 +507,7 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
     };
     // Bellman-Ford with non-negative weights settles within n rounds.
     const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n + 1) + 3;
-    if ((rc = drive(g, step, max_steps, KIND_SSSP, (uint64_t)o.directed * 2 + (uint64_t)a.sliced))) return rc;
+    if ((rc = drive(g, step, max_steps, KIND_SSSP, (uint64_t)o.directed * 4 + (uint64_t)a.sliced * 2 + (uint64_t)a.edge_pull)))
+        return rc;
     return finish_search(g, stats, ws.vals, INT_MAX, dist_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
 }
