@@ -26,6 +26,10 @@ from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
+# Serving runs one stream per (graph, query kind) engine (18 for six graphs); with HIP's
+# default of 4 hardware queues per process, short rounds queue behind long ones.  Read at
+# runtime initialisation, so it is set before the library loads.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 import numpy as np  # noqa: E402
 

@@ -7,6 +7,7 @@ namespace mg {
 
 constexpr int SB_TASK = 32;  // batched SSSP: edges per wave task
 constexpr int SB_RING = 16;  // batched SSSP: bucket bitmaps in the ring
+constexpr int SB_PAIR_MAX = 16;  // batched SSSP: vertices with at most this many live slots relax per pair
 
 
 struct M8 {
@@ -110,6 +111,8 @@ struct BatchDev {
     uint4 *V = nullptr, *F = nullptr, *N = nullptr, *FAR = nullptr;
     int *D = nullptr;                            // SSSP distances, n x KS
     int *rows = nullptr;                         // S x n results
+    int2 *pairs = nullptr;                       // SSSP: (vertex, slot) pairs of sparse vertices
+    int64_t pair_cap = 0;
     int nwords = 0;
 };
 

@@ -490,7 +490,7 @@ int64_t batch_dev_bytes(const mg_graph *g, bool sssp, int slots) {
     int64_t b = (int64_t)sizeof(Ctl) + sizeof(SlotCtl) + 3LL * MAX_GRID * sizeof(LevelPart) + 32LL * MAX_GRID;
     b += 2 * 4 * nwords + 3 * (32 * n + 32) + 4LL * slots * n + 4;
     if (sssp) {
-        b += (32 * n + 32) + 4LL * SB_RING * nwords + 4LL * KS * n + 64;
+        b += (32 * n + 32) + 4LL * SB_RING * nwords + 4LL * KS * n + 64 + 8LL * 4 * n;
         b += (int64_t)sizeof(ChunkRec) * (g->m / SB_TASK + n + 64);
     } else {
         b += 4 * n + 4 + (int64_t)sizeof(ChunkRec) * (2 * (g->m / TASK) + 64);
@@ -513,6 +513,7 @@ void batch_dev_free(BatchDev &b) {
     free_dev(b.N);
     free_dev(b.FAR);
     free_dev(b.D);
+    free_dev(b.pairs);
     free_dev(b.rows);
     if (b.h_ctl) (void)hipHostFree(b.h_ctl);
     if (b.h_sc) (void)hipHostFree(b.h_sc);
@@ -547,6 +548,8 @@ int batch_dev_alloc(BatchDev &b, mg_graph *g, bool sssp, int slots) {
         A((void **)&b.FAR, 32 * n + 32);
         A((void **)&b.ring, 4 * (size_t)SB_RING * b.nwords);
         A((void **)&b.D, 4 * (size_t)b.KS * n + 64);
+        b.pair_cap = 4 * (int64_t)n;  // more pairs in one iteration fall back to wave tasks
+        A((void **)&b.pairs, sizeof(int2) * (size_t)b.pair_cap);
         A((void **)&b.V, 32);  // unused by SSSP
     } else {
         A((void **)&b.V, 32 * n + 32);

@@ -47,6 +47,8 @@ struct SBatchArgs {
     LevelPart *part;       // [0, G): finalize counts, [G, 2G): relax counts
     uint4 *slot_or;        // per-workgroup OR of new frontier masks
     int *out;              // S x n rows
+    int2 *pairs;           // (vertex, slot) pairs of sparse vertices, counted in ctl->nhard
+    int64_t pair_cap;
     int n, nwords, KS, G;
     long long delta;       // bucket width (LLONG_MAX / 4: one bucket, plain Bellman-Ford)
 };
@@ -75,6 +77,7 @@ __global__ void sb_init(SBatchArgs a, int iter_limit) {
         ctl_reset(a.ctl, a.n, iter_limit, 0);  // idle until a slot is seeded
         a.ctl->thr = a.delta;
         a.ctl->far_hi = 0;
+        a.ctl->pair_end = INT_MAX;
     }
 }
 
@@ -141,6 +144,8 @@ __global__ __launch_bounds__(SMALL_BLK) void sb_close(SBatchArgs a) {
         c->far_hi = max(c->far_hi, hi);
         c->pending = 0;
         c->nchunks = 0;
+        c->nhard = 0;
+        c->pair_end = INT_MAX;
         c->level = L + 1;
         c->nf_prev = (int)nf;
         c->aux = 0;
@@ -245,19 +250,49 @@ __global__ __launch_bounds__(BLK) void sb_activate(SBatchArgs a) {
     });
 }
 
+// Active vertices become work: a vertex with few live slots and a short list becomes
+// (vertex, slot) pairs, one lane each in sb_relax; others are cut into SB_TASK-edge
+// tasks for whole waves (lanes = slots).  Pairs that do not fit the pair buffer
+// fall back to tasks.
 __global__ __launch_bounds__(BLK) void sb_tasks(SBatchArgs a) {
     Ctl *c = a.ctl;
     if (c->done) return;
     const int L = c->level;
     __shared__ int s_w[BLK / WAVE];
-    __shared__ int s_tb;
+    __shared__ int s_tb, s_pb;
     for_each_bit(a.act[L & 1], a.nwords, [&](int v) {
-        int beg = 0, deg = 0;
+        int beg = 0, deg = 0, pc = 0;
+        M8 fv = zero8s();
         if (v >= 0) {
             beg = a.off[v];
             deg = a.off[v + 1] - beg;
+            fv = ld8(a.F, v);
+            pc = popc8(fv);
         }
-        const int nt = (deg + SB_TASK - 1) / SB_TASK;
+        bool as_pairs = v >= 0 && deg > 0 && deg <= LANE_T && pc <= SB_PAIR_MAX;
+        int ptot;
+        const int pex = block_exclusive_scan<BLK>(as_pairs ? pc : 0, s_w, &ptot);
+        if (ptot > 0) {
+            if (threadIdx.x == 0) {
+                s_pb = atomicAdd(&c->nhard, ptot);
+                if ((int64_t)s_pb + ptot > a.pair_cap) atomicMin(&c->pair_end, s_pb);
+            }
+            __syncthreads();
+            const bool fits = (int64_t)s_pb + ptot <= a.pair_cap;
+            if (fits && as_pairs) {
+                int k = s_pb + pex;
+                for (int w = 0; w < 8; ++w) {
+                    uint32_t b = word8(fv, w);
+                    while (b) {
+                        a.pairs[k++] = make_int2(v, w * 32 + __ffs(b) - 1);
+                        b &= b - 1;
+                    }
+                }
+            }
+            if (!fits) as_pairs = false;
+            __syncthreads();
+        }
+        const int nt = (!as_pairs && deg > 0) ? (deg + SB_TASK - 1) / SB_TASK : 0;
         int tot;
         const int tex = block_exclusive_scan<BLK>(nt, s_w, &tot);
         if (tot > 0) {
@@ -344,6 +379,44 @@ __global__ __launch_bounds__(BLK) void sb_relax(SBatchArgs a) {
 #pragma unroll
                     for (int o = 1; o < WAVE; o <<= 1) slots |= __shfl_xor(slots, o);
                     if (lane < SB_RING && (slots >> lane) & 1) mark_active(a.ring + (int64_t)lane * a.nwords, u[j]);
+                }
+            }
+        }
+    }
+    // pairs: one lane walks a sparse vertex's list for one slot
+    const int npairs = (int)min(min((int64_t)c->nhard, (int64_t)c->pair_end), a.pair_cap);
+    for (int p = blockIdx.x * BLK + (int)threadIdx.x; p < npairs; p += gridDim.x * BLK) {
+        const int2 pr = a.pairs[p];
+        const int v = pr.x, sl = pr.y;
+        const long long offs = a.sc->off[sl];
+        const long long dvs = a.D[(int64_t)v * a.KS + sl];
+        const uint32_t bit = 1u << (sl & 31);
+        const int beg = a.off[v], end = a.off[v + 1];
+        ex += end - beg;
+        for (int e = beg; e < end; e += 4) {
+            int u[4], wt[4], du[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                u[j] = e + j < end ? a.adj[e + j] : -1;
+                wt[j] = u[j] >= 0 ? a.w[e + j] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) du[j] = u[j] >= 0 ? a.D[(int64_t)u[j] * a.KS + sl] : INT_MIN;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long long cst = dvs + wt[j];
+                if (u[j] < 0 || cst >= du[j]) continue;
+                atomicMin(a.D + (int64_t)u[j] * a.KS + sl, (int)cst);
+                const long long vd = cst + offs;
+                uint32_t *mask = reinterpret_cast<uint32_t *>((vd < T ? a.N : a.FAR) + 2 * u[j]) + (sl >> 5);
+                if (!(*mask & bit)) atomicOr(mask, bit);
+                if (vd < T) {
+                    mark_active(act_next, u[j]);
+                } else {
+                    const long long bk = vd / delta;
+                    mark_active(a.ring + (int64_t)(bk % SB_RING) * a.nwords, u[j]);
+                    hi = max(hi, bk);
+                    if ((unsigned long long)bk > a.sc->far_hi[sl]) atomicMax(&a.sc->far_hi[sl], (unsigned long long)bk);
                 }
             }
         }
@@ -465,6 +538,8 @@ static SBatchArgs sssp_args(BatchDev &b, mg_graph *g, long long delta) {
     a.part = b.part;
     a.slot_or = b.slot_or;
     a.out = b.rows;
+    a.pairs = b.pairs;
+    a.pair_cap = b.pair_cap;
     a.n = g->n;
     a.nwords = b.nwords;
     a.KS = b.KS;

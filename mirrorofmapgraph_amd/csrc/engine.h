@@ -77,7 +77,8 @@ struct Ctl {
     long long mf_cur;         // out-edges of the current frontier
     long long thr;            // bucketed SSSP: distances below thr are in the current bucket
     long long far_hi;         // bucketed SSSP: highest bucket holding a waiting distance
-    long long pad1[1];
+    int pair_end;             // batched SSSP: pair slots below this are valid this iteration
+    int pad2;
 };
 
 // Pooled per-search device state, grown on demand and reused across searches.
