@@ -69,6 +69,37 @@ __device__ __forceinline__ int block_exclusive_scan(int x, int *s_wave, int *tot
     return wave_off + incl - x;
 }
 
+// p[i] when ok, else dflt, with the load made unconditional (from p[safe] when !ok;
+// safe must be in bounds).  A conditional load compiles to a branch that waits for
+// the load before the next one; loads written this way issue back to back, so a
+// batch of gathers is in flight together.
+template <typename T, typename I>
+__device__ __forceinline__ T load_or(const T *p, I i, bool ok, I safe, T dflt) {
+    const T v = p[ok ? i : safe];
+    return ok ? v : dflt;
+}
+
+// Inclusive minimum scan over runs of lanes: lane l gets the minimum of x over lanes
+// [start, l], where start <= l is the first lane of l's run (runs are contiguous).
+// DPP row shifts within rows of 16 lanes, then the row_bcast:15 / row_bcast:31 steps
+// carry row ends across rows; lanes whose source lies outside the run ignore it.
+__device__ __forceinline__ int seg_min_scan(int x, int lane, int start) {
+    int y;
+    y = __builtin_amdgcn_update_dpp(INT32_MAX, x, 0x111, 0xF, 0xF, false);  // row_shr:1
+    if (lane - 1 >= start) x = y < x ? y : x;
+    y = __builtin_amdgcn_update_dpp(INT32_MAX, x, 0x112, 0xF, 0xF, false);  // row_shr:2
+    if (lane - 2 >= start) x = y < x ? y : x;
+    y = __builtin_amdgcn_update_dpp(INT32_MAX, x, 0x114, 0xF, 0xF, false);  // row_shr:4
+    if (lane - 4 >= start) x = y < x ? y : x;
+    y = __builtin_amdgcn_update_dpp(INT32_MAX, x, 0x118, 0xF, 0xF, false);  // row_shr:8
+    if (lane - 8 >= start) x = y < x ? y : x;
+    y = __builtin_amdgcn_update_dpp(INT32_MAX, x, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
+    if ((lane & ~15) - 1 >= start && (lane & 16)) x = y < x ? y : x;
+    y = __builtin_amdgcn_update_dpp(INT32_MAX, x, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
+    if (lane >= 32 && 31 >= start) x = y < x ? y : x;
+    return x;
+}
+
 __device__ __forceinline__ bool test_bit(const uint32_t *bm, int v) {
     return (bm[v >> 5] >> (v & 31)) & 1u;
 }

@@ -78,7 +78,7 @@ struct Ctl {
     long long thr;            // bucketed SSSP: distances below thr are in the current bucket
     long long far_hi;         // bucketed SSSP: highest bucket holding a waiting distance
     int pair_end;             // batched SSSP: pair slots below this are valid this iteration
-    int pad2;
+    int dense;                // SSSP pull level over every in-neighbour (no frontier test)
 };
 
 // Pooled per-search device state, grown on demand and reused across searches.
@@ -115,6 +115,9 @@ struct Workspace {
     ChunkRec *pull_tasks = nullptr;
     int pull_ntasks = 0;
     const int32_t *pull_view = nullptr;
+    ChunkRec *dense_tasks = nullptr;  // dense SSSP pull tasks (pieces and vertex ranges)
+    int dense_ntasks = 0;
+    const int32_t *dense_view = nullptr;
     // XCD-sliced pull: in-lists cut by source slice (PULL_SLICES slices of slice_len
     // vertices, slice x gathered by the workgroups on XCD x)
     int32_t *slice_off = nullptr;      // (PULL_SLICES - 1) x n interior cut points
@@ -181,9 +184,15 @@ int deliver(mg_graph *g, void *dst, const void *src, size_t bytes, bool out_on_d
 // arguments are fixed per (graph, kind, key), so the executable graph is reused by
 // every later search with the same key.
 constexpr int STEPS_PER_GRAPH = 8;
-constexpr int PULL_SLICES = 8;
-constexpr int PULL_CHUNK = 256;
-constexpr int PULL_TASK = 512;   // static pull tasks: in-edges per wave (8 per lane in flight)  // edge-centric pull: in-edge slots per wave task (4 per lane)  // one source slice per XCD (workgroup b runs on XCD b % 8)
+constexpr int PULL_SLICES = 8;  // one source slice per XCD (workgroup b runs on XCD b % 8)
+constexpr int PULL_CHUNK = 256;  // edge-centric pull: in-edge slots per wave task (4 per lane)
+constexpr int PULL_TASK = 512;   // static pull tasks: in-edges per wave (8 per lane in flight)
+// Dense SSSP pull tasks: in-lists longer than DENSE_BIG are cut into DENSE_T-slot
+// pieces; the other vertices are grouped into ranges of whole bitmap words (at most
+// DENSE_V vertices) holding about DENSE_T in-edges.
+constexpr int DENSE_BIG = 512;
+constexpr int DENSE_T = 2048;
+constexpr int DENSE_V = 256;
 enum { KIND_BFS = 0, KIND_SSSP = 1, KIND_PR = 2, KIND_GAS = 3, KIND_BATCH = 4, KIND_BATCH_SSSP = 5, NKINDS = 8 };
 
 struct StepGraph {
@@ -201,6 +210,9 @@ bool pull_slicing_enabled();
 // MG_EDGE_PULL=1 selects edge-centric pull levels (off by default: measured slower
 // than the vertex-centric pull at scale 22).
 bool edge_pull_enabled();
+// SSSP pull levels whose frontier holds more than this percentage of the edges skip
+// the frontier test (MG_SSSP_DENSE_PCT, default 0: every pull level; above 100 never).
+int sssp_dense_pct();
 
 // Generic GAS enactor over the ported programs (engine = MG_ENGINE_GAS).
 struct GasStateBase;

@@ -65,6 +65,7 @@ static void free_workspace(Workspace &ws) {
     dfree(ws.pr_head);
     dfree(ws.pr_tail);
     dfree(ws.pull_tasks);
+    dfree(ws.dense_tasks);
     dfree(ws.slice_off);
     dfree(ws.slice_min);
     dfree(ws.slice_tasks);
@@ -133,6 +134,14 @@ bool edge_pull_enabled() {
         return e && *e && *e != '0';
     }();
     return on;
+}
+
+int sssp_dense_pct() {
+    static const int pct = [] {
+        const char *e = getenv("MG_SSSP_DENSE_PCT");
+        return e && *e ? atoi(e) : 0;
+    }();
+    return pct;
 }
 
 bool pull_slicing_enabled() {

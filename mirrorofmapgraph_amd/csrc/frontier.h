@@ -333,10 +333,7 @@ __device__ void expand_tasks(const FrontierArgs &fa, P &pol, int L) {
 // ids loaded together: out[k] = adj[slot_k] (or -1 past the end).
 __device__ __forceinline__ void load4(const int *adj, int e, int end, int stride, int (&u)[4]) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int s = e + k * stride;
-        u[k] = s < end ? adj[s] : -1;
-    }
+    for (int k = 0; k < 4; ++k) u[k] = load_or(adj, e + k * stride, e + k * stride < end, e, -1);  // e < end
 }
 
 // The edge-chunk records of the level, one workgroup per record at a time.
@@ -437,6 +434,7 @@ __device__ __forceinline__ void ctl_reset(Ctl *c, int n, int iter_limit, int fro
     c->edges_examined = 0;
     c->dsum = 0;
     c->mf_cur = 0;
+    c->dense = 0;
 }
 
 // Advances the control block past level L given the next frontier's size nf and
@@ -463,6 +461,11 @@ int launch_fill(mg_graph *g, int *p, int v, int64_t n);
 int launch_fill(mg_graph *g, uint32_t *p, uint32_t v, int64_t n);
 int launch_reached_stats(mg_graph *g, const int *vals, int unreached, mg_stats *st);
 int ensure_pull_tasks(mg_graph *g, const int *d_in_off);
+// Tasks of the dense SSSP pull levels: {v, beg, end} = slots [beg, end) of the
+// in-list of a vertex v with more than DENSE_BIG in-edges (DENSE_T per piece), or
+// {-1, va, vb} = the vertices [va, vb) (va a multiple of 32, vb too or n, at most
+// DENSE_V of them) whose in-lists are at most DENSE_BIG long, about DENSE_T edges in all.
+int ensure_dense_tasks(mg_graph *g, const int *d_in_off);
 // Per-slot owner array of the in-edge view (d_in_off) for edge-centric pulls.
 int ensure_edge_dst(mg_graph *g, const int *d_in_off);
 // Builds the XCD-sliced pull structures for the in-edge view (d_in_off, d_in_adj);

@@ -103,26 +103,73 @@ int ensure_edge_dst(mg_graph *g, const int *d_in_off) {
     return MG_OK;
 }
 
-int ensure_pull_tasks(mg_graph *g, const int *d_in_off) {
-    Workspace &ws = g->ws;
-    if (ws.pull_view == d_in_off) return MG_OK;
+// Static pull tasks: in-lists longer than `longer` cut into PULL_TASK-slot pieces.
+static int build_pull_tasks(mg_graph *g, const int *d_in_off, int longer, ChunkRec **tasks, int *ntasks,
+                            const int32_t **view) {
+    if (*view == d_in_off) return MG_OK;
     const int n = g->n;
     std::vector<int32_t> off((size_t)n + 1);
     MG_HIP(hipMemcpy(off.data(), d_in_off, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
     std::vector<ChunkRec> t;
     for (int v = 0; v < n; ++v) {
-        if (off[v + 1] - off[v] <= LANE_T) continue;
+        if (off[v + 1] - off[v] <= longer) continue;
         for (int b = off[v]; b < off[v + 1]; b += PULL_TASK)
             t.push_back(ChunkRec{v, b, std::min(b + PULL_TASK, off[v + 1]), 2});
     }
-    free_dev(ws.pull_tasks);
-    ws.pull_tasks = nullptr;
-    ws.pull_view = nullptr;
-    int rc = alloc_chunks(&ws.pull_tasks, t.size());
+    free_dev(*tasks);
+    *tasks = nullptr;
+    *view = nullptr;
+    int rc = alloc_chunks(tasks, t.size());
     if (rc) return rc;
-    if (!t.empty()) MG_HIP(hipMemcpy(ws.pull_tasks, t.data(), sizeof(ChunkRec) * t.size(), hipMemcpyHostToDevice));
-    ws.pull_ntasks = (int)t.size();
-    ws.pull_view = d_in_off;
+    if (!t.empty()) MG_HIP(hipMemcpy(*tasks, t.data(), sizeof(ChunkRec) * t.size(), hipMemcpyHostToDevice));
+    *ntasks = (int)t.size();
+    *view = d_in_off;
+    return MG_OK;
+}
+
+int ensure_pull_tasks(mg_graph *g, const int *d_in_off) {
+    Workspace &ws = g->ws;
+    return build_pull_tasks(g, d_in_off, LANE_T, &ws.pull_tasks, &ws.pull_ntasks, &ws.pull_view);
+}
+
+int ensure_dense_tasks(mg_graph *g, const int *d_in_off) {
+    Workspace &ws = g->ws;
+    if (ws.dense_view == d_in_off) return MG_OK;
+    const int n = g->n;
+    std::vector<int32_t> off((size_t)n + 1);
+    MG_HIP(hipMemcpy(off.data(), d_in_off, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
+    std::vector<ChunkRec> t;
+    // pieces of the long lists
+    for (int v = 0; v < n; ++v)
+        if (off[v + 1] - off[v] > DENSE_BIG)
+            for (int b = off[v]; b < off[v + 1]; b += DENSE_T)
+                t.push_back(ChunkRec{v, b, std::min(b + DENSE_T, off[v + 1]), 0});
+    // ranges of whole words over the other vertices (ranges without in-edges are dropped)
+    int va = 0;
+    int64_t edges = 0;
+    for (int w0 = 0; w0 < n; w0 += 32) {
+        const int w1 = std::min(w0 + 32, n);
+        int64_t we = 0;
+        for (int v = w0; v < w1; ++v) {
+            const int d = off[v + 1] - off[v];
+            if (d <= DENSE_BIG) we += d;
+        }
+        if (w0 > va && (edges + we > DENSE_T || w0 - va >= DENSE_V)) {
+            if (edges) t.push_back(ChunkRec{-1, va, w0, 0});
+            va = w0;
+            edges = 0;
+        }
+        edges += we;
+    }
+    if (edges) t.push_back(ChunkRec{-1, va, n, 0});
+    free_dev(ws.dense_tasks);
+    ws.dense_tasks = nullptr;
+    ws.dense_view = nullptr;
+    int rc = alloc_chunks(&ws.dense_tasks, t.size());
+    if (rc) return rc;
+    if (!t.empty()) MG_HIP(hipMemcpy(ws.dense_tasks, t.data(), sizeof(ChunkRec) * t.size(), hipMemcpyHostToDevice));
+    ws.dense_ntasks = (int)t.size();
+    ws.dense_view = d_in_off;
     return MG_OK;
 }
 

@@ -38,10 +38,13 @@ struct SsspArgs {
     int *slice_min;              // PULL_SLICES x n partial minima
     const ChunkRec *slice_tasks; // long ranges, grouped by slice
     int sto[PULL_SLICES + 1];    // slice task offsets
-    // edge-centric pull (edge_pull = 1): waves walk in-edge slots, whatever vertex owns them
-    int edge_pull;
-    const int *edge_dst;         // owner of each in-edge slot
-    long long m_in;              // in-edge slots
+    // dense pull levels: the dense tasks, all in one launch (null: dense levels use the
+    // LANE_T split above)
+    const ChunkRec *dpull;
+    int ndpull;
+    // dense pull: a pull level whose frontier holds more than dense_pct % of the edges
+    // reads every in-neighbour's distance without the frontier test (> 100: never)
+    int dense_pct;
 };
 
 __device__ __forceinline__ long long edge_w(const SsspArgs &a, int view, int slot) {
@@ -73,17 +76,32 @@ struct SsspPush {
     // four targets, weights and target distances in flight together
     __device__ void visit_range(int view, int, int d, int beg, int end, int stride, int L) {
         const int *adj = a.fa.adj[view];
+        const int *eid = a.fa.eid[view];
         for (int e = beg; e < end; e += 4 * stride) {
-            int u[4], wt[4], du[4];
-            load4(adj, e, end, stride, u);
+            int sl[4], u[4], wt[4], du[4];
+            bool ok[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                wt[k] = u[k] >= 0 ? (int)edge_w(a, view, e + k * stride) : 0;
-                du[k] = u[k] >= 0 ? a.dist[u[k]] : INT_MIN;
+                ok[k] = e + k * stride < end;
+                sl[k] = ok[k] ? e + k * stride : e;  // e < end: in bounds
             }
 #pragma unroll
+            for (int k = 0; k < 4; ++k) u[k] = adj[sl[k]];
+            if (eid) {
+                int ei[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) ei[k] = eid[sl[k]];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) wt[k] = a.w[ei[k]];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) wt[k] = a.w[sl[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) du[k] = a.dist[u[k]];
+#pragma unroll
             for (int k = 0; k < 4; ++k)
-                if (u[k] >= 0) {
+                if (ok[k]) {
                     ++ex;
                     relax((long long)d + wt[k], du[k], u[k], L);
                 }
@@ -94,25 +112,236 @@ struct SsspPush {
 // Pull: the best dist[u] + w over in-edge slots [beg, end) from frontier vertices u.
 // Loads go out PULL_U at a time: neighbour ids, then their frontier bits, then the
 // distances and weights of the frontier ones (three dependent round trips per batch).
+// Dense levels (ALL) skip the frontier test and take every in-neighbour: a vertex u
+// outside the frontier has relaxed its out-edges with its present distance already
+// (its last change put it in a frontier), so its candidate dist[u] + w cannot beat
+// dist[v] and the minimum is unchanged; unreached u give INT_MAX + w.  That trades
+// a bitmap gather per edge for a distance gather per non-frontier edge, a gain once
+// the frontier holds most edges.
 constexpr int PULL_U = 8;
-__device__ __forceinline__ long long pull_best(const SsspArgs &a, const uint32_t *fr, int beg, int end, int step) {
+template <bool ALL, bool EID>
+__device__ __forceinline__ long long pull_best_t(const SsspArgs &a, const uint32_t *fr, int beg, int end, int step) {
     long long best = INT_MAX;
     const int *adj = a.fa.adj[2];
+    const int *eid = a.fa.eid[2];
     for (int e = beg; e < end; e += PULL_U * step) {
-        int u[PULL_U];
+        int sl[PULL_U], u[PULL_U], wt[PULL_U];
+        bool ok[PULL_U];
 #pragma unroll
-        for (int k = 0; k < PULL_U; ++k) u[k] = e + k * step < end ? adj[e + k * step] : -1;
-        bool f[PULL_U];
+        for (int k = 0; k < PULL_U; ++k) {
+            ok[k] = e + k * step < end;
+            sl[k] = ok[k] ? e + k * step : e;  // e < end: in bounds
+        }
 #pragma unroll
-        for (int k = 0; k < PULL_U; ++k) f[k] = u[k] >= 0 && test_bit(fr, u[k]);
+        for (int k = 0; k < PULL_U; ++k) u[k] = adj[sl[k]];
+        if (EID) {
+            int ei[PULL_U];
+#pragma unroll
+            for (int k = 0; k < PULL_U; ++k) ei[k] = eid[sl[k]];
+#pragma unroll
+            for (int k = 0; k < PULL_U; ++k) wt[k] = a.w[ei[k]];
+        } else {
+#pragma unroll
+            for (int k = 0; k < PULL_U; ++k) wt[k] = a.w[sl[k]];
+        }
         long long cand[PULL_U];
+        if (ALL) {
+            int du[PULL_U];
 #pragma unroll
-        for (int k = 0; k < PULL_U; ++k)
-            cand[k] = f[k] ? (long long)a.dist[u[k]] + edge_w(a, 2, e + k * step) : (long long)INT_MAX;
+            for (int k = 0; k < PULL_U; ++k) du[k] = a.dist[u[k]];
+#pragma unroll
+            for (int k = 0; k < PULL_U; ++k) cand[k] = ok[k] ? (long long)du[k] + wt[k] : (long long)INT_MAX;
+        } else {
+            uint32_t wd[PULL_U];
+#pragma unroll
+            for (int k = 0; k < PULL_U; ++k) wd[k] = fr[u[k] >> 5];
+#pragma unroll
+            for (int k = 0; k < PULL_U; ++k)
+                cand[k] = ok[k] && ((wd[k] >> (u[k] & 31)) & 1u) ? (long long)a.dist[u[k]] + wt[k] : (long long)INT_MAX;
+        }
 #pragma unroll
         for (int k = 0; k < PULL_U; ++k) best = cand[k] < best ? cand[k] : best;
     }
     return best;
+}
+template <bool ALL>
+__device__ __forceinline__ long long pull_best(const SsspArgs &a, const uint32_t *fr, int beg, int end, int step) {
+    return a.fa.eid[2] ? pull_best_t<ALL, true>(a, fr, beg, end, step) : pull_best_t<ALL, false>(a, fr, beg, end, step);
+}
+__device__ __forceinline__ long long pull_best(const SsspArgs &a, const uint32_t *fr, int beg, int end, int step,
+                                               bool all) {
+    return all ? pull_best<true>(a, fr, beg, end, step) : pull_best<false>(a, fr, beg, end, step);
+}
+
+// Dense pull level: every vertex takes min(dist[u] + w) over all its in-edges (no
+// frontier test, see pull_best), in one launch over the static dense tasks:
+//   * a piece of a long list: a wave minimum, one atomicMin into dist[v];
+//   * a range of up to DENSE_V vertices of whole bitmap words with short lists: the
+//     wave walks the concatenation of their in-lists in windows of 64 slots, one per
+//     lane; a lane finds its slot's owner by binary search in the lists' prefix sums
+//     (LDS), so neighbour ids and weights are coalesced loads and the neighbour
+//     distances the only gathers, DENSE_WIN windows in flight together.  Owners are
+//     non-decreasing along a window, so a segmented minimum scan across the wave
+//     reduces each owner's run and its last lane folds it into the owner's LDS
+//     minimum; the range's own vertices are then updated with plain stores (no other
+//     task writes them) and their frontier words set with one atomicOr each (long-list
+//     vertices of the same words are marked by their pieces).
+constexpr int DENSE_WIN = 4;
+constexpr int DENSE_PER = DENSE_V / WAVE;  // vertices per lane in a range task
+struct DenseLds {
+    int pre[BLK / WAVE][DENSE_V + 1];
+    int beg[BLK / WAVE][DENSE_V];
+    int min[BLK / WAVE][DENSE_V];
+};
+template <bool EID>
+__device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, int vb, DenseLds &lds, long long &nf,
+                                            long long &mf, long long &ex) {
+    const FrontierArgs &fa = a.fa;
+    uint32_t *nx = fa.bm[(L + 1) % 3];
+    const int *adj = fa.adj[2];
+    const int *eid = fa.eid[2];
+    const int *off = fa.off[2];
+    const int lane = lane_id(), wv = (int)threadIdx.x / WAVE;
+    int *pre = lds.pre[wv], *begs = lds.beg[wv], *mins = lds.min[wv];
+    const uint64_t upto = lane == WAVE - 1 ? ~0ull : ((2ull << lane) - 1);  // lanes 0..lane
+    // vertex va + 64 j + lane is list 64 j + lane
+    int beg[DENSE_PER], len[DENSE_PER];
+#pragma unroll
+    for (int j = 0; j < DENSE_PER; ++j) {
+        const int v = va + j * WAVE + lane;
+        const bool in = v < vb;
+        const int vv = in ? v : va;
+        const int o0 = off[vv], o1 = off[vv + 1];
+        beg[j] = o0;
+        len[j] = in && o1 - o0 <= DENSE_BIG ? o1 - o0 : 0;
+    }
+    int total = 0;
+#pragma unroll
+    for (int j = 0; j < DENSE_PER; ++j) {
+        int incl = len[j];
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const int y = __shfl_up(incl, d);
+            if (lane >= d) incl += y;
+        }
+        pre[j * WAVE + lane + 1] = total + incl;
+        begs[j * WAVE + lane] = beg[j];
+        mins[j * WAVE + lane] = INT_MAX;
+        total += __shfl(incl, WAVE - 1);
+    }
+    if (lane == 0) pre[0] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    ex += total;
+    // an in-bounds slot for the lanes past the end: off[va] (the windows run only when
+    // some list of the range is non-empty, and it starts at or after off[va])
+    const int safe = begs[0];
+    for (int p0 = 0; p0 < total; p0 += DENSE_WIN * WAVE) {
+        int p[DENSE_WIN], own[DENSE_WIN], slot[DENSE_WIN];
+        bool ok[DENSE_WIN];
+#pragma unroll
+        for (int k = 0; k < DENSE_WIN; ++k) {
+            p[k] = p0 + k * WAVE + lane;
+            ok[k] = p[k] < total;
+            own[k] = 0;
+        }
+        // owner = last list whose prefix is <= p (binary search, all windows in step)
+#pragma unroll
+        for (int st = DENSE_V / 2; st > 0; st >>= 1) {
+            int q[DENSE_WIN];
+#pragma unroll
+            for (int k = 0; k < DENSE_WIN; ++k) q[k] = pre[own[k] + st];
+#pragma unroll
+            for (int k = 0; k < DENSE_WIN; ++k)
+                if (q[k] <= p[k]) own[k] += st;
+        }
+        int first[DENSE_WIN], nextp[DENSE_WIN];
+#pragma unroll
+        for (int k = 0; k < DENSE_WIN; ++k) {
+            first[k] = pre[own[k]];
+            nextp[k] = pre[own[k] + 1];
+            slot[k] = ok[k] ? begs[own[k]] + (p[k] - first[k]) : safe;
+        }
+        int u[DENSE_WIN], wt[DENSE_WIN];
+#pragma unroll
+        for (int k = 0; k < DENSE_WIN; ++k) u[k] = adj[slot[k]];
+        if (EID) {
+            int ei[DENSE_WIN];
+#pragma unroll
+            for (int k = 0; k < DENSE_WIN; ++k) ei[k] = eid[slot[k]];
+#pragma unroll
+            for (int k = 0; k < DENSE_WIN; ++k) wt[k] = a.w[ei[k]];
+        } else {
+#pragma unroll
+            for (int k = 0; k < DENSE_WIN; ++k) wt[k] = a.w[slot[k]];
+        }
+        int du[DENSE_WIN];
+#pragma unroll
+        for (int k = 0; k < DENSE_WIN; ++k) du[k] = a.dist[u[k]];
+#pragma unroll
+        for (int k = 0; k < DENSE_WIN; ++k) {
+            const long long d = (long long)du[k] + wt[k];
+            int best = ok[k] && d < INT_MAX ? (int)d : INT_MAX;
+            // runs: one per owner; the lanes past the end form one more run
+            const bool head = lane == 0 || (ok[k] ? p[k] == first[k] : p[k] == total);
+            const uint64_t heads = wave_ballot(head);
+            const int start = 63 - __builtin_clzll(heads & upto);
+            best = seg_min_scan(best, lane, start);
+            const bool tail = lane == WAVE - 1 || p[k] + 1 == nextp[k];
+            if (ok[k] && tail && best < INT_MAX) atomicMin(&mins[own[k]], best);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int j = 0; j < DENSE_PER; ++j) {
+        const int v = va + j * WAVE + lane;
+        bool found = false;
+        if (len[j] > 0) {
+            const int best = mins[j * WAVE + lane];
+            if (best < a.dist[v]) {
+                a.dist[v] = best;
+                found = true;
+                ++nf;
+                mf += view_degree(fa, v);
+            }
+        }
+        const uint64_t fm = wave_ballot(found);
+        const uint32_t mine = (lane < 32) ? (uint32_t)fm : (uint32_t)(fm >> 32);
+        if ((lane & 31) == 0 && mine) atomicOr(&nx[v >> 5], mine);
+    }
+    // the next task rewrites this wave's LDS rows
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <bool EID>
+__device__ __forceinline__ void dense_pull(const SsspArgs &a, int L, DenseLds &lds, long long &nf, long long &mf,
+                                           long long &ex) {
+    const int lane = lane_id();
+    const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
+    for (int t = wave; t < a.ndpull; t += nwaves) {
+        const ChunkRec r = a.dpull[t];
+        if (r.v < 0) {
+            dense_range<EID>(a, L, r.beg, r.end, lds, nf, mf, ex);
+            continue;
+        }
+        ex += (r.end - r.beg + WAVE - 1 - lane) / WAVE;
+        long long best = pull_best_t<true, EID>(a, nullptr, r.beg + lane, r.end, WAVE);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const long long y = __shfl_xor(best, o);
+            best = y < best ? y : best;
+        }
+        if (lane == 0 && best < a.dist[r.v]) {
+            const int old = atomicMin(&a.dist[r.v], (int)best);
+            if (best < old && set_bit_first(a.fa.bm[(L + 1) % 3], r.v)) {
+                ++nf;
+                mf += view_degree(a.fa, r.v);
+            }
+        }
+    }
 }
 
 __device__ void sssp_close(const SsspArgs &a, int L, long long nf, long long mf) {
@@ -125,6 +354,7 @@ __device__ void sssp_close(const SsspArgs &a, int L, long long nf, long long mf)
         else if (dir == DIR_PULL && mf * c->beta < m) dir = DIR_PUSH;
     }
     c->dir = dir;
+    c->dense = dir == DIR_PULL && mf * 100 > c->m_unexplored * a.dense_pct;
     advance_level(c, L, nf, mf);
 }
 
@@ -230,70 +460,8 @@ __global__ __launch_bounds__(BLK) void sssp_level_a(SsspArgs a, int view) {
         write_part<BLK>(fa, view * a.G + blockIdx.x, pol.nf, pol.mf, pol.ex);
         return;
     }
-    if (a.edge_pull) {
-        // Edge-centric pull: one wave per PULL_CHUNK consecutive in-edge slots, four per
-        // lane (all loads in flight), whatever vertices own them.  Candidates dist[u] + w
-        // from frontier in-neighbours u are staged in LDS; the first slot of each run of
-        // one owner v takes the run's minimum and, if it beats dist[v], lowers it with
-        // one atomicMin (min is order-free, so the result is exact) and marks v.
-        __shared__ int s_d[BLK / WAVE][PULL_CHUNK];
-        __shared__ int s_c[BLK / WAVE][PULL_CHUNK];
-        const uint32_t *fr = fa.bm[L % 3];
-        uint32_t *nx = fa.bm[(L + 1) % 3];
-        const int *adj = fa.adj[2];
-        const int lane = lane_id(), wv = (int)threadIdx.x / WAVE;
-        const long long nch = (a.m_in + PULL_CHUNK - 1) / PULL_CHUNK;
-        const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
-        long long nf = 0, mf = 0, ex = 0;
-        for (long long ch = wave; ch < nch; ch += nwaves) {
-            const long long e0 = ch * PULL_CHUNK + 4 * lane;
-            int u[4], d[4], c[4];
-            if (e0 + 3 < a.m_in) {
-                const int4 u4 = *reinterpret_cast<const int4 *>(adj + e0);
-                const int4 d4 = *reinterpret_cast<const int4 *>(a.edge_dst + e0);
-                u[0] = u4.x; u[1] = u4.y; u[2] = u4.z; u[3] = u4.w;
-                d[0] = d4.x; d[1] = d4.y; d[2] = d4.z; d[3] = d4.w;
-            } else {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    u[j] = e0 + j < a.m_in ? adj[e0 + j] : -1;
-                    d[j] = e0 + j < a.m_in ? a.edge_dst[e0 + j] : -1;
-                }
-            }
-            bool f[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) f[j] = u[j] >= 0 && test_bit(fr, u[j]);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                long long cand = INT_MAX;
-                if (f[j]) cand = (long long)a.dist[u[j]] + edge_w(a, 2, (int)(e0 + j));
-                c[j] = cand < INT_MAX ? (int)cand : INT_MAX;
-                s_d[wv][4 * lane + j] = d[j];
-                s_c[wv][4 * lane + j] = c[j];
-            }
-            ex += (e0 < a.m_in) ? min(4LL, a.m_in - e0) : 0;
-            // the wave's LDS writes before any lane reads its neighbours' slots
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int i = 4 * lane + j;
-                const int v = d[j];
-                if (v < 0 || (i > 0 && s_d[wv][i - 1] == v)) continue;  // not the first slot of its run
-                int best = INT_MAX;
-                for (int k = i; k < PULL_CHUNK && s_d[wv][k] == v; ++k) best = min(best, s_c[wv][k]);
-                if (best < a.dist[v]) {
-                    const int old = atomicMin(&a.dist[v], best);
-                    if (best < old && set_bit_first(nx, v)) {
-                        ++nf;
-                        mf += view_degree(fa, v);
-                    }
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-        }
-        write_part<BLK>(fa, blockIdx.x, nf, mf, ex);
+    if (c->dense && a.dpull) {
+        write_part<BLK>(fa, blockIdx.x, 0, 0, 0);  // the level is sssp_level_b's
         return;
     }
     if (a.sliced) {
@@ -315,7 +483,7 @@ __global__ __launch_bounds__(BLK) void sssp_level_a(SsspArgs a, int view) {
                 long long best = INT_MAX;
                 if (end - beg <= LANE_T) {
                     ex += end - beg;
-                    best = pull_best(a, fr, beg, end, 1);
+                    best = pull_best<false>(a, fr, beg, end, 1);
                 }
                 pm[v] = (int)best;  // long ranges stay INT_MAX until their tasks
             }
@@ -329,6 +497,7 @@ __global__ __launch_bounds__(BLK) void sssp_level_a(SsspArgs a, int view) {
     const uint32_t *fr = fa.bm[L % 3];
     uint32_t *nx = fa.bm[(L + 1) % 3];
     const int *off = fa.off[2];
+    const bool all = c->dense;
     long long nf = 0, mf = 0, ex = 0;
     const int lane = lane_id();
     for (int base = blockIdx.x * BLK; base < fa.n; base += gridDim.x * BLK) {
@@ -338,7 +507,7 @@ __global__ __launch_bounds__(BLK) void sssp_level_a(SsspArgs a, int view) {
             const int beg = off[v], end = off[v + 1];
             if (end - beg <= LANE_T) {
                 ex += end - beg;
-                const long long best = pull_best(a, fr, beg, end, 1);
+                const long long best = pull_best(a, fr, beg, end, 1, all);
                 if (best < a.dist[v]) {
                     a.dist[v] = (int)best;
                     found = true;
@@ -366,8 +535,10 @@ __global__ __launch_bounds__(BLK) void sssp_level_b(SsspArgs a) {
         nf = pol.nf;
         mf = pol.mf;
         ex = pol.ex;
-    } else if (a.edge_pull) {
-        // the edge-centric pull did the whole level in sssp_level_a
+    } else if (c->dense && a.dpull) {
+        __shared__ DenseLds lds;
+        if (fa.eid[2]) dense_pull<true>(a, L, lds, nf, mf, ex);
+        else dense_pull<false>(a, L, lds, nf, mf, ex);
     } else if (a.sliced) {
         // long (vertex, slice) ranges of this workgroup's slice: wave minimum, one
         // atomicMin into the slice's partial
@@ -380,7 +551,7 @@ __global__ __launch_bounds__(BLK) void sssp_level_b(SsspArgs a) {
         for (int t = a.sto[x] + wave; t < a.sto[x + 1]; t += nwaves) {
             const ChunkRec r = a.slice_tasks[t];
             ex += (r.end - r.beg + WAVE - 1 - lane) / WAVE;
-            long long best = pull_best(a, fr, r.beg + lane, r.end, WAVE);
+            long long best = pull_best<false>(a, fr, r.beg + lane, r.end, WAVE);
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) {
                 const long long y = __shfl_xor(best, o);
@@ -391,12 +562,15 @@ __global__ __launch_bounds__(BLK) void sssp_level_b(SsspArgs a) {
     } else {
         // static in-edge tasks: wave minimum, one atomicMin per task
         const uint32_t *fr = fa.bm[L % 3];
+        const bool all = c->dense;
+        const ChunkRec *tasks = a.pull;
+        const int ntasks = a.npull;
         const int lane = lane_id();
         const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
-        for (int t = wave; t < a.npull; t += nwaves) {
-            const ChunkRec r = a.pull[t];
+        for (int t = wave; t < ntasks; t += nwaves) {
+            const ChunkRec r = tasks[t];
             ex += (r.end - r.beg + WAVE - 1 - lane) / WAVE;
-            long long best = pull_best(a, fr, r.beg + lane, r.end, WAVE);
+            long long best = pull_best(a, fr, r.beg + lane, r.end, WAVE, all);
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) {
                 const long long y = __shfl_xor(best, o);
@@ -468,17 +642,13 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
     a.dist = ws.vals;
     a.w = g->w;
     a.G = std::max(PULL_SLICES, grid_blocks(std::max<int64_t>(g->n / 4, g->m / 64), BLK) / PULL_SLICES * PULL_SLICES);
+    a.dense_pct = sssp_dense_pct();
     const int can_pull = a.fa.off[2] != nullptr && a.fa.nviews == 1 && o.engine != MG_ENGINE_GAS;
     if (can_pull) {
         if ((rc = ensure_pull_tasks(g, a.fa.off[2]))) return rc;
         a.pull = ws.pull_tasks;
         a.npull = ws.pull_ntasks;
-        if (edge_pull_enabled()) {
-            if ((rc = ensure_edge_dst(g, a.fa.off[2]))) return rc;
-            a.edge_pull = 1;
-            a.edge_dst = ws.edge_dst;
-            a.m_in = g->m;
-        } else if (pull_slicing_enabled()) {
+        if (pull_slicing_enabled()) {
             if ((rc = ensure_pull_slices(g, a.fa.off[2], a.fa.adj[2]))) return rc;
             if (ws.slice_ok) {
                 a.sliced = 1;
@@ -487,6 +657,11 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
                 a.slice_tasks = ws.slice_tasks;
                 for (int x = 0; x <= PULL_SLICES; ++x) a.sto[x] = ws.slice_task_off[x];
             }
+        }
+        if (!a.sliced && a.dense_pct <= 100) {
+            if ((rc = ensure_dense_tasks(g, a.fa.off[2]))) return rc;
+            a.dpull = ws.dense_tasks;
+            a.ndpull = ws.dense_ntasks;
         }
     }
 
@@ -507,7 +682,7 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
     };
     // Bellman-Ford with non-negative weights settles within n rounds.
     const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n + 1) + 3;
-    if ((rc = drive(g, step, max_steps, KIND_SSSP, (uint64_t)o.directed * 4 + (uint64_t)a.sliced * 2 + (uint64_t)a.edge_pull)))
+    if ((rc = drive(g, step, max_steps, KIND_SSSP, (uint64_t)o.directed * 4 + (uint64_t)a.sliced * 2 + (uint64_t)(a.dpull != nullptr))))
         return rc;
     return finish_search(g, stats, ws.vals, INT_MAX, dist_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
 }
