@@ -333,20 +333,25 @@ __global__ __launch_bounds__(BLK) void sb_relax(SBatchArgs a) {
         if (nib) dv = reinterpret_cast<const int4 *>(a.D + (int64_t)r.v * a.KS)[lane];
         ex += (long long)__popc(nib) * (r.end - r.beg);
         for (int e = r.beg; e < r.end; e += 4) {
+            // every load of the group unconditional (clamped to slot e), so they go out together
             int u[4], wt[4];
             int4 du[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                u[j] = e + j < r.end ? a.adj[e + j] : -1;
-                wt[j] = u[j] >= 0 ? a.w[e + j] : 0;
+                const int sl = e + j < r.end ? e + j : e;
+                u[j] = a.adj[sl];
+                wt[j] = a.w[sl];
+            }
+            if (nib) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) du[j] = reinterpret_cast<const int4 *>(a.D + (int64_t)u[j] * a.KS)[lane];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) du[j] = make_int4(INT_MIN, INT_MIN, INT_MIN, INT_MIN);
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                du[j] = (u[j] >= 0 && nib) ? reinterpret_cast<const int4 *>(a.D + (int64_t)u[j] * a.KS)[lane]
-                                           : make_int4(INT_MIN, INT_MIN, INT_MIN, INT_MIN);
-#pragma unroll
             for (int j = 0; j < 4; ++j) {
-                if (u[j] < 0) break;  // wave-uniform
+                if (e + j >= r.end) break;  // wave-uniform
                 uint32_t near = 0, far = 0, slots = 0;
                 if (nib) {
                     const long long cs[4] = {(long long)dv.x + wt[j], (long long)dv.y + wt[j],
@@ -397,15 +402,16 @@ __global__ __launch_bounds__(BLK) void sb_relax(SBatchArgs a) {
             int u[4], wt[4], du[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                u[j] = e + j < end ? a.adj[e + j] : -1;
-                wt[j] = u[j] >= 0 ? a.w[e + j] : 0;
+                const int s2 = e + j < end ? e + j : e;
+                u[j] = a.adj[s2];
+                wt[j] = a.w[s2];
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) du[j] = u[j] >= 0 ? a.D[(int64_t)u[j] * a.KS + sl] : INT_MIN;
+            for (int j = 0; j < 4; ++j) du[j] = a.D[(int64_t)u[j] * a.KS + sl];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const long long cst = dvs + wt[j];
-                if (u[j] < 0 || cst >= du[j]) continue;
+                if (e + j >= end || cst >= du[j]) continue;
                 atomicMin(a.D + (int64_t)u[j] * a.KS + sl, (int)cst);
                 const long long vd = cst + offs;
                 uint32_t *mask = reinterpret_cast<uint32_t *>((vd < T ? a.N : a.FAR) + 2 * u[j]) + (sl >> 5);

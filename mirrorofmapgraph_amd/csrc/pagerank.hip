@@ -154,21 +154,26 @@ __global__ __launch_bounds__(BLK) void pr_k1(PrArgs a) {
         if (end - beg > PR_T1) continue;
         float s = 0.0f;
         bool act = all;
-        // four in-edges in flight, summed in order
-        for (int e = beg; e < end; e += 4) {
+        // four in-edges in flight (full groups load unconditionally), summed in order
+        int e = beg;
+        for (; e + 4 <= end; e += 4) {
             int u[4];
             float cv[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) u[j] = e + j < end ? a.in_adj[e + j] : -1;
+            for (int j = 0; j < 4; ++j) u[j] = a.in_adj[e + j];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) cv[j] = u[j] >= 0 ? I.c_old[u[j]] : 0.0f;
+            for (int j = 0; j < 4; ++j) cv[j] = I.c_old[u[j]];
             if (!all) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) act = act || (u[j] >= 0 && test_bit(I.prev, u[j]));
+                for (int j = 0; j < 4; ++j) act = act || test_bit(I.prev, u[j]);
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (u[j] >= 0) s = s + cv[j];
+            for (int j = 0; j < 4; ++j) s = s + cv[j];
+        }
+        for (; e < end; ++e) {
+            const int u = a.in_adj[e];
+            if (!all) act = act || test_bit(I.prev, u);
+            s = s + I.c_old[u];
         }
         delta += pr_apply(a, I, v, s, act, nchanged);
     }
@@ -182,21 +187,27 @@ __device__ __forceinline__ float wave_gather(const PrArgs &a, const PrIter &I, i
     const bool all = act;  // activity already known: skip the bitmap
     float s = 0.0f;
     bool any = false;
-    // lane l sums edges l, l+64, ... in order, four in flight
-    for (int e = beg + lane; e < end; e += 4 * WAVE) {
+    // lane l sums edges l, l+64, ... in order, four in flight (full groups load
+    // unconditionally), then the remainder one at a time
+    int e = beg + lane;
+    for (; e + 3 * WAVE < end; e += 4 * WAVE) {
         int u[4];
         float cv[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) u[j] = e + j * WAVE < end ? a.in_adj[e + j * WAVE] : -1;
+        for (int j = 0; j < 4; ++j) u[j] = a.in_adj[e + j * WAVE];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) cv[j] = u[j] >= 0 ? I.c_old[u[j]] : 0.0f;
+        for (int j = 0; j < 4; ++j) cv[j] = I.c_old[u[j]];
         if (!all) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) any = any || (u[j] >= 0 && test_bit(I.prev, u[j]));
+            for (int j = 0; j < 4; ++j) any = any || test_bit(I.prev, u[j]);
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (u[j] >= 0) s = s + cv[j];
+        for (int j = 0; j < 4; ++j) s = s + cv[j];
+    }
+    for (; e < end; e += WAVE) {
+        const int u = a.in_adj[e];
+        if (!all) any = any || test_bit(I.prev, u);
+        s = s + I.c_old[u];
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s = s + __shfl_xor(s, o);

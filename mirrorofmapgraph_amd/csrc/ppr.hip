@@ -47,20 +47,25 @@ __global__ __launch_bounds__(BLK) void ppr_contrib(PprArgs a) {
 }
 
 // Sum of C over in-edges [beg, end) in order, four loads in flight.
+// Sum over in-edges [beg, end) of the lane's column, in edge order: full groups of
+// PPR_U rows with every load in flight (the group test is wave-uniform, so the
+// loads need no per-load branch), then the remainder one row at a time.
+constexpr int PPR_U = 8;
 __device__ __forceinline__ float gather_sum(const PprArgs &a, int beg, int end, int lane, bool on) {
     float acc = 0.f;
-    for (int e = beg; e < end; e += 4) {
-        float c[4];
+    int e = beg;
+    for (; e + PPR_U <= end; e += PPR_U) {
+        int v[PPR_U];
+        float c[PPR_U];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int v = e + j < end ? a.in_adj[e + j] : -1;
-            c[j] = (on && v >= 0) ? a.C[(int64_t)v * PPR_SLOTS + lane] : 0.f;
-        }
+        for (int j = 0; j < PPR_U; ++j) v[j] = a.in_adj[e + j];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (e + j < end) acc += c[j];
+        for (int j = 0; j < PPR_U; ++j) c[j] = a.C[(int64_t)v[j] * PPR_SLOTS + lane];
+#pragma unroll
+        for (int j = 0; j < PPR_U; ++j) acc += c[j];
     }
-    return acc;
+    for (; e < end; ++e) acc += a.C[(int64_t)a.in_adj[e] * PPR_SLOTS + lane];
+    return on ? acc : 0.f;
 }
 
 __device__ __forceinline__ float ppr_apply(float s, float t, bool is_src) {
