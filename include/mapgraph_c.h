@@ -68,7 +68,8 @@ int mg_device_synchronize(void);
 #define MG_GRAPH_WEIGHTED   0x1u  /* edge_values is given (int32 per CSR edge)          */
 #define MG_GRAPH_BUILD_CSC  0x2u  /* build the in-edge (CSC) view on the device         */
 #define MG_GRAPH_SYMMETRIC  0x4u  /* caller asserts the CSR is symmetric: in-edges of v
-                                     equal out-edges of v (lets pull steps read the CSR) */
+                                     equal out-edges of v, with w(u,v) == w(v,u) when
+                                     weighted (lets pull steps read the CSR) */
 
 typedef struct mg_graph mg_graph;
 
