@@ -121,7 +121,9 @@ typedef struct mg_options {
     int32_t batch_beta;     /* mg_bfs_batch pull -> push switch (frontier edges < m/batch_beta); 0 = 16  */
     int32_t sssp_delta;     /* SSSP bucket width: relax only distances below the current
                                bucket bound, raised by sssp_delta when no such distance is
-                               left; 0 = chosen from the mean weight, -1 = unbucketed     */
+                               left (mg_sssp: in its small-frontier levels; the batched
+                               searches: always); 0 = auto (mg_sssp: 4 x the largest
+                               weight; batched: about 4 x the mean weight), -1 = unbucketed */
     int32_t ppr_iters;      /* personalized PageRank iterations (0 = 20)                   */
     int32_t reserved[4];
 } mg_options;

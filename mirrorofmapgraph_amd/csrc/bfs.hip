@@ -73,6 +73,7 @@ __device__ void bfs_close(const BfsArgs &a, int L, long long nf, long long mf) {
     Ctl *c = a.fa.ctl;
     c->m_unexplored -= mf;
     int dir = c->dir;
+    trace_level(a.fa, L, dir, nf, mf);
     if (dir == DIR_BU) c->bu_levels++;
     if (c->can_bu) {
         if (dir == DIR_TD && mf > c->m_unexplored / c->alpha) dir = DIR_BU;
@@ -119,8 +120,6 @@ __global__ void bfs_seed(BfsArgs a, int src, int iter_limit, int can_bu, int alp
 // the direction test), visits claim the visited bit with one atomic (no pre-test),
 // and a frontier's bitmap words are cleared as its vertices are expanded.  Vertices
 // with more than LANE_T edges go through the LDS-scan tile path.
-constexpr int SMALL_PER = SMALL_Q / SMALL_BLK;           // queue entries per thread
-constexpr int SMALL_BIG = SMALL_E / (LANE_T + 1) + 1;     // lists > LANE_T in a frontier of <= SMALL_E edges
 
 __global__ __launch_bounds__(SMALL_BLK) void bfs_small(BfsArgs a) {
     Ctl *c = a.fa.ctl;
@@ -289,6 +288,7 @@ __global__ __launch_bounds__(SMALL_BLK) void bfs_small(BfsArgs a) {
             // close level L on the LDS copy of the direction state (as bfs_close)
             long long mu = s_mu - mf;
             int dir = s_dir;
+            trace_level(fa, L, dir, nf, mf);
             if (s_canbu && mf > mu / s_alpha) dir = DIR_BU;
             s_mu = mu;
             s_dir = dir;
@@ -532,5 +532,7 @@ extern "C" int mg_bfs(mg_graph *g, int32_t src, int32_t *depth_out, const mg_opt
     };
     const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n) + 3;
     if ((rc = drive(g, step, max_steps, KIND_BFS, (uint64_t)o.directed))) return rc;
-    return finish_search(g, stats, ws.vals, -1, depth_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
+    rc = finish_search(g, stats, ws.vals, -1, depth_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
+    if (rc == MG_OK && o.verbose) rc = print_trace(g, "bfs", "top-down", "bottom-up");
+    return rc;
 }

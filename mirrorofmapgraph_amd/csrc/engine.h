@@ -30,6 +30,8 @@ constexpr int SMALL_E = 65536;    // small-frontier kernel: max frontier edges
 constexpr int BU_INLINE = 32;     // bottom-up: edges a lane scans before handing the vertex to a wave
 constexpr int LANE_T = 32;        // lane/wave scheme: a lane walks vertices of up to LANE_T edges ...
 constexpr int TASK = 256;         // ... longer lists become TASK-edge tasks, one wave each
+constexpr int SMALL_PER = SMALL_Q / SMALL_BLK;        // small-frontier loops: queue entries per thread
+constexpr int SMALL_BIG = SMALL_E / (LANE_T + 1) + 1;  // lists > LANE_T in a frontier of <= SMALL_E edges
 
 enum { DIR_TD = 0, DIR_BU = 1 };
 
@@ -87,6 +89,7 @@ struct Workspace {
     int nwords = 0;           // 32-bit words of one vertex bitmap (rounded to even)
     int64_t chunk_cap = 0;
     int32_t *vals = nullptr;  // depth / dist / label
+    long long *trace = nullptr;  // per level: next frontier size | direction << 40, its out-edges
     int32_t *q[3] = {nullptr, nullptr, nullptr};
     int32_t *hard = nullptr;
     uint32_t *visited = nullptr;
@@ -184,6 +187,7 @@ int deliver(mg_graph *g, void *dst, const void *src, size_t bytes, bool out_on_d
 // arguments are fixed per (graph, kind, key), so the executable graph is reused by
 // every later search with the same key.
 constexpr int STEPS_PER_GRAPH = 8;
+constexpr int TRACE_CAP = 16384;  // levels recorded in the per-level trace (verbose)
 constexpr int PULL_SLICES = 8;  // one source slice per XCD (workgroup b runs on XCD b % 8)
 constexpr int PULL_CHUNK = 256;  // edge-centric pull: in-edge slots per wave task (4 per lane)
 constexpr int PULL_TASK = 512;   // static pull tasks: in-edges per wave (8 per lane in flight)

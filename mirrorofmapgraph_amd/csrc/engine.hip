@@ -64,6 +64,7 @@ static void free_workspace(Workspace &ws) {
     dfree(ws.pr_span);
     dfree(ws.pr_head);
     dfree(ws.pr_tail);
+    dfree(ws.trace);
     dfree(ws.pull_tasks);
     dfree(ws.dense_tasks);
     dfree(ws.slice_off);
@@ -90,7 +91,8 @@ int ensure_workspace(mg_graph *g) {
         (rc = dalloc(&ws.q[2], n)) || (rc = dalloc(&ws.hard, n)) || (rc = dalloc(&ws.visited, ws.nwords)) ||
         (rc = dalloc(&ws.bm[0], ws.nwords)) || (rc = dalloc(&ws.bm[1], ws.nwords)) ||
         (rc = dalloc(&ws.bm[2], ws.nwords)) || (rc = dalloc(&ws.chunks, ws.chunk_cap)) ||
-        (rc = dalloc(&ws.part, 3 * MAX_GRID)) || (rc = dalloc(&ws.ctl, 1))) {
+        (rc = dalloc(&ws.part, 3 * MAX_GRID)) || (rc = dalloc(&ws.ctl, 1)) ||
+        (rc = dalloc(&ws.trace, 2 * (size_t)TRACE_CAP))) {
         free_workspace(ws);
         return rc;
     }

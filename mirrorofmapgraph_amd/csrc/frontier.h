@@ -41,7 +41,35 @@ struct FrontierArgs {
     int64_t chunk_cap;
     LevelPart *part;     // per-workgroup partial counters, 2 * MAX_GRID slots
     int n, nwords;
+    long long *trace;    // per-level trace (TRACE_CAP levels), written by the level close
 };
+
+// Records level L's outcome: the next frontier's size and out-edges and the
+// direction the level ran in (one thread).
+__device__ __forceinline__ void trace_level(const FrontierArgs &fa, int L, int dir, long long nf, long long mf) {
+    if (fa.trace && L < TRACE_CAP) {
+        fa.trace[2 * L] = nf | ((long long)dir << 40);
+        fa.trace[2 * L + 1] = mf;
+    }
+}
+
+// Workgroup-scope access for single-workgroup loops: every reader and writer of the
+// data during the launch is in the one workgroup (one CU), so the CU's caches are
+// coherent enough and nothing needs the device-wide path (other kernels see the
+// results after the launch ends).
+__device__ __forceinline__ int ld_wg(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void min_wg(int *p, int v) {
+    __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ bool set_bit_first_wg(uint32_t *bm, int v) {
+    const uint32_t bit = 1u << (v & 31);
+    return !(__hip_atomic_fetch_or(&bm[v >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & bit);
+}
+__device__ __forceinline__ void clear_bit_wg(uint32_t *bm, int v) {
+    __hip_atomic_fetch_and(&bm[v >> 5], ~(1u << (v & 31)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 __device__ __forceinline__ int ld_dev(const int *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -471,6 +499,9 @@ int ensure_edge_dst(mg_graph *g, const int *d_in_off);
 // Builds the XCD-sliced pull structures for the in-edge view (d_in_off, d_in_adj);
 // ws.slice_ok reports whether every in-list is sorted (slicing needs it).
 int ensure_pull_slices(mg_graph *g, const int *d_in_off, const int *d_in_adj);
+// Prints the per-level trace of the search that just finished to stderr (the
+// reference's verbose per-iteration frontier sizes).
+int print_trace(mg_graph *g, const char *what, const char *dir0, const char *dir1);
 int finish_search(mg_graph *g, mg_stats *stats, const int *vals, int unreached, void *out, const void *result,
                   size_t bytes, bool out_on_device, const CallTimer &timer);
 

@@ -2,6 +2,8 @@
 #include <thread>
 #include <vector>
 
+#include <cstdio>
+
 #include "frontier.h"
 
 namespace mg {
@@ -64,6 +66,7 @@ FrontierArgs make_frontier_args(mg_graph *g, int directed) {
         fa.bm[i] = ws.bm[i];
         fa.q[i] = ws.q[i];
     }
+    fa.trace = ws.trace;
     fa.chunks = ws.chunks;
     fa.chunk_cap = ws.chunk_cap;
     fa.part = ws.part;
@@ -301,6 +304,23 @@ int launch_reached_stats(mg_graph *g, const int *vals, int unreached, mg_stats *
 // Records the closing event, fills *stats from the control block and copies
 // `bytes` of the device result `result` to `out`.  `vals` (nullable) is the int
 // result the TEPS count reads.
+int print_trace(mg_graph *g, const char *what, const char *dir0, const char *dir1) {
+    Workspace &ws = g->ws;
+    MG_HIP(hipMemcpy(ws.h_ctl, ws.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    const int levels = ws.h_ctl->level;
+    const int nl = std::min(levels, TRACE_CAP);
+    std::vector<long long> t(2 * (size_t)std::max(nl, 1));
+    if (nl) MG_HIP(hipMemcpy(t.data(), ws.trace, 16 * (size_t)nl, hipMemcpyDeviceToHost));
+    for (int L = 0; L < nl; ++L) {
+        const long long nf = t[2 * L] & ((1LL << 40) - 1);
+        const int dir = (int)(t[2 * L] >> 40);
+        std::fprintf(stderr, "%s level %d (%s): next frontier %lld vertices, %lld out-edges\n", what, L,
+                     dir ? dir1 : dir0, nf, t[2 * L + 1]);
+    }
+    if (levels > nl) std::fprintf(stderr, "%s: %d more levels not recorded\n", what, levels - nl);
+    return MG_OK;
+}
+
 int finish_search(mg_graph *g, mg_stats *stats, const int *vals, int unreached, void *out, const void *result,
                   size_t bytes, bool out_on_device, const CallTimer &timer) {
     Workspace &ws = g->ws;

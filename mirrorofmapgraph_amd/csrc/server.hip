@@ -136,7 +136,7 @@ static int engine_alloc(Server &s, Engine &e) {
         e.delta = sssp_delta(e.g, o);
         if ((rc = sssp_batch_reset(e.b, e.g, e.st, INT_MAX, e.delta))) return rc;
     }
-    if (e.kind != MG_Q_PPR) {
+    if (e.kind != MG_Q_PPR && graphs_enabled()) {
         MG_HIP(hipStreamBeginCapture(e.st, hipStreamCaptureModeThreadLocal));
         int src = MG_OK;
         for (int i = 0; i < engine_steps(e) && src == MG_OK; ++i)
@@ -285,8 +285,13 @@ static int engine_round(Server &s, Engine &e, bool *queued) {
             if ((rc = ppr_iterate(e.p, e.g, e.st, s.cfg.ppr_teleport, it))) return rc;
             for (Query *r : e.slot_q)
                 if (r) r->iters_left -= it;
-        } else {
+        } else if (e.chunk) {
             MG_HIP(hipGraphLaunch(e.chunk, e.st));
+        } else {
+            for (int i = 0; i < engine_steps(e); ++i)
+                if ((rc = e.kind == MG_Q_BFS ? bfs_batch_step(e.b, e.g, e.st) : sssp_batch_step(e.b, e.g, e.st, e.delta)))
+                    return rc;
+            if ((rc = batch_live(e.b, e.st))) return rc;
         }
     }
     if (e.kind != MG_Q_PPR) {

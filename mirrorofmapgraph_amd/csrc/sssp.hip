@@ -45,6 +45,10 @@ struct SsspArgs {
     // dense pull: a pull level whose frontier holds more than dense_pct % of the edges
     // reads every in-neighbour's distance without the frontier test (> 100: never)
     int dense_pct;
+    // buckets of the small-frontier loop: width, far list (n entries) and its bitmap
+    long long delta;
+    int *far;
+    uint32_t *farbm;
 };
 
 __device__ __forceinline__ long long edge_w(const SsspArgs &a, int view, int slot) {
@@ -55,14 +59,17 @@ __device__ __forceinline__ long long edge_w(const SsspArgs &a, int view, int slo
 struct SsspPush {
     const SsspArgs &a;
     long long nf = 0, mf = 0, ex = 0;
+    bool mark = true;  // false: report every improvement, the caller deduplicates
     __device__ explicit SsspPush(const SsspArgs &args) : a(args) {}
-    __device__ int load(int v) const { return a.dist[v]; }
+    // device-coherent: the small-frontier loop reads distances lowered earlier in the launch
+    __device__ int load(int v) const { return ld_dev(&a.dist[v]); }
     // u's distance dropped (to at most nd) this level: mark it once.  A stale
     // (larger) dist[u] read only causes a redundant relaxation: dist[u] dropped
     // this level either way, so u belongs to the next frontier.
     __device__ bool relax(long long nd, int du, int u, int L) {
         if (nd >= (long long)du) return false;
         atomicMin(&a.dist[u], (int)nd);
+        if (!mark) return true;
         if (!mark_next(a.fa, L, u)) return false;
         ++nf;
         mf += view_degree(a.fa, u);
@@ -71,7 +78,7 @@ struct SsspPush {
     __device__ bool visit(int view, int, int d, int slot, int u, bool valid, int L) {
         if (!valid) return false;
         ++ex;
-        return relax((long long)d + edge_w(a, view, slot), a.dist[u], u, L);
+        return relax((long long)d + edge_w(a, view, slot), ld_dev(&a.dist[u]), u, L);
     }
     // four targets, weights and target distances in flight together
     __device__ void visit_range(int view, int, int d, int beg, int end, int stride, int L) {
@@ -347,6 +354,7 @@ __device__ __forceinline__ void dense_pull(const SsspArgs &a, int L, DenseLds &l
 __device__ void sssp_close(const SsspArgs &a, int L, long long nf, long long mf) {
     Ctl *c = a.fa.ctl;
     int dir = c->dir;
+    trace_level(a.fa, L, dir, nf, mf);
     if (dir == DIR_PULL) c->bu_levels++;
     if (c->can_bu) {
         const long long m = c->m_unexplored;  // total edges (set at seed)
@@ -364,6 +372,7 @@ __global__ void sssp_seed(SsspArgs a, int src, int iter_limit, int can_pull, lon
     a.fa.bm[0][src >> 5] = 1u << (src & 31);
     a.fa.q[0][0] = src;
     ctl_reset(c, a.fa.n, iter_limit, 1);
+    c->thr = a.delta;  // the first bucket: [0, delta)
     c->mf_cur = view_degree(a.fa, src);
     // pull only pays on the densest levels: every in-edge is read and the frontier
     // ones gather dist[u]; push gathers only along frontier out-edges
@@ -373,10 +382,41 @@ __global__ void sssp_seed(SsspArgs a, int src, int iter_limit, int can_pull, lon
     c->m_unexplored = m;
 }
 
+// Closes a pending multi-workgroup level, then runs small push levels back to back.
+//
+// Narrow levels are latency-bound (a grid has thousands of them), so the loop keeps
+// the dependent chain per level short.  The frontier (at most SMALL_Q vertices and
+// SMALL_E edges), its distances, list starts and the prefix sums of its list lengths
+// live in LDS; the threads walk the concatenated lists, SMALL_EW edges each in flight,
+// finding each slot's owner by binary search over the prefix sums, so a level is one
+// round of neighbour/weight loads and one of distance loads whatever the degrees.  The
+// next frontier is de-duplicated in an LDS hash set and queued in LDS; the frontier
+// bitmaps and the control block are written only when the loop hands over.  Only this
+// one workgroup touches the distances while it runs, so loads and atomics take
+// workgroup scope (the CU's caches) rather than the device-wide path.
+//
+// Buckets (delta-stepping, Meyer and Sanders): inside the loop only vertices below the
+// threshold thr are relaxed; a vertex improved to thr or more waits in the far list
+// (ws.hard, deduplicated by the far bitmap ws.visited).  When no vertex below thr is
+// left, thr moves to the bucket of the smallest waiting distance and those below it
+// become the frontier.  Every improved vertex is relaxed again later, so the fixed
+// point is Bellman-Ford's; on a weighted grid this cuts the relaxations from ~75 per
+// edge to ~1.3.  When the loop hands over to the whole-GPU levels (which relax every
+// frontier vertex, unbucketed), the queue and the far list become the next bitmap.
+constexpr int SMALL_HASH = 2 * SMALL_Q;  // LDS hash set of a level's next frontier
+constexpr int SMALL_EW = 4;              // edges per thread in flight
+constexpr int SMALL_QPT = SMALL_Q / SMALL_BLK;  // queue entries per thread (contiguous)
 __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
     Ctl *c = a.fa.ctl;
     const FrontierArgs &fa = a.fa;
-    __shared__ int s_go, s_L, s_qv, s_fin, s_cnt;
+    __shared__ int s_q[SMALL_Q];             // next-frontier appends (and the entry frontier)
+    __shared__ int s_hash[SMALL_HASH];
+    __shared__ int s_pre[2 * SMALL_Q + 1];   // list l = view * nq + i: prefix of lengths
+    __shared__ int s_beg[2 * SMALL_Q];       // list starts
+    __shared__ int s_d[SMALL_Q];             // frontier distances
+    __shared__ int s_wave[SMALL_BLK / WAVE];
+    __shared__ int s_go, s_L, s_cnt, s_levels, s_limit, s_canpull, s_alpha, s_nfar, s_keep, s_qv, s_nq;
+    __shared__ long long s_m, s_thr;
     if (c->pending) {
         const int L = c->level;
         // pull levels: level_a (and level_c when sliced) slots; push: one slot set per view
@@ -399,50 +439,322 @@ __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
         }
         __syncthreads();
     }
-    for (;;) {
+    if (threadIdx.x == 0) {
+        const int L = c->level;
+        s_go = !c->done && c->dir == DIR_PUSH && c->qsize[L % 3] <= SMALL_Q && c->mf_cur <= SMALL_E;
+        s_L = L;
+        s_cnt = c->qsize[L % 3];
+        s_levels = 0;
+        s_limit = c->iter_limit;
+        s_canpull = c->can_bu;
+        s_alpha = c->alpha;
+        s_m = c->m_unexplored;
+        s_thr = c->thr;
+        s_nfar = 0;
+        s_qv = c->qvalid;
+    }
+    __syncthreads();
+    if (!s_go) return;
+    const long long delta = a.delta;
+    const int nviews = fa.nviews;
+    int *far = a.far;
+    uint32_t *farbm = a.farbm;
+    long long mf_over = 0;  // out-edges of next-frontier vertices that did not fit the queue
+    uint32_t *nx_over = nullptr;  // where such vertices are marked (the next bitmap)
+    auto clear_hash = [&]() {
+        for (int i = threadIdx.x; i < SMALL_HASH; i += SMALL_BLK) s_hash[i] = -1;
+    };
+    // u joins the next frontier once: LDS hash set (linear probing, up to 32 slots),
+    // then the queue.  A crowded set (only when the next frontier outgrows the queue,
+    // which ends the loop) falls back to the next bitmap; past the queue's capacity a
+    // vertex is marked in the next bitmap instead of queued.  A vertex let through
+    // twice is only relaxed twice.
+    // True for the one call that adds u to the next frontier's hash set (or, when the
+    // set is crowded, to the next bitmap).
+    auto claim = [&](int u) -> bool {
+        uint32_t h = ((uint32_t)u * 2654435761u) >> 19;  // 13 bits: SMALL_HASH slots
+        for (int probe = 0; probe < 32; ++probe, h = (h + 1) & (SMALL_HASH - 1)) {
+            const int old = atomicCAS(&s_hash[h], -1, u);
+            if (old == u) return false;  // already in
+            if (old == -1) return true;
+        }
+        return set_bit_first_wg(nx_over, u);
+    };
+    // Queues claimed vertices (every lane of the wave calls it; `want` selects), one
+    // LDS counter reservation per wave; past the queue's capacity a vertex is marked
+    // in the next bitmap instead.
+    auto enqueue = [&](bool want, int u) {
+        const uint64_t mask = wave_ballot(want);
+        if (!mask) return;
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        int base = 0;
+        if (lane_id() == leader) base = atomicAdd(&s_cnt, __popcll(mask));
+        base = __shfl(base, leader);
+        if (!want) return;
+        const int idx = base + lanes_below(mask);
+        if (idx < SMALL_Q) {
+            s_q[idx] = u;
+        } else {
+            atomicOr(&nx_over[u >> 5], 1u << (u & 31));
+            mf_over += view_degree(fa, u);
+        }
+    };
+    auto append = [&](int u) {
+        if (!claim(u)) return;
+        const int idx = atomicAdd(&s_cnt, 1);
+        if (idx < SMALL_Q) {
+            s_q[idx] = u;
+        } else {
+            atomicOr(&nx_over[u >> 5], 1u << (u & 31));
+            mf_over += view_degree(fa, u);
+        }
+    };
+    auto to_far = [&](int u) {
+        if (set_bit_first_wg(farbm, u)) far[atomicAdd(&s_nfar, 1)] = u;
+    };
+    // Moves the threshold to the bucket of the smallest waiting distance and the far
+    // vertices below it into the next frontier; far entries below the old threshold
+    // were relaxed as frontier vertices already and are dropped.
+    auto next_bucket = [&]() {
+        const long long old_thr = s_thr;
+        const int nfar = s_nfar;
+        long long lo = LLONG_MAX;
+        for (int i = threadIdx.x; i < nfar; i += SMALL_BLK) {
+            const long long d = ld_wg(&a.dist[far[i]]);
+            if (d >= old_thr) lo = d < lo ? d : lo;
+        }
+        lo = block_min<SMALL_BLK>(lo);
+        const long long thr = lo == LLONG_MAX ? old_thr : (lo / delta + 1) * delta;
+        // stable in-place compaction of the kept entries, SMALL_BLK at a time
+        int kept = 0;
+        for (int b = 0; b < nfar; b += SMALL_BLK) {
+            const int i = b + (int)threadIdx.x;
+            const int u = i < nfar ? far[i] : -1;
+            bool keep = false;
+            if (u >= 0) {
+                const long long d = ld_wg(&a.dist[u]);
+                keep = d >= thr;
+                if (!keep) {
+                    clear_bit_wg(farbm, u);
+                    if (d >= old_thr) append(u);
+                }
+            }
+            int total;
+            const int ex = block_exclusive_scan<SMALL_BLK>(keep ? 1 : 0, s_wave, &total);
+            if (keep) far[kept + ex] = u;  // kept + ex <= i: no unread entry is overwritten
+            kept += total;
+            __syncthreads();
+        }
         if (threadIdx.x == 0) {
-            const int L = c->level;
-            s_go = !c->done && c->dir == DIR_PUSH && c->qsize[L % 3] <= SMALL_Q && c->mf_cur <= SMALL_E;
-            s_L = L;
-            s_qv = c->qvalid;
-            s_fin = c->qsize[L % 3];
-            s_cnt = 0;
+            s_thr = thr;
+            s_nfar = kept;
         }
         __syncthreads();
-        if (!s_go) break;
-        const int L = s_L, cur = L % 3, prv = (L + 2) % 3;
-        if (!(s_qv & (1 << cur))) {
-            queue_from_bitmap<SMALL_BLK>(fa, L, &s_cnt);
-            if (threadIdx.x == 0) {
-                s_fin = s_cnt;
-                s_cnt = 0;
+    };
+    // The queued next frontier s_q[0, nq) becomes the frontier: distances, list starts
+    // and length prefix sums into LDS.  Returns this thread's share of its out-edges.
+    auto build = [&](int nq) -> long long {
+        int v[SMALL_QPT], len[2][SMALL_QPT];
+        long long mf = 0;
+#pragma unroll
+        for (int k = 0; k < SMALL_QPT; ++k) {
+            const int i = SMALL_QPT * (int)threadIdx.x + k;
+            v[k] = i < nq ? s_q[i] : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < SMALL_QPT; ++k) {
+            const int i = SMALL_QPT * (int)threadIdx.x + k;
+            len[0][k] = len[1][k] = 0;
+            if (v[k] < 0) continue;
+            s_d[i] = ld_wg(&a.dist[v[k]]);
+            for (int view = 0; view < nviews; ++view) {
+                const int b = fa.off[view][v[k]];
+                s_beg[view * nq + i] = b;
+                len[view][k] = fa.off[view][v[k] + 1] - b;
+                mf += len[view][k];
             }
         }
-        if (s_qv & (1 << prv)) {
+        int base = 0;
+        for (int view = 0; view < nviews; ++view) {
+            int mine = 0;
+#pragma unroll
+            for (int k = 0; k < SMALL_QPT; ++k) mine += len[view][k];
+            int total;
+            int ex = block_exclusive_scan<SMALL_BLK>(mine, s_wave, &total) + base;
+#pragma unroll
+            for (int k = 0; k < SMALL_QPT; ++k) {
+                const int i = SMALL_QPT * (int)threadIdx.x + k;
+                if (i < nq) s_pre[view * nq + i] = ex;
+                ex += len[view][k];
+            }
+            base += total;
+        }
+        if (threadIdx.x == 0) {
+            s_pre[nviews * nq] = base;
+            s_nq = nq;
+        }
+        return mf;
+    };
+    {
+        const int L = s_L, cur = L % 3, prv = (L + 2) % 3;
+        const int qv = s_qv;
+        int n0 = s_cnt;
+        __syncthreads();
+        if (qv & (1 << cur)) {
+            for (int i = threadIdx.x; i < n0; i += SMALL_BLK) s_q[i] = fa.q[cur][i];
+        } else {
+            if (threadIdx.x == 0) s_cnt = 0;
+            __syncthreads();
+            queue_from_bitmap_to<SMALL_BLK>(fa, L, s_q, &s_cnt, SMALL_Q);
+            __syncthreads();
+            n0 = min(s_cnt, SMALL_Q);
+        }
+        if (qv & (1 << prv)) {
             const int *qp = fa.q[prv];
             for (int i = threadIdx.x; i < c->qsize[prv]; i += SMALL_BLK) fa.bm[prv][qp[i] >> 5] = 0;
         } else {
             clear_prev_sweep(fa, L, threadIdx.x, SMALL_BLK);
         }
+        clear_hash();
         __syncthreads();
-        SsspPush pol(a);
-        int *qout = fa.q[(L + 1) % 3];
-        auto emit = [&](int u, bool newly) { wave_append(newly, u, qout, &s_cnt); };
-        const int *qin = fa.q[cur];
-        for (int view = 0; view < fa.nviews; ++view)
-            for (int t0 = 0; t0 < s_fin; t0 += SMALL_BLK) {
-                const int i = t0 + (int)threadIdx.x;
-                expand_tile<SMALL_BLK, false>(fa, pol, view, L, i < s_fin ? qin[i] : -1, emit);
-            }
-        const long long mf = block_sum<SMALL_BLK>(pol.mf);
-        const long long ex = block_sum<SMALL_BLK>(pol.ex);
-        if (threadIdx.x == 0) {
-            c->small_levels++;
-            c->edges_examined += ex;
-            sssp_close(a, L, s_cnt, mf);
-            c->qvalid = (1 << cur) | (1 << ((L + 1) % 3));
+        // the frontier leaves the bitmap for the queue (every bitmap is clean while the
+        // loop runs); vertices at or above the threshold wait in the far list
+        int vs[SMALL_QPT];
+#pragma unroll
+        for (int k = 0; k < SMALL_QPT; ++k) {
+            const int i = (int)threadIdx.x + k * SMALL_BLK;
+            vs[k] = i < n0 ? s_q[i] : -1;
         }
         __syncthreads();
+        if (threadIdx.x == 0) s_cnt = 0;
+        __syncthreads();
+        nx_over = fa.bm[cur];  // cannot overflow: at most n0 <= SMALL_Q entries
+#pragma unroll
+        for (int k = 0; k < SMALL_QPT; ++k) {
+            if (vs[k] < 0) continue;
+            fa.bm[cur][vs[k] >> 5] = 0;
+            if (ld_wg(&a.dist[vs[k]]) >= s_thr) to_far(vs[k]);
+            else append(vs[k]);
+        }
+        __syncthreads();
+        if (s_cnt == 0 && s_nfar > 0) next_bucket();
+    }
+    build(min(s_cnt, SMALL_Q));
+    clear_hash();
+    __syncthreads();  // the frontier is in s_d / s_beg / s_pre: the queue is free
+    long long ex_all = 0;
+    for (;;) {
+        const int L = s_L;
+        nx_over = fa.bm[(L + 1) % 3];
+        const long long thr = s_thr;
+        const int nq = s_nq, nl = nviews * nq, T = s_pre[nviews * nq];
+        if (threadIdx.x == 0) s_cnt = 0;
+        __syncthreads();
+        for (int p0 = 0; p0 < T; p0 += SMALL_EW * SMALL_BLK) {
+            int p[SMALL_EW], li[SMALL_EW];
+#pragma unroll
+            for (int k = 0; k < SMALL_EW; ++k) {
+                p[k] = p0 + k * SMALL_BLK + (int)threadIdx.x;
+                li[k] = 0;
+            }
+            // owner list = the last one whose prefix is <= p (all lanes' searches in step)
+            for (int st = 1 << (31 - __builtin_clz(max(nl, 1))); st > 0; st >>= 1) {
+#pragma unroll
+                for (int k = 0; k < SMALL_EW; ++k)
+                    if (li[k] + st < nl && s_pre[li[k] + st] <= p[k]) li[k] += st;
+            }
+            int sl[SMALL_EW], u[SMALL_EW], wt[SMALL_EW], du[SMALL_EW], dv[SMALL_EW];
+            bool ok[SMALL_EW];
+#pragma unroll
+            for (int k = 0; k < SMALL_EW; ++k) {
+                ok[k] = p[k] < T;
+                const int view = li[k] >= nq;  // view-major lists
+                sl[k] = ok[k] ? s_beg[li[k]] + (p[k] - s_pre[li[k]]) : 0;  // slot 0 exists: T > 0
+                dv[k] = s_d[li[k] - (view ? nq : 0)];
+                u[k] = (view ? fa.adj[1] : fa.adj[0])[sl[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < SMALL_EW; ++k) {
+                const int *eid = li[k] >= nq ? fa.eid[1] : fa.eid[0];
+                wt[k] = a.w[eid ? eid[sl[k]] : sl[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < SMALL_EW; ++k) du[k] = ld_wg(&a.dist[u[k]]);
+#pragma unroll
+            for (int k = 0; k < SMALL_EW; ++k) {
+                bool add = false;
+                if (ok[k]) {
+                    ++ex_all;
+                    const long long nd = (long long)dv[k] + wt[k];
+                    if (nd < du[k]) {
+                        min_wg(&a.dist[u[k]], (int)nd);
+                        if (nd >= thr) to_far(u[k]);
+                        else add = claim(u[k]);
+                    }
+                }
+                enqueue(add, u[k]);
+            }
+        }
+        __syncthreads();
+        if (s_cnt == 0 && s_nfar > 0) next_bucket();
+        const int nf = s_cnt;
+        long long mf = mf_over;
+        mf_over = 0;
+        if (nf <= SMALL_Q) mf += build(nf);  // the next frontier's lists (reads s_q: before any append)
+        clear_hash();
+        mf = block_sum<SMALL_BLK>(mf);
+        if (threadIdx.x == 0) {
+            // close level L on the LDS copy of the direction state (as sssp_close)
+            trace_level(fa, L, DIR_PUSH, nf, mf);
+            s_levels++;
+            s_go = nf > 0 && nf <= SMALL_Q && !(s_canpull && mf * s_alpha > s_m) && mf <= SMALL_E &&
+                   L + 1 < s_limit;
+            s_L = L + 1;
+            s_keep = nf;
+        }
+        __syncthreads();
+        if (s_go) continue;
+        // hand over: the queue and the far list become the next bitmap (the whole-GPU
+        // levels relax every frontier vertex), then the control block takes the state
+        uint32_t *nx = fa.bm[(L + 1) % 3];
+        if (nf <= SMALL_Q) {
+            // build() does not run past the queue's capacity; the overflowing vertices'
+            // degrees were summed into mf_over as they were marked
+            for (int i = threadIdx.x; i < nf; i += SMALL_BLK) atomicOr(&nx[s_q[i] >> 5], 1u << (s_q[i] & 31));
+        } else {
+            for (int i = threadIdx.x; i < SMALL_Q; i += SMALL_BLK) {
+                atomicOr(&nx[s_q[i] >> 5], 1u << (s_q[i] & 31));
+            }
+        }
+        __syncthreads();
+        long long nf_far = 0, mf_far = 0, mf_q = 0;
+        if (nf > SMALL_Q)  // out-edges of the queued part (build() skipped)
+            for (int i = threadIdx.x; i < SMALL_Q; i += SMALL_BLK) mf_q += view_degree(fa, s_q[i]);
+        const int nfar = s_nfar;
+        for (int i = threadIdx.x; i < nfar; i += SMALL_BLK) {
+            const int u = far[i];
+            clear_bit_wg(farbm, u);
+            if (set_bit_first(nx, u)) {
+                ++nf_far;
+                mf_far += view_degree(fa, u);
+            }
+        }
+        nf_far = block_sum<SMALL_BLK>(nf_far);
+        mf_far = block_sum<SMALL_BLK>(mf_far);
+        mf_q = block_sum<SMALL_BLK>(mf_q);
+        const long long ex = block_sum<SMALL_BLK>(ex_all);
+        if (threadIdx.x == 0) {
+            const long long nft = s_keep + nf_far, mft = mf + mf_q + mf_far;
+            const int dir = s_canpull && mft * s_alpha > s_m ? DIR_PULL : DIR_PUSH;
+            c->edges_examined += ex;
+            c->dir = dir;
+            c->dense = dir == DIR_PULL && mft * 100 > s_m * a.dense_pct;
+            c->small_levels += s_levels;
+            c->thr = s_thr;
+            advance_level(c, L, nft, mft);
+            c->qvalid = 0;  // the queue stayed in LDS
+        }
+        break;
     }
 }
 
@@ -643,6 +955,14 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
     a.w = g->w;
     a.G = std::max(PULL_SLICES, grid_blocks(std::max<int64_t>(g->n / 4, g->m / 64), BLK) / PULL_SLICES * PULL_SLICES);
     a.dense_pct = sssp_dense_pct();
+    // bucket width of the small-frontier loop: the option, else four maximum edge weights
+    // (a frontier vertex's improvements then stay within the next few buckets; wider
+    // buckets mean fewer, fuller rounds); sssp_delta < 0: unbucketed
+    a.delta = o.sssp_delta < 0 ? LLONG_MAX / 4
+                               : o.sssp_delta > 0 ? (long long)o.sssp_delta
+                                                  : std::max<long long>(1, 4LL * g->max_weight);
+    a.far = ws.hard;
+    a.farbm = ws.visited;
     const int can_pull = a.fa.off[2] != nullptr && a.fa.nviews == 1 && o.engine != MG_ENGINE_GAS;
     if (can_pull) {
         if ((rc = ensure_pull_tasks(g, a.fa.off[2]))) return rc;
@@ -669,6 +989,7 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
     if ((rc = launch_fill(g, ws.vals, INT_MAX, g->n))) return rc;
     for (int i = 0; i < 3; ++i)
         if ((rc = launch_fill(g, ws.bm[i], 0u, ws.nwords))) return rc;
+    if ((rc = launch_fill(g, ws.visited, 0u, ws.nwords))) return rc;  // far bitmap
     hipLaunchKernelGGL(sssp_seed, dim3(1), dim3(1), 0, g->stream, a, src, o.iter_num, can_pull, (long long)g->m);
     MG_HIP(hipGetLastError());
     auto step = [&]() -> int {
@@ -682,7 +1003,9 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
     };
     // Bellman-Ford with non-negative weights settles within n rounds.
     const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n + 1) + 3;
-    if ((rc = drive(g, step, max_steps, KIND_SSSP, (uint64_t)o.directed * 4 + (uint64_t)a.sliced * 2 + (uint64_t)(a.dpull != nullptr))))
+    if ((rc = drive(g, step, max_steps, KIND_SSSP, (uint64_t)o.directed * 4 + (uint64_t)a.sliced * 2 + (uint64_t)(a.dpull != nullptr) + ((uint64_t)a.delta << 3))))
         return rc;
-    return finish_search(g, stats, ws.vals, INT_MAX, dist_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
+    rc = finish_search(g, stats, ws.vals, INT_MAX, dist_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
+    if (rc == MG_OK && o.verbose) rc = print_trace(g, "sssp", "push", "pull");
+    return rc;
 }

@@ -116,6 +116,7 @@ int main(int argc, char **argv) {
     o.threshold = cfg.getParameter<int>("threshold");
     o.directed = cfg.getParameter<int>("directed");
     o.max_queue_sizing = cfg.getParameter<double>("max_queue_sizing");
+    o.verbose = cfg.getParameter<int>("verbose");
 
     // sources (bfs.cu:340-401)
     std::vector<int32_t> srcs;

@@ -41,3 +41,20 @@ def test_driver_random_sources_batch(mg, oracle, device, tmp_path):
     g = load_mtx(mg, "chesapeake.mtx")
     last = int(mg.pick_sources(g, 300, seed=5)[-1])
     assert np.array_equal(d, oracle.bfs(g.row_offsets, g.col_idx, last))
+
+
+def test_driver_verbose_level_trace(mg, oracle, device, tmp_path):
+    """verbose=1 prints one line per level (the reference's per-iteration frontier
+    sizes); for BFS the next frontier of level L is the set of vertices at depth L+1."""
+    out = tmp_path / "out.txt"
+    r = subprocess.run([str(DRIVER), "bfs", "-g", str(GOLDEN / "chesapeake.mtx"), "-p", "src=1,verbose=1", "-o",
+                        str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = [ln for ln in r.stderr.splitlines() if ln.startswith("bfs level ")]
+    g = load_mtx(mg, "chesapeake.mtx")
+    depth = oracle.bfs(g.row_offsets, g.col_idx, 0)
+    levels = int(depth.max()) + 1  # the last level finds nothing
+    assert len(lines) == levels
+    for L, ln in enumerate(lines):
+        nf = int(ln.split("next frontier ")[1].split()[0])
+        assert nf == int((depth == L + 1).sum()), ln
