@@ -51,7 +51,7 @@ struct PrArgs {
     int *pact;
     double *partials;    // per-workgroup L1 change
     LevelPart *cpart;    // per-workgroup count of changed vertices (nf)
-    int n, nwords, nmid, nchunks, nbig, G1, G2;
+    int n, nwords, nmid, nchunks, nbig, G1, G2, G3;  // G3: workgroups of pr_k3b (0: not run)
     int mode;
     float base, damp, eps;
     double tol_n;
@@ -244,39 +244,53 @@ __global__ __launch_bounds__(BLK) void pr_k2(PrArgs a) {
     pr_write_part<BLK>(a, a.G1 + blockIdx.x, delta, nchanged);
 }
 
-// One workgroup: combines the chunk sums of high in-degree vertices (in chunk
-// order), then closes the iteration from the per-workgroup partials.
-__global__ __launch_bounds__(SMALL_BLK) void pr_k3(PrArgs a) {
+// High in-degree vertices, a thread each: their chunk sums added in chunk order
+// (eight loads in flight), then applied.  Partials go to slots [G1 + G2, + G3).
+__global__ __launch_bounds__(BLK) void pr_k3b(PrArgs a) {
     Ctl *c = a.ctl;
     if (c->done) return;
     const PrIter I = pr_iter(a);
     const bool all = a.mode == MG_PR_ALL || I.it == 0;
     double delta = 0;
     int nchanged = 0;
-    for (int k = threadIdx.x; k < (a.sliced || a.edge ? 0 : a.nbig); k += SMALL_BLK) {
+    for (int k = blockIdx.x * BLK + (int)threadIdx.x; k < a.nbig; k += gridDim.x * BLK) {
         const int c0 = a.bigv[k], c1 = a.bigv[k + 1];
         const int v = a.chunks[c0].v;
         float s = a.part[c0];
         bool act = a.pact[c0] != 0;
-        for (int ci = c0 + 1; ci < c1; ci += 4) {  // chunk sums in order, four loads in flight
-            float p[4];
-            int q[4];
+        int ci = c0 + 1;
+        for (; ci + 8 <= c1; ci += 8) {
+            float p[8];
+            int q[8];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                p[j] = ci + j < c1 ? a.part[ci + j] : 0.0f;
-                q[j] = ci + j < c1 ? a.pact[ci + j] : 0;
+            for (int j = 0; j < 8; ++j) {
+                p[j] = a.part[ci + j];
+                q[j] = a.pact[ci + j];
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (ci + j < c1) {
-                    s = s + p[j];
-                    act = act || q[j] != 0;
-                }
+            for (int j = 0; j < 8; ++j) {
+                s = s + p[j];
+                act = act || q[j] != 0;
+            }
+        }
+        for (; ci < c1; ++ci) {
+            s = s + a.part[ci];
+            act = act || a.pact[ci] != 0;
         }
         delta += pr_apply(a, I, v, s, act || all, nchanged);
     }
+    pr_write_part<BLK>(a, a.G1 + a.G2 + blockIdx.x, delta, nchanged);
+}
+
+// One workgroup: closes the iteration from the per-workgroup partials.
+__global__ __launch_bounds__(SMALL_BLK) void pr_k3(PrArgs a) {
+    Ctl *c = a.ctl;
+    if (c->done) return;
+    const PrIter I = pr_iter(a);
+    double delta = 0;
+    int nchanged = 0;
     long long nch = nchanged;
-    const int np = a.G1 + a.G2;
+    const int np = a.G1 + a.G2 + a.G3;
     for (int i = threadIdx.x; i < np; i += SMALL_BLK) {
         delta += a.partials[i];
         nch += a.cpart[i].nf;
@@ -687,6 +701,9 @@ extern "C" int mg_pagerank(mg_graph *g, float *rank_out, const mg_options *opt, 
         }
     }
 
+    // high in-degree vertices' chunk sums (the default gather only)
+    a.G3 = !a.edge && !a.sliced && a.nbig > 0 ? grid_blocks(a.nbig, BLK) : 0;
+
     MG_HIP(hipEventRecord(g->ev0, g->stream));
     hipLaunchKernelGGL(pr_init, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, g->stream, a, o.iter_num);
     MG_HIP(hipGetLastError());
@@ -704,6 +721,7 @@ extern "C" int mg_pagerank(mg_graph *g, float *rank_out, const mg_options *opt, 
             hipLaunchKernelGGL(pr_k1, dim3(a.G1), dim3(BLK), 0, g->stream, a);
             hipLaunchKernelGGL(pr_k2, dim3(a.G2), dim3(BLK), 0, g->stream, a);
         }
+        if (a.G3) hipLaunchKernelGGL(pr_k3b, dim3(a.G3), dim3(BLK), 0, g->stream, a);
         hipLaunchKernelGGL(pr_k3, dim3(1), dim3(SMALL_BLK), 0, g->stream, a);
         MG_HIP(hipGetLastError());
         return MG_OK;
