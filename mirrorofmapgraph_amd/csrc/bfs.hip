@@ -117,9 +117,10 @@ __global__ void bfs_seed(BfsArgs a, int src, int iter_limit, int can_bu, int alp
 // LDS (the control block is written only when the loop exits), the frontier queue
 // lives in LDS, each thread keeps its vertices of the next level and their edge
 // offsets in registers (loading them also gives the next frontier's out-edges for
-// the direction test), visits claim the visited bit with one atomic (no pre-test),
-// and a frontier's bitmap words are cleared as its vertices are expanded.  Vertices
-// with more than LANE_T edges go through the LDS-scan tile path.
+// the direction test), visits claim the visited bit with one atomic (no pre-test;
+// workgroup scope: only this workgroup touches the bitmaps during the launch), and a
+// frontier's bitmap words are cleared as its vertices are expanded.  Vertices with
+// more than LANE_T edges go through the LDS-scan tile path.
 
 __global__ __launch_bounds__(SMALL_BLK) void bfs_small(BfsArgs a) {
     Ctl *c = a.fa.ctl;
@@ -250,13 +251,17 @@ __global__ __launch_bounds__(SMALL_BLK) void bfs_small(BfsArgs a) {
 #pragma unroll
                     for (int j = 0; j < 4; ++j) u[j] = p + j < e ? adj[p + j] : -1;
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) old[j] = u[j] >= 0 ? atomicOr(&a.visited[u[j] >> 5], 1u << (u[j] & 31)) : ~0u;
+                    for (int j = 0; j < 4; ++j)
+                        old[j] = u[j] >= 0 ? __hip_atomic_fetch_or(&a.visited[u[j] >> 5], 1u << (u[j] & 31),
+                                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                                           : ~0u;
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const bool newly = u[j] >= 0 && !((old[j] >> (u[j] & 31)) & 1u);
                         if (newly) {
                             a.depth[u[j]] = L + 1;
-                            atomicOr(&nx[u[j] >> 5], 1u << (u[j] & 31));
+                            __hip_atomic_fetch_or(&nx[u[j] >> 5], 1u << (u[j] & 31), __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
                             const int idx = atomicAdd(&s_cnt, 1);
                             if (idx < SMALL_Q) s_q[idx] = u[j];
                             else mf_over += view_degree(fa, u[j]);

@@ -301,9 +301,6 @@ int launch_reached_stats(mg_graph *g, const int *vals, int unreached, mg_stats *
     return MG_OK;
 }
 
-// Records the closing event, fills *stats from the control block and copies
-// `bytes` of the device result `result` to `out`.  `vals` (nullable) is the int
-// result the TEPS count reads.
 int print_trace(mg_graph *g, const char *what, const char *dir0, const char *dir1) {
     Workspace &ws = g->ws;
     MG_HIP(hipMemcpy(ws.h_ctl, ws.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
@@ -321,6 +318,9 @@ int print_trace(mg_graph *g, const char *what, const char *dir0, const char *dir
     return MG_OK;
 }
 
+// Records the closing event, fills *stats from the control block and copies
+// `bytes` of the device result `result` to `out`.  `vals` (nullable) is the int
+// result the TEPS count reads.
 int finish_search(mg_graph *g, mg_stats *stats, const int *vals, int unreached, void *out, const void *result,
                   size_t bytes, bool out_on_device, const CallTimer &timer) {
     Workspace &ws = g->ws;

@@ -123,6 +123,22 @@ def test_grid_bfs_sssp(mg, oracle, device):
         assert np.array_equal(dg.cc(), oracle.cc(g.row_offsets, g.col_idx))
 
 
+@pytest.mark.parametrize("delta", [0, 1, 5, 64, -1])
+def test_sssp_bucket_widths(mg, oracle, device, delta):
+    """Every bucket width of the narrow-frontier loop (0 = auto, 1 = Dijkstra-like
+    single-distance buckets, -1 = unbucketed Bellman-Ford) gives the oracle's
+    distances, on a grid (all levels narrow) and on R-MAT (narrow and whole-GPU levels
+    mixed, so the far list is handed over)."""
+    grid = mg.grid_graph(128, seed=9, weight_max=16)
+    rm = mg.rmat_graph(14, 8, seed=11, weight_max=64)
+    for g, src in ((grid, 0), (grid, 128 * 64 + 64), (rm, 0), (rm, int(mg.pick_sources(rm, 1, 3)[0]))):
+        with mg.Graph(g, symmetric=True) as dg:
+            got, st = dg.sssp(src, mg.options(sssp_delta=delta), stats=True)
+        exp = oracle.sssp(g.row_offsets, g.col_idx, g.values, src)
+        assert np.array_equal(got, exp), (delta, src, np.nonzero(got != exp)[0][:10])
+        assert st.edges_traversed == oracle.teps_edges(g.row_offsets, exp, INT_MAX)
+
+
 # ---------------------------------------------------------------- limits and errors
 def test_iter_num_limits_levels(mg, oracle, device, s16):
     with mg.Graph(s16, symmetric=True) as dg:
