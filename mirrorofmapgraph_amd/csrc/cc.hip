@@ -94,40 +94,52 @@ __global__ void cc_compress(FrontierArgs fa, int *p) {
 }
 
 // Afforest step 2: most frequent root among CC_SAMPLES hashed vertices (one
-// workgroup; bitonic sort in LDS, then the longest run).
+// workgroup; roots counted in an LDS hash table, then the largest count, ties to the
+// smaller root).  Which component is skipped only decides the work of step 3, not the
+// labels.
 __global__ __launch_bounds__(CC_SAMPLES) void cc_sample(FrontierArgs fa, const int *p, uint64_t seed) {
-    __shared__ int s[CC_SAMPLES];
+    constexpr int H = 2 * CC_SAMPLES;
+    __shared__ int s_key[H];
+    __shared__ int s_cnt[H];
+    __shared__ unsigned long long s_best[CC_SAMPLES / WAVE];
     const int t = threadIdx.x;
+    for (int i = t; i < H; i += CC_SAMPLES) {
+        s_key[i] = -1;
+        s_cnt[i] = 0;
+    }
     uint64_t h = (seed + (uint64_t)t * 0x9e3779b97f4a7c15ULL);
     h = (h ^ (h >> 30)) * 0xbf58476d1ce4e5b9ULL;
     h = (h ^ (h >> 27)) * 0x94d049bb133111ebULL;
     h ^= h >> 31;
-    s[t] = p[(int)(h % (uint64_t)fa.n)];
+    const int r = p[(int)(h % (uint64_t)fa.n)];
     __syncthreads();
-    for (int k = 2; k <= CC_SAMPLES; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            const int i = t ^ j;
-            if (i > t) {
-                const int a = s[t], b = s[i];
-                const bool up = (t & k) == 0;
-                if ((a > b) == up) {
-                    s[t] = b;
-                    s[i] = a;
-                }
-            }
-            __syncthreads();
+    // insert r (linear probing; at most CC_SAMPLES distinct keys in 2 * CC_SAMPLES slots)
+    uint32_t slot = ((uint32_t)r * 2654435761u) % H;
+    for (;;) {
+        const int old = atomicCAS(&s_key[slot], -1, r);
+        if (old == -1 || old == r) break;
+        slot = (slot + 1) % H;
+    }
+    atomicAdd(&s_cnt[slot], 1);
+    __syncthreads();
+    // the largest (count, -root) over the slots
+    unsigned long long best = 0;
+    for (int i = t; i < H; i += CC_SAMPLES)
+        if (s_cnt[i] > 0) {
+            const unsigned long long k = ((unsigned long long)s_cnt[i] << 32) | (0xFFFFFFFFu - (uint32_t)s_key[i]);
+            best = k > best ? k : best;
         }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long y = __shfl_xor(best, o);
+        best = y > best ? y : best;
+    }
+    if ((t & (WAVE - 1)) == 0) s_best[t / WAVE] = best;
+    __syncthreads();
     if (t == 0) {
-        int best = s[0], len = 0, run = 0;
-        for (int i = 0; i < CC_SAMPLES; ++i) {
-            run = (i > 0 && s[i] == s[i - 1]) ? run + 1 : 1;
-            if (run > len) {
-                len = run;
-                best = s[i];
-            }
-        }
-        fa.ctl->aux = best;  // the skipped component's root
-        fa.ctl->nf_prev = len;
+        for (int w = 1; w < CC_SAMPLES / WAVE; ++w) best = s_best[w] > best ? s_best[w] : best;
+        fa.ctl->aux = (int)(0xFFFFFFFFu - (uint32_t)(best & 0xFFFFFFFFu));  // the skipped component's root
+        fa.ctl->nf_prev = (int)(best >> 32);
     }
 }
 
