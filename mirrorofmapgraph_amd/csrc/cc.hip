@@ -28,13 +28,17 @@ namespace mg {
 constexpr int CC_SAMPLE_ROUNDS = 2;
 constexpr int CC_SAMPLES = 1024;
 
+// Parent loads may return a stale value (the caches of this CU/XCD): parents only
+// ever move to ancestors, so a stale parent is still an ancestor, a stale "root" is
+// caught by the hooking compare-and-swap (device scope), and two vertices seen under
+// one root really are in one component.  The near caches are cheaper than the
+// device-coherent path, which costs more than the extra retries.
 __device__ __forceinline__ int ld_parent(const int *p, int x) {
-    return __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Root of x with path halving (parents only move to ancestors, so concurrent
-// halving keeps every parent[] entry pointing to an ancestor).  Loads are
-// device-coherent: other workgroups hook roots during the same kernel.
+// halving keeps every parent[] entry pointing to an ancestor).
 __device__ __forceinline__ int uf_find(int *p, int x) {
     for (;;) {
         const int px = ld_parent(p, x);
