@@ -104,6 +104,9 @@ struct Workspace {
     double *partials = nullptr;                       // per-workgroup L1 partial sums
     int32_t *pr_mid = nullptr;                        // vertices of medium in-degree (a wave each)
     ChunkRec *pr_chunks = nullptr;                    // in-edge chunks of high in-degree vertices
+    double *pr_partials = nullptr;                    // per-workgroup partials of the PageRank gather
+    LevelPart *pr_cpart = nullptr;
+    int pr_nslots = 0;
     int32_t *pr_bigv = nullptr;                       // first chunk of each high in-degree vertex (+ end)
     float *pr_part = nullptr;                         // per-chunk partial sums
     int32_t *pr_pact = nullptr;                       // per-chunk "an in-neighbour changed" flags
@@ -167,6 +170,8 @@ int ensure_pr_workspace(mg_graph *g);
 int alloc_i32(int32_t **p, size_t n);
 int alloc_f32(float **p, size_t n);
 int alloc_chunks(ChunkRec **p, size_t n);
+int alloc_f64(double **p, size_t n);
+int alloc_parts(LevelPart **p, size_t n);
 void free_dev(void *p);
 int grid_blocks(int64_t work_items, int block);
 

@@ -58,6 +58,8 @@ static void free_workspace(Workspace &ws) {
     dfree(ws.partials);
     dfree(ws.pr_mid);
     dfree(ws.pr_chunks);
+    dfree(ws.pr_partials);
+    dfree(ws.pr_cpart);
     dfree(ws.pr_part);
     dfree(ws.pr_pact);
     dfree(ws.pr_bigv);
@@ -118,6 +120,8 @@ int ensure_pr_workspace(mg_graph *g) {
 int alloc_i32(int32_t **p, size_t n) { return dalloc(p, n); }
 int alloc_f32(float **p, size_t n) { return dalloc(p, n); }
 int alloc_chunks(ChunkRec **p, size_t n) { return dalloc(p, n); }
+int alloc_f64(double **p, size_t n) { return dalloc(p, n); }
+int alloc_parts(LevelPart **p, size_t n) { return dalloc(p, n); }
 void free_dev(void *p) {
     if (p) (void)hipFree(p);
 }

@@ -37,6 +37,7 @@ namespace mg {
 
 constexpr int PR_T1 = 32;
 constexpr int PR_T2 = 1024;
+constexpr int PR_MAX_GRID = 32768;  // workgroups of pr_k1 / pr_k2
 
 struct PrArgs {
     Ctl *ctl;
@@ -703,6 +704,27 @@ extern "C" int mg_pagerank(mg_graph *g, float *rank_out, const mg_options *opt, 
 
     // high in-degree vertices' chunk sums (the default gather only)
     a.G3 = !a.edge && !a.sliced && a.nbig > 0 ? grid_blocks(a.nbig, BLK) : 0;
+    if (!a.edge && !a.sliced) {
+        // The default gather is latency-bound: with a grid-stride loop each thread walks
+        // several vertices (pr_k1) or each wave several lists (pr_k2) one after another.
+        // Up to PR_MAX_GRID workgroups give every vertex / list its own thread / wave and
+        // let the hardware overlap them; the partial slots grow to match.
+        auto blocks = [](int64_t items) { return (int)std::max<int64_t>(1, std::min<int64_t>((items + BLK - 1) / BLK, PR_MAX_GRID)); };
+        a.G1 = blocks(g->n);
+        a.G2 = blocks((int64_t)(a.nmid + a.nchunks) * WAVE);
+        const int np = a.G1 + a.G2 + a.G3;
+        if (ws.pr_nslots < np) {
+            free_dev(ws.pr_partials);
+            free_dev(ws.pr_cpart);
+            ws.pr_partials = nullptr;
+            ws.pr_cpart = nullptr;
+            ws.pr_nslots = 0;
+            if ((rc = alloc_f64(&ws.pr_partials, np)) || (rc = alloc_parts(&ws.pr_cpart, np))) return rc;
+            ws.pr_nslots = np;
+        }
+        a.partials = ws.pr_partials;
+        a.cpart = ws.pr_cpart;
+    }
 
     MG_HIP(hipEventRecord(g->ev0, g->stream));
     hipLaunchKernelGGL(pr_init, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, g->stream, a, o.iter_num);
