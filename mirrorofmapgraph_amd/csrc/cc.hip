@@ -203,8 +203,8 @@ extern "C" int mg_cc(mg_graph *g, int32_t *label_out, const mg_options *opt, mg_
     Workspace &ws = g->ws;
     FrontierArgs fa = make_frontier_args(g, 1);
     int *p = ws.vals;
-    const int G = grid_blocks(std::max<int64_t>(g->n / 4, g->m / 64), BLK);
-    const int Gn = grid_blocks(g->n, BLK);
+    const int G = grid_blocks_cap(std::max<int64_t>(g->n / 4, g->m / 64), BLK, BIG_GRID);
+    const int Gn = grid_blocks_cap(g->n, BLK, BIG_GRID);
     MG_HIP(hipEventRecord(g->ev0, g->stream));
     hipLaunchKernelGGL(cc_init, dim3(Gn), dim3(BLK), 0, g->stream, fa, p);
     if ((g->flags & MG_GRAPH_SYMMETRIC) && o.engine != MG_ENGINE_GAS && g->n > 0) {

@@ -36,6 +36,7 @@ constexpr int SMALL_BIG = SMALL_E / (LANE_T + 1) + 1;  // lists > LANE_T in a fr
 enum { DIR_TD = 0, DIR_BU = 1 };
 
 constexpr int MAX_GRID = 2048;    // workgroups of a multi-workgroup launch (256 CUs x 8)
+constexpr int BIG_GRID = 8192;    // SSSP level and CC kernels: more workgroups hide their gather latency
 
 struct ChunkRec {
     int v, beg, end, pad;
@@ -95,7 +96,7 @@ struct Workspace {
     uint32_t *visited = nullptr;
     uint32_t *bm[3] = {nullptr, nullptr, nullptr};
     ChunkRec *chunks = nullptr;
-    LevelPart *part = nullptr;  // 3 * MAX_GRID slots
+    LevelPart *part = nullptr;  // 3 * BIG_GRID slots
     Ctl *ctl = nullptr;
     Ctl *h_ctl = nullptr;     // pinned host mirror
     // PageRank state (allocated on first use)
@@ -174,6 +175,8 @@ int alloc_f64(double **p, size_t n);
 int alloc_parts(LevelPart **p, size_t n);
 void free_dev(void *p);
 int grid_blocks(int64_t work_items, int block);
+// As grid_blocks, capped at `cap` workgroups instead of MAX_GRID.
+int grid_blocks_cap(int64_t work_items, int block, int cap);
 
 // Per-call bookkeeping shared by the search entry points.
 struct CallTimer {

@@ -18,10 +18,12 @@ namespace mg {
 
 static thread_local int t_device = 0;
 
-int grid_blocks(int64_t work_items, int block) {
+int grid_blocks(int64_t work_items, int block) { return grid_blocks_cap(work_items, block, MAX_GRID); }
+
+int grid_blocks_cap(int64_t work_items, int block, int cap) {
     int64_t b = (work_items + block - 1) / block;
     if (b < 1) b = 1;
-    if (b > MAX_GRID) b = MAX_GRID;  // kernels stride beyond this
+    if (b > cap) b = cap;  // kernels stride beyond this
     return (int)b;
 }
 
@@ -93,7 +95,7 @@ int ensure_workspace(mg_graph *g) {
         (rc = dalloc(&ws.q[2], n)) || (rc = dalloc(&ws.hard, n)) || (rc = dalloc(&ws.visited, ws.nwords)) ||
         (rc = dalloc(&ws.bm[0], ws.nwords)) || (rc = dalloc(&ws.bm[1], ws.nwords)) ||
         (rc = dalloc(&ws.bm[2], ws.nwords)) || (rc = dalloc(&ws.chunks, ws.chunk_cap)) ||
-        (rc = dalloc(&ws.part, 3 * MAX_GRID)) || (rc = dalloc(&ws.ctl, 1)) ||
+        (rc = dalloc(&ws.part, 3 * BIG_GRID)) || (rc = dalloc(&ws.ctl, 1)) ||
         (rc = dalloc(&ws.trace, 2 * (size_t)TRACE_CAP))) {
         free_workspace(ws);
         return rc;

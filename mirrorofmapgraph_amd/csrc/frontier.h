@@ -39,7 +39,7 @@ struct FrontierArgs {
     int *q[3];
     ChunkRec *chunks;
     int64_t chunk_cap;
-    LevelPart *part;     // per-workgroup partial counters, 2 * MAX_GRID slots
+    LevelPart *part;     // per-workgroup partial counters, 3 * BIG_GRID slots
     int n, nwords;
     long long *trace;    // per-level trace (TRACE_CAP levels), written by the level close
 };

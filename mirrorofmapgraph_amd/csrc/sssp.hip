@@ -953,7 +953,8 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
     a.fa = make_frontier_args(g, o.directed);
     a.dist = ws.vals;
     a.w = g->w;
-    a.G = std::max(PULL_SLICES, grid_blocks(std::max<int64_t>(g->n / 4, g->m / 64), BLK) / PULL_SLICES * PULL_SLICES);
+    a.G = std::max(PULL_SLICES,
+                   grid_blocks_cap(std::max<int64_t>(g->n / 4, g->m / 64), BLK, BIG_GRID) / PULL_SLICES * PULL_SLICES);
     a.dense_pct = sssp_dense_pct();
     // bucket width of the small-frontier loop: the option, else four maximum edge weights
     // (a frontier vertex's improvements then stay within the next few buckets; wider
