@@ -386,7 +386,7 @@ __global__ void sssp_seed(SsspArgs a, int src, int iter_limit, int can_pull, lon
 //
 // Narrow levels are latency-bound (a grid has thousands of them), so the loop keeps
 // the dependent chain per level short.  The frontier (at most SMALL_Q vertices and
-// SMALL_E edges), its distances, list starts and the prefix sums of its list lengths
+// SSSP_SMALL_E edges), its distances, list starts and the prefix sums of its list lengths
 // live in LDS; the threads walk the concatenated lists, SMALL_EW edges each in flight,
 // finding each slot's owner by binary search over the prefix sums, so a level is one
 // round of neighbour/weight loads and one of distance loads whatever the degrees.  The
@@ -405,6 +405,9 @@ __global__ void sssp_seed(SsspArgs a, int src, int iter_limit, int can_pull, lon
 // frontier vertex, unbucketed), the queue and the far list become the next bitmap.
 constexpr int SMALL_HASH = 2 * SMALL_Q;  // LDS hash set of a level's next frontier
 constexpr int SMALL_EW = 4;              // edges per thread in flight
+// Largest frontier (out-edges) the loop keeps: one workgroup relaxes ~0.4 G edges/s,
+// so wider levels are cheaper as whole-GPU launches (about 25 us for the three).
+constexpr int SSSP_SMALL_E = 16384;
 constexpr int SMALL_QPT = SMALL_Q / SMALL_BLK;  // queue entries per thread (contiguous)
 __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
     Ctl *c = a.fa.ctl;
@@ -441,7 +444,7 @@ __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
     }
     if (threadIdx.x == 0) {
         const int L = c->level;
-        s_go = !c->done && c->dir == DIR_PUSH && c->qsize[L % 3] <= SMALL_Q && c->mf_cur <= SMALL_E;
+        s_go = !c->done && c->dir == DIR_PUSH && c->qsize[L % 3] <= SMALL_Q && c->mf_cur <= SSSP_SMALL_E;
         s_L = L;
         s_cnt = c->qsize[L % 3];
         s_levels = 0;
@@ -707,7 +710,7 @@ __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
             // close level L on the LDS copy of the direction state (as sssp_close)
             trace_level(fa, L, DIR_PUSH, nf, mf);
             s_levels++;
-            s_go = nf > 0 && nf <= SMALL_Q && !(s_canpull && mf * s_alpha > s_m) && mf <= SMALL_E &&
+            s_go = nf > 0 && nf <= SMALL_Q && !(s_canpull && mf * s_alpha > s_m) && mf <= SSSP_SMALL_E &&
                    L + 1 < s_limit;
             s_L = L + 1;
             s_keep = nf;
