@@ -42,6 +42,7 @@ struct SsspArgs {
     // LANE_T split above)
     const ChunkRec *dpull;
     int ndpull;
+    int push_range;  // symmetric graph with dense tasks: wide push levels use range_push
     // dense pull: a pull level whose frontier holds more than dense_pct % of the edges
     // reads every in-neighbour's distance without the frontier test (> 100: never)
     int dense_pct;
@@ -348,6 +349,128 @@ __device__ __forceinline__ void dense_pull(const SsspArgs &a, int L, DenseLds &l
                 mf += view_degree(a.fa, r.v);
             }
         }
+    }
+}
+
+// Wide push levels of a symmetric graph (frontier out-edges above m / 16) reuse the
+// dense tasks, restricted to frontier vertices: a range task concatenates the lists of
+// its frontier vertices (the same LDS prefix sums and owner search as the dense pull,
+// so neighbour ids and weights are coalesced loads), a piece of a frontier hub's list
+// is walked by its wave; every edge relaxes its target with atomicMin and marks it.
+// One launch balances hubs and low-degree vertices, instead of lanes per vertex.
+__device__ __forceinline__ bool range_push_level(const SsspArgs &a, const Ctl *c) {
+    return a.push_range && c->mf_cur * 16 > c->m_unexplored;
+}
+
+__device__ __forceinline__ void range_push(const SsspArgs &a, int L, DenseLds &lds, long long &nf, long long &mf,
+                                           long long &ex) {
+    const FrontierArgs &fa = a.fa;
+    const uint32_t *fr = fa.bm[L % 3];
+    uint32_t *nx = fa.bm[(L + 1) % 3];
+    const int *adj = fa.adj[0];
+    const int *off = fa.off[0];
+    const int lane = lane_id(), wv = (int)threadIdx.x / WAVE;
+    int *pre = lds.pre[wv], *begs = lds.beg[wv], *dsrc = lds.min[wv];
+    const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
+    auto relax = [&](int u, long long nd, int du) {
+        if (nd >= du) return;
+        atomicMin(&a.dist[u], (int)nd);
+        if (set_bit_first(nx, u)) {
+            ++nf;
+            mf += view_degree(fa, u);
+        }
+    };
+    for (int t = wave; t < a.ndpull; t += nwaves) {
+        const ChunkRec r = a.dpull[t];
+        if (r.v >= 0) {
+            if (!test_bit(fr, r.v)) continue;  // wave-uniform
+            const long long dv = a.dist[r.v];
+            for (int e = r.beg + lane; e < r.end; e += 4 * WAVE) {
+                int sl[4], u[4], wt[4], du[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) sl[k] = e + k * WAVE < r.end ? e + k * WAVE : e;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) u[k] = adj[sl[k]];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) wt[k] = a.w[sl[k]];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) du[k] = a.dist[u[k]];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (e + k * WAVE < r.end) {
+                        ++ex;
+                        relax(u[k], dv + wt[k], du[k]);
+                    }
+            }
+            continue;
+        }
+        const int va = r.beg, vb = r.end;
+        int beg[DENSE_PER], len[DENSE_PER];
+#pragma unroll
+        for (int j = 0; j < DENSE_PER; ++j) {
+            const int v = va + j * WAVE + lane;
+            const int vv = v < vb ? v : va;
+            const int o0 = off[vv], o1 = off[vv + 1];
+            beg[j] = o0;
+            len[j] = v < vb && o1 - o0 <= DENSE_BIG && test_bit(fr, v) ? o1 - o0 : 0;
+        }
+        int total = 0;
+#pragma unroll
+        for (int j = 0; j < DENSE_PER; ++j) {
+            int incl = len[j];
+#pragma unroll
+            for (int d = 1; d < WAVE; d <<= 1) {
+                const int y = __shfl_up(incl, d);
+                if (lane >= d) incl += y;
+            }
+            const int v = va + j * WAVE + lane;
+            pre[j * WAVE + lane + 1] = total + incl;
+            begs[j * WAVE + lane] = beg[j];
+            dsrc[j * WAVE + lane] = len[j] ? a.dist[v] : 0;
+            total += __shfl(incl, WAVE - 1);
+        }
+        if (lane == 0) pre[0] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int safe = begs[0];  // in bounds when total > 0
+        for (int p0 = 0; p0 < total; p0 += DENSE_WIN * WAVE) {
+            int p[DENSE_WIN], own[DENSE_WIN];
+#pragma unroll
+            for (int k = 0; k < DENSE_WIN; ++k) {
+                p[k] = p0 + k * WAVE + lane;
+                own[k] = 0;
+            }
+#pragma unroll
+            for (int st = DENSE_V / 2; st > 0; st >>= 1) {
+                int q[DENSE_WIN];
+#pragma unroll
+                for (int k = 0; k < DENSE_WIN; ++k) q[k] = pre[own[k] + st];
+#pragma unroll
+                for (int k = 0; k < DENSE_WIN; ++k)
+                    if (q[k] <= p[k]) own[k] += st;
+            }
+            int sl[DENSE_WIN], u[DENSE_WIN], wt[DENSE_WIN], du[DENSE_WIN];
+            bool ok[DENSE_WIN];
+#pragma unroll
+            for (int k = 0; k < DENSE_WIN; ++k) {
+                ok[k] = p[k] < total;
+                sl[k] = ok[k] ? begs[own[k]] + (p[k] - pre[own[k]]) : safe;
+            }
+#pragma unroll
+            for (int k = 0; k < DENSE_WIN; ++k) u[k] = adj[sl[k]];
+#pragma unroll
+            for (int k = 0; k < DENSE_WIN; ++k) wt[k] = a.w[sl[k]];
+#pragma unroll
+            for (int k = 0; k < DENSE_WIN; ++k) du[k] = a.dist[u[k]];
+#pragma unroll
+            for (int k = 0; k < DENSE_WIN; ++k)
+                if (ok[k]) {
+                    ++ex;
+                    relax(u[k], (long long)dsrc[own[k]] + wt[k], du[k]);
+                }
+        }
+        __builtin_amdgcn_wave_barrier();  // the next task rewrites this wave's LDS rows
     }
 }
 
@@ -770,6 +893,10 @@ __global__ __launch_bounds__(BLK) void sssp_level_a(SsspArgs a, int view) {
     if (dir == DIR_PULL && view != 0) return;
     if (view == 0) clear_prev_sweep(fa, L, blockIdx.x * BLK + threadIdx.x, gridDim.x * BLK);
     if (dir == DIR_PUSH) {
+        if (range_push_level(a, c)) {
+            write_part<BLK>(fa, view * a.G + blockIdx.x, 0, 0, 0);  // the level is sssp_level_b's
+            return;
+        }
         SsspPush pol(a);
         expand_lanes<BLK>(fa, pol, view, L, fa.bm[L % 3]);
         write_part<BLK>(fa, view * a.G + blockIdx.x, pol.nf, pol.mf, pol.ex);
@@ -844,14 +971,16 @@ __global__ __launch_bounds__(BLK) void sssp_level_b(SsspArgs a) {
     if (c->done) return;
     const int L = c->level;
     long long nf = 0, mf = 0, ex = 0;
-    if (c->dir == DIR_PUSH) {
+    __shared__ DenseLds lds;
+    if (c->dir == DIR_PUSH && range_push_level(a, c)) {
+        range_push(a, L, lds, nf, mf, ex);
+    } else if (c->dir == DIR_PUSH) {
         SsspPush pol(a);
         expand_tasks<BLK>(fa, pol, L);
         nf = pol.nf;
         mf = pol.mf;
         ex = pol.ex;
     } else if (c->dense && a.dpull) {
-        __shared__ DenseLds lds;
         if (fa.eid[2]) dense_pull<true>(a, L, lds, nf, mf, ex);
         else dense_pull<false>(a, L, lds, nf, mf, ex);
     } else if (a.sliced) {
@@ -986,6 +1115,8 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
             if ((rc = ensure_dense_tasks(g, a.fa.off[2]))) return rc;
             a.dpull = ws.dense_tasks;
             a.ndpull = ws.dense_ntasks;
+            // the tasks cut the in-lists; on a symmetric graph those are the out-lists too
+            a.push_range = a.fa.off[2] == a.fa.off[0] && a.fa.nviews == 1 && a.fa.eid[0] == nullptr;
         }
     }
 
