@@ -78,6 +78,33 @@ __device__ void for_each_bit(const uint32_t *bm, int nwords, Fn fn) {
     }
 }
 
+// As for_each_bit, and every word is cleared once read (each word is read by one
+// thread, so nothing else in the launch may read the bitmap).
+template <class Fn>
+__device__ void for_each_bit_clear(uint32_t *bm, int nwords, Fn fn) {
+    constexpr int WPR = BLK / 2;
+    __shared__ int s_q[WPR * 32];
+    __shared__ int s_w[BLK / WAVE];
+    for (int w0 = blockIdx.x * WPR; w0 < nwords; w0 += gridDim.x * WPR) {
+        const int w = w0 + (int)threadIdx.x;
+        uint32_t word = (threadIdx.x < WPR && w < nwords) ? bm[w] : 0u;
+        if (word) bm[w] = 0;
+        int total;
+        int k = block_exclusive_scan<BLK>(__popc(word), s_w, &total);
+        while (word) {
+            const int b = __ffs(word) - 1;
+            s_q[k++] = w * 32 + b;
+            word &= word - 1;
+        }
+        __syncthreads();
+        for (int t0 = 0; t0 < total; t0 += BLK) {
+            const int i = t0 + (int)threadIdx.x;
+            fn(i < total ? s_q[i] : -1);
+        }
+        __syncthreads();
+    }
+}
+
 // Per-slot state of a batched search (device memory).  A slot holds one query
 // (source); slots are seeded and retired while the search keeps running.
 struct SlotCtl {

@@ -142,7 +142,7 @@ __global__ void batch_seed(BatchArgs a, SeedList sl) {
     }
 }
 
-// One workgroup: closes the pending level (partials of batch_fin_b).
+// One workgroup: closes the pending level (partials of batch_fin).
 __global__ __launch_bounds__(SMALL_BLK) void batch_close(BatchArgs a) {
     Ctl *c = a.ctl;
     if (!c->pending) return;
@@ -291,26 +291,20 @@ __global__ __launch_bounds__(BLK) void batch_b(BatchArgs a) {
     }
 }
 
-// Finalize, part 1: the old frontier's masks are cleared.
-__global__ __launch_bounds__(BLK) void batch_fin_a(BatchArgs a) {
+// Finalize: the old frontier's masks are cleared (except for vertices of the new
+// frontier, whose masks are overwritten below, so no vertex is written twice) along
+// with the old active bitmap, which becomes the next level's empty target; new bits
+// become visited and the next frontier; depths are written.
+__global__ __launch_bounds__(BLK) void batch_fin(BatchArgs a) {
     Ctl *c = a.ctl;
     if (c->done) return;
     const int L = c->level;
-    for_each_bit(a.act[L & 1], a.nwords, [&](int v) {
-        if (v >= 0) st8(a.F, v, zero8());
+    const uint32_t *nxt = a.act[(L + 1) & 1];
+    for_each_bit_clear(a.act[L & 1], a.nwords, [&](int v) {
+        if (v >= 0 && !test_bit(nxt, v)) st8(a.F, v, zero8());
     });
-}
-
-// Finalize, part 2: new bits become visited and the next frontier; depths written.
-__global__ __launch_bounds__(BLK) void batch_fin_b(BatchArgs a) {
-    Ctl *c = a.ctl;
-    if (c->done) return;
-    const int L = c->level;
     long long nf = 0, mf = 0, te = 0, nr = 0;
     M8 lv = zero8();
-    // the old active bitmap becomes the next level's (empty) target
-    uint32_t *old = a.act[L & 1];
-    for (int i = blockIdx.x * BLK + (int)threadIdx.x; i < a.nwords; i += gridDim.x * BLK) old[i] = 0;
     for_each_bit(a.act[(L + 1) & 1], a.nwords, [&](int u) {
         if (u < 0) return;
         const M8 nw = ld8(a.N, u);
@@ -455,8 +449,7 @@ int bfs_batch_step(BatchDev &b, mg_graph *g, hipStream_t st) {
     hipLaunchKernelGGL(batch_close, dim3(1), dim3(SMALL_BLK), 0, st, a);
     hipLaunchKernelGGL(batch_a, dim3(a.G), dim3(BLK), 0, st, a);
     hipLaunchKernelGGL(batch_b, dim3(a.G), dim3(BLK), 0, st, a);
-    hipLaunchKernelGGL(batch_fin_a, dim3(a.G), dim3(BLK), 0, st, a);
-    hipLaunchKernelGGL(batch_fin_b, dim3(a.G), dim3(BLK), 0, st, a);
+    hipLaunchKernelGGL(batch_fin, dim3(a.G), dim3(BLK), 0, st, a);
     MG_HIP(hipGetLastError());
     return MG_OK;
 }

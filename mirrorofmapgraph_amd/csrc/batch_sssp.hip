@@ -440,20 +440,17 @@ __global__ __launch_bounds__(BLK) void sb_relax(SBatchArgs a) {
     }
 }
 
-__global__ __launch_bounds__(BLK) void sb_fin_a(SBatchArgs a) {
-    Ctl *c = a.ctl;
-    if (c->done) return;
-    for_each_bit(a.act[c->level & 1], a.nwords, [&](int v) {
-        if (v >= 0) st8(a.F, v, zero8s());
-    });
-}
-
-__global__ __launch_bounds__(BLK) void sb_fin_b(SBatchArgs a) {
+// Finalize: the old frontier's masks are cleared (except for vertices of the new
+// frontier, overwritten below) with the old active bitmap, which becomes the next
+// level's empty target; the new masks become the frontier.
+__global__ __launch_bounds__(BLK) void sb_fin(SBatchArgs a) {
     Ctl *c = a.ctl;
     if (c->done) return;
     const int L = c->level;
-    uint32_t *old = a.act[L & 1];
-    for (int i = blockIdx.x * BLK + (int)threadIdx.x; i < a.nwords; i += gridDim.x * BLK) old[i] = 0;
+    const uint32_t *nxt = a.act[(L + 1) & 1];
+    for_each_bit_clear(a.act[L & 1], a.nwords, [&](int v) {
+        if (v >= 0 && !test_bit(nxt, v)) st8(a.F, v, zero8s());
+    });
     long long nf = 0;
     M8 lv = zero8s();
     for_each_bit(a.act[(L + 1) & 1], a.nwords, [&](int u) {
@@ -586,8 +583,7 @@ int sssp_batch_step(BatchDev &b, mg_graph *g, hipStream_t st, long long delta) {
     hipLaunchKernelGGL(sb_activate, dim3(a.G), dim3(BLK), 0, st, a);
     hipLaunchKernelGGL(sb_tasks, dim3(a.G), dim3(BLK), 0, st, a);
     hipLaunchKernelGGL(sb_relax, dim3(a.G), dim3(BLK), 0, st, a);
-    hipLaunchKernelGGL(sb_fin_a, dim3(a.G), dim3(BLK), 0, st, a);
-    hipLaunchKernelGGL(sb_fin_b, dim3(a.G), dim3(BLK), 0, st, a);
+    hipLaunchKernelGGL(sb_fin, dim3(a.G), dim3(BLK), 0, st, a);
     MG_HIP(hipGetLastError());
     return MG_OK;
 }
