@@ -52,18 +52,28 @@ struct BfsPolicy {
     __device__ bool visit(int, int, int, int, int u, bool valid, int L) {
         return valid && !test_bit(a.visited, u) && claim(u, L);
     }
-    // four neighbour ids, then their four visited words, in flight together
+    // Four neighbour ids, then their visited words, then the claims (test-and-set of
+    // the visited bits, claim_batch) with the degrees of the claimed vertices.
     __device__ void visit_range(int view, int, int, int beg, int end, int stride, int L) {
-        const int *adj = a.fa.adj[view];
+        const FrontierArgs &fa = a.fa;
+        const int *adj = fa.adj[view];
+        uint32_t *nx = fa.bm[(L + 1) % 3];
         for (int e = beg; e < end; e += 4 * stride) {
-            int u[4];
+            int u[4], uu[4];
             load4(adj, e, end, stride, u);
-            uint32_t w[4];
+            uint32_t w[4], need = 0;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) w[k] = u[k] >= 0 ? a.visited[u[k] >> 5] : ~0u;
+            for (int k = 0; k < 4; ++k) {
+                uu[k] = u[k] >= 0 ? u[k] : 0;  // in bounds; its result is ignored
+                w[k] = a.visited[uu[k] >> 5];
+            }
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                if (u[k] >= 0 && !((w[k] >> (u[k] & 31)) & 1u)) claim(u[k], L);
+                if (u[k] >= 0 && !((w[k] >> (u[k] & 31)) & 1u)) need |= 1u << k;
+            claim_batch<4>(fa, a.visited, uu, need, nf, mf, [&](int x) {
+                a.depth[x] = L + 1;
+                atomicOr(&nx[x >> 5], 1u << (x & 31));
+            });
         }
     }
 };
@@ -127,22 +137,15 @@ __global__ __launch_bounds__(SMALL_BLK) void bfs_small(BfsArgs a) {
     const FrontierArgs &fa = a.fa;
     __shared__ int s_q[SMALL_Q];
     __shared__ int s_big[SMALL_BIG];
+    __shared__ int s_bpre[SMALL_BIG], s_bbeg[SMALL_BIG], s_bwave[SMALL_BLK / WAVE];
     __shared__ int s_go, s_L, s_cnt, s_nbig, s_levels, s_dir, s_canbu, s_alpha, s_limit;
     __shared__ long long s_mu;
     if (c->pending) {
         const int L = c->level;
         const bool bu = c->dir == DIR_BU;
         const int na = (bu ? 1 : fa.nviews) * a.G;
-        long long nf = 0, mf = 0, ex = 0;
-        for (int i = threadIdx.x; i < na + a.G; i += SMALL_BLK) {
-            const LevelPart p = fa.part[i < na ? i : 2 * a.G + (i - na)];
-            nf += p.nf;
-            mf += p.mf;
-            ex += p.ex;
-        }
-        nf = block_sum<SMALL_BLK>(nf);
-        mf = block_sum<SMALL_BLK>(mf);
-        ex = block_sum<SMALL_BLK>(ex);
+        long long nf, mf, ex;
+        sum_level_parts<SMALL_BLK>(fa, na, a.G, nf, mf, ex);
         if (threadIdx.x == 0) {
             c->edges_examined += ex;
             bfs_close(a, L, nf, mf);
@@ -220,8 +223,52 @@ __global__ __launch_bounds__(SMALL_BLK) void bfs_small(BfsArgs a) {
     };
     load_mine(n0);
     __syncthreads();  // every entry is in registers: the queue is free for the next level
-    BfsPolicy pol(a);
     long long mf_over = 0;  // out-edges of claimed vertices that did not fit the queue
+    // Edge slots sl[j] (where ok[j]; sl[j] is in bounds either way): the neighbour ids,
+    // then the claims (workgroup-scope test-and-set: this workgroup owns the bitmaps
+    // while the loop runs), each stage in flight together; the claimed vertices are
+    // labelled, marked and queued.
+    auto claim4 = [&](const int *adj, const int (&sl)[4], const bool (&ok)[4], int L, uint32_t *nx) {
+        int u[4];
+        uint32_t old[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) u[j] = adj[sl[j]];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (ok[j])
+                old[j] = __hip_atomic_fetch_or(&a.visited[u[j] >> 5], 1u << (u[j] & 31), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        bool over[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            over[j] = false;
+            if (ok[j] && !((old[j] >> (u[j] & 31)) & 1u)) {
+                a.depth[u[j]] = L + 1;
+                __hip_atomic_fetch_or(&nx[u[j] >> 5], 1u << (u[j] & 31), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+                const int idx = atomicAdd(&s_cnt, 1);
+                if (idx < SMALL_Q) s_q[idx] = u[j];
+                else over[j] = true;
+            }
+        }
+        // claimed vertices past the queue's capacity: their degrees, loaded together
+        // (vertex 0's where not needed)
+        if (!wave_ballot(over[0] || over[1] || over[2] || over[3])) return;
+        int o0[4], o1[4];
+#pragma unroll
+        for (int view = 0; view < 2; ++view) {
+            if (view >= fa.nviews) break;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int d = over[j] ? u[j] : 0;
+                o0[j] = fa.off[view][d];
+                o1[j] = fa.off[view][d + 1];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (over[j]) mf_over += o1[j] - o0[j];
+        }
+    };
     for (;;) {
         const int L = s_L;
         uint32_t *nx = fa.bm[(L + 1) % 3];
@@ -271,18 +318,41 @@ __global__ __launch_bounds__(SMALL_BLK) void bfs_small(BfsArgs a) {
             }
         }
         __syncthreads();
-        // tile path for the few long lists
+        // the few long lists, flattened: the workgroup walks their concatenation, four
+        // slots per thread in flight, each slot's list found by binary search over the
+        // prefix sums of the lengths
         const int nbig = min(s_nbig, SMALL_BIG);
-        for (int t0 = 0; t0 < nbig; t0 += SMALL_BLK) {
-            const int i = t0 + (int)threadIdx.x;
-            auto emit = [&](int u, bool newly) {
-                if (!newly) return;
-                const int idx = atomicAdd(&s_cnt, 1);
-                if (idx < SMALL_Q) s_q[idx] = u;
-                else mf_over += view_degree(fa, u);
-            };
-            for (int view = 0; view < fa.nviews; ++view)
-                expand_tile<SMALL_BLK, false>(fa, pol, view, L, i < nbig ? s_big[i] : -1, emit);
+        for (int view = 0; view < fa.nviews && nbig > 0; ++view) {
+            const int *off = fa.off[view];
+            int len = 0, beg = 0;
+            if ((int)threadIdx.x < nbig) {
+                const int v = s_big[threadIdx.x];
+                beg = off[v];
+                len = off[v + 1] - beg;
+            }
+            int total;
+            const int ex = block_exclusive_scan<SMALL_BLK>(len, s_bwave, &total);
+            if ((int)threadIdx.x < nbig) {
+                s_bpre[threadIdx.x] = ex;
+                s_bbeg[threadIdx.x] = beg;
+            }
+            __syncthreads();
+            for (int p0 = 0; p0 < total; p0 += 4 * SMALL_BLK) {
+                int sl[4];
+                bool ok[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    int p = p0 + j * SMALL_BLK + (int)threadIdx.x;
+                    ok[j] = p < total;
+                    p = ok[j] ? p : total - 1;  // past the end: a real slot, not claimed
+                    int o = 0;  // last list whose prefix is <= p (a non-empty one)
+                    for (int st = 1 << (31 - __builtin_clz(nbig)); st > 0; st >>= 1)
+                        if (o + st < nbig && s_bpre[o + st] <= p) o += st;
+                    sl[j] = s_bbeg[o] + (p - s_bpre[o]);
+                }
+                claim4(fa.adj[view], sl, ok, L, nx);
+            }
+            __syncthreads();  // s_bpre / s_bbeg are rewritten for the next view
         }
         __syncthreads();
         const int nf = s_cnt;

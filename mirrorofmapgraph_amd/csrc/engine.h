@@ -26,7 +26,7 @@ constexpr int SMALL_BLK = 1024;   // threads of the single-workgroup small-front
 constexpr int CHUNK_T = 256;      // frontier vertices above this degree are split ...
 constexpr int CHUNK = 256;        // ... into edge chunks of this size (one workgroup pass each)
 constexpr int SMALL_Q = 4096;     // small-frontier kernel: max frontier vertices
-constexpr int SMALL_E = 65536;    // small-frontier kernel: max frontier edges
+constexpr int SMALL_E = 16384;    // small-frontier kernel: max frontier edges (BFS)
 constexpr int BU_INLINE = 32;     // bottom-up: edges a lane scans before handing the vertex to a wave
 constexpr int LANE_T = 32;        // lane/wave scheme: a lane walks vertices of up to LANE_T edges ...
 constexpr int TASK = 256;         // ... longer lists become TASK-edge tasks, one wave each

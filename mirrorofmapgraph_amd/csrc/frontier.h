@@ -151,6 +151,35 @@ __device__ __forceinline__ LevelPart sum_parts(const FrontierArgs &fa, int nslot
     return s;
 }
 
+// Sums the counters a level left in level_a's slots [0, na) and level_b's slots
+// [2G, 3G) with the whole workgroup.  The slots come from other XCDs, so every load
+// misses this CU's caches: each thread keeps PART_U of them in flight.
+constexpr int PART_U = 8;
+template <int BLOCK>
+__device__ __forceinline__ void sum_level_parts(const FrontierArgs &fa, int na, int G, long long &nf,
+                                                long long &mf, long long &ex) {
+    const int tot = na + G;
+    nf = mf = ex = 0;
+    for (int i0 = threadIdx.x; i0 < tot; i0 += PART_U * BLOCK) {
+        LevelPart p[PART_U];
+#pragma unroll
+        for (int k = 0; k < PART_U; ++k) {
+            const int i = i0 + k * BLOCK < tot ? i0 + k * BLOCK : i0;  // i0 < tot
+            p[k] = fa.part[i < na ? i : 2 * G + (i - na)];
+        }
+#pragma unroll
+        for (int k = 0; k < PART_U; ++k)
+            if (i0 + k * BLOCK < tot) {
+                nf += p[k].nf;
+                mf += p[k].mf;
+                ex += p[k].ex;
+            }
+    }
+    nf = block_sum<BLOCK>(nf);
+    mf = block_sum<BLOCK>(mf);
+    ex = block_sum<BLOCK>(ex);
+}
+
 // Clears bm[(L+2)%3] (frontier L-1) during level L by a sweep of 16-byte stores
 // (bitmaps are 256-byte aligned; nwords is even).
 __device__ __forceinline__ void clear_prev_sweep(const FrontierArgs &fa, int L, int tid, int nt) {
@@ -166,6 +195,44 @@ __device__ __forceinline__ bool mark_next(const FrontierArgs &fa, int L, int u) 
     uint32_t *bm = fa.bm[(L + 1) % 3];
     if (test_bit(bm, u)) return false;
     return set_bit_first(bm, u);
+}
+
+// Sets the bits of bitmap bm for the targets u[k] flagged in `need` (bit k) and,
+// for each bit this lane set first, calls on_first(u) and counts it in nf with its
+// out-degree in mf.  u[k] must be a vertex for every k.  The lane's test-and-sets go
+// out together, then the degree loads (clamped to vertex 0's shared line where not
+// needed): old[k] is left unassigned where no atomic is issued (it is read only
+// under the same flag), so no merge of the atomic's result waits for it inside the
+// branch.
+template <int K, class F>
+__device__ __forceinline__ void claim_batch(const FrontierArgs &fa, uint32_t *bm, const int (&u)[K], uint32_t need,
+                                            long long &nf, long long &mf, F on_first) {
+    if (!wave_ballot(need != 0)) return;
+    uint32_t old[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if ((need >> k) & 1u) old[k] = atomicOr(&bm[u[k] >> 5], 1u << (u[k] & 31));
+    bool first[K];
+    int o0[K], o1[K], p0[K], p1[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        first[k] = ((need >> k) & 1u) ? !((old[k] >> (u[k] & 31)) & 1u) : false;
+        const int d = first[k] ? u[k] : 0;
+        o0[k] = fa.off[0][d];
+        o1[k] = fa.off[0][d + 1];
+        if (fa.nviews > 1) {
+            p0[k] = fa.off[1][d];
+            p1[k] = fa.off[1][d + 1];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (first[k]) {
+            on_first(u[k]);
+            ++nf;
+            mf += o1[k] - o0[k];
+            if (fa.nviews > 1) mf += p1[k] - p0[k];
+        }
 }
 
 // One tile of up to BLOCK frontier vertices (thread i holds vertex v_i, or -1).
@@ -298,6 +365,7 @@ __device__ void expand_lanes(const FrontierArgs &fa, P &pol, int view, int L, co
     __shared__ int s_q[WPR * 32];
     __shared__ int s_wave2[BLOCK / WAVE];
     __shared__ int s_tbase;
+    __shared__ int s_tex[BLOCK], s_tv[BLOCK], s_tbeg[BLOCK], s_tend[BLOCK];
     const int *off = fa.off[view];
     for (int w0 = blockIdx.x * WPR; w0 < fa.nwords; w0 += gridDim.x * WPR) {
         const int w = w0 + (int)threadIdx.x;
@@ -322,20 +390,30 @@ __device__ void expand_lanes(const FrontierArgs &fa, P &pol, int view, int L, co
             int ttot;
             const int tex = block_exclusive_scan<BLOCK>(nt, s_wave2, &ttot);
             if (ttot > 0) {
+                // the workgroup writes the records together (a hub has hundreds): record j
+                // belongs to the last thread whose exclusive offset is <= j
                 if (threadIdx.x == 0) s_tbase = atomicAdd(&fa.ctl->nchunks, ttot);
+                s_tex[threadIdx.x] = tex;
+                s_tv[threadIdx.x] = v;
+                s_tbeg[threadIdx.x] = beg;
+                s_tend[threadIdx.x] = beg + deg;
                 __syncthreads();
-                const int c0 = s_tbase + tex;
-                if (nt && c0 + nt <= fa.chunk_cap) {
-                    for (int c = 0; c < nt; ++c) {
+                for (int j = threadIdx.x; j < ttot; j += BLOCK) {
+                    int o = 0;
+#pragma unroll
+                    for (int st = BLOCK / 2; st > 0; st >>= 1)
+                        if (s_tex[o + st] <= j) o += st;
+                    const int idx = s_tbase + j;
+                    if (idx < fa.chunk_cap) {
                         ChunkRec r;
-                        r.v = v;
-                        r.beg = beg + c * TASK;
-                        r.end = min(beg + (c + 1) * TASK, beg + deg);
+                        r.v = s_tv[o];
+                        r.beg = s_tbeg[o] + (j - s_tex[o]) * TASK;
+                        r.end = min(r.beg + TASK, s_tend[o]);
                         r.pad = view;
-                        fa.chunks[c0 + c] = r;
+                        fa.chunks[idx] = r;
+                    } else {
+                        atomicOr(&fa.ctl->changed, 1 << 30);  // capacity is sized so this cannot happen
                     }
-                } else if (nt) {
-                    atomicOr(&fa.ctl->changed, 1 << 30);  // capacity is sized so this cannot happen
                 }
                 __syncthreads();
             }

@@ -57,10 +57,28 @@ __device__ __forceinline__ long long edge_w(const SsspArgs &a, int view, int slo
 }
 
 // Push visit: relax (v, u); mark u when it improved.
+// Lowers dist[u[k]] to nd[k] where that improves it and marks the improved targets
+// in the next frontier nx (claim_batch: the lane's atomics go out together).  u[k]
+// must be a vertex even where ok[k] is false (callers load from clamped slots).  A
+// stale (larger) du[k] only causes a redundant relaxation: the target's distance
+// dropped this level either way, so it belongs to the next frontier.
+template <int K>
+__device__ __forceinline__ void relax_k(const SsspArgs &a, uint32_t *nx, const int (&u)[K],
+                                        const long long (&nd)[K], const int (&du)[K], const bool (&ok)[K],
+                                        long long &nf, long long &mf) {
+    uint32_t need = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (ok[k] && nd[k] < (long long)du[k]) need |= 1u << k;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if ((need >> k) & 1u) atomicMin(&a.dist[u[k]], (int)nd[k]);
+    claim_batch<K>(a.fa, nx, u, need, nf, mf, [](int) {});
+}
+
 struct SsspPush {
     const SsspArgs &a;
     long long nf = 0, mf = 0, ex = 0;
-    bool mark = true;  // false: report every improvement, the caller deduplicates
     __device__ explicit SsspPush(const SsspArgs &args) : a(args) {}
     // device-coherent: the small-frontier loop reads distances lowered earlier in the launch
     __device__ int load(int v) const { return ld_dev(&a.dist[v]); }
@@ -70,7 +88,6 @@ struct SsspPush {
     __device__ bool relax(long long nd, int du, int u, int L) {
         if (nd >= (long long)du) return false;
         atomicMin(&a.dist[u], (int)nd);
-        if (!mark) return true;
         if (!mark_next(a.fa, L, u)) return false;
         ++nf;
         mf += view_degree(a.fa, u);
@@ -81,12 +98,14 @@ struct SsspPush {
         ++ex;
         return relax((long long)d + edge_w(a, view, slot), ld_dev(&a.dist[u]), u, L);
     }
-    // four targets, weights and target distances in flight together
+    // four targets, weights and target distances in flight together, then relax_k
     __device__ void visit_range(int view, int, int d, int beg, int end, int stride, int L) {
         const int *adj = a.fa.adj[view];
         const int *eid = a.fa.eid[view];
+        uint32_t *nx = a.fa.bm[(L + 1) % 3];
         for (int e = beg; e < end; e += 4 * stride) {
             int sl[4], u[4], wt[4], du[4];
+            long long nd[4];
             bool ok[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -108,11 +127,11 @@ struct SsspPush {
 #pragma unroll
             for (int k = 0; k < 4; ++k) du[k] = a.dist[u[k]];
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (ok[k]) {
-                    ++ex;
-                    relax((long long)d + wt[k], du[k], u[k], L);
-                }
+            for (int k = 0; k < 4; ++k) {
+                nd[k] = (long long)d + wt[k];
+                ex += ok[k];
+            }
+            relax_k<4>(a, nx, u, nd, du, ok, nf, mf);
         }
     }
 };
@@ -303,20 +322,42 @@ __device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, in
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // write-back: the range's distances, then the degrees of the improved vertices,
+    // each stage's loads in flight together (clamped to va where not needed)
+    int cur[DENSE_PER];
+    bool found[DENSE_PER];
 #pragma unroll
     for (int j = 0; j < DENSE_PER; ++j) {
         const int v = va + j * WAVE + lane;
-        bool found = false;
-        if (len[j] > 0) {
-            const int best = mins[j * WAVE + lane];
-            if (best < a.dist[v]) {
-                a.dist[v] = best;
-                found = true;
-                ++nf;
-                mf += view_degree(fa, v);
-            }
+        cur[j] = a.dist[v < vb ? v : va];
+    }
+#pragma unroll
+    for (int j = 0; j < DENSE_PER; ++j) {
+        const int v = va + j * WAVE + lane;
+        const int best = mins[j * WAVE + lane];
+        found[j] = len[j] > 0 && best < cur[j];
+        if (found[j]) a.dist[v] = best;
+    }
+    int o0[DENSE_PER], o1[DENSE_PER], p0[DENSE_PER], p1[DENSE_PER];
+#pragma unroll
+    for (int j = 0; j < DENSE_PER; ++j) {
+        const int d = found[j] ? va + j * WAVE + lane : va;
+        o0[j] = fa.off[0][d];
+        o1[j] = fa.off[0][d + 1];
+        if (fa.nviews > 1) {
+            p0[j] = fa.off[1][d];
+            p1[j] = fa.off[1][d + 1];
         }
-        const uint64_t fm = wave_ballot(found);
+    }
+#pragma unroll
+    for (int j = 0; j < DENSE_PER; ++j) {
+        const int v = va + j * WAVE + lane;
+        if (found[j]) {
+            ++nf;
+            mf += o1[j] - o0[j];
+            if (fa.nviews > 1) mf += p1[j] - p0[j];
+        }
+        const uint64_t fm = wave_ballot(found[j]);
         const uint32_t mine = (lane < 32) ? (uint32_t)fm : (uint32_t)(fm >> 32);
         if ((lane & 31) == 0 && mine) atomicOr(&nx[v >> 5], mine);
     }
@@ -372,14 +413,6 @@ __device__ __forceinline__ void range_push(const SsspArgs &a, int L, DenseLds &l
     const int lane = lane_id(), wv = (int)threadIdx.x / WAVE;
     int *pre = lds.pre[wv], *begs = lds.beg[wv], *dsrc = lds.min[wv];
     const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
-    auto relax = [&](int u, long long nd, int du) {
-        if (nd >= du) return;
-        atomicMin(&a.dist[u], (int)nd);
-        if (set_bit_first(nx, u)) {
-            ++nf;
-            mf += view_degree(fa, u);
-        }
-    };
     for (int t = wave; t < a.ndpull; t += nwaves) {
         const ChunkRec r = a.dpull[t];
         if (r.v >= 0) {
@@ -395,12 +428,15 @@ __device__ __forceinline__ void range_push(const SsspArgs &a, int L, DenseLds &l
                 for (int k = 0; k < 4; ++k) wt[k] = a.w[sl[k]];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) du[k] = a.dist[u[k]];
+                long long nd[4];
+                bool ok[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    if (e + k * WAVE < r.end) {
-                        ++ex;
-                        relax(u[k], dv + wt[k], du[k]);
-                    }
+                for (int k = 0; k < 4; ++k) {
+                    ok[k] = e + k * WAVE < r.end;
+                    nd[k] = dv + wt[k];
+                    ex += ok[k];
+                }
+                relax_k<4>(a, nx, u, nd, du, ok, nf, mf);
             }
             continue;
         }
@@ -463,12 +499,13 @@ __device__ __forceinline__ void range_push(const SsspArgs &a, int L, DenseLds &l
             for (int k = 0; k < DENSE_WIN; ++k) wt[k] = a.w[sl[k]];
 #pragma unroll
             for (int k = 0; k < DENSE_WIN; ++k) du[k] = a.dist[u[k]];
+            long long nd[DENSE_WIN];
 #pragma unroll
-            for (int k = 0; k < DENSE_WIN; ++k)
-                if (ok[k]) {
-                    ++ex;
-                    relax(u[k], (long long)dsrc[own[k]] + wt[k], du[k]);
-                }
+            for (int k = 0; k < DENSE_WIN; ++k) {
+                nd[k] = (long long)dsrc[own[k]] + wt[k];
+                ex += ok[k];
+            }
+            relax_k<DENSE_WIN>(a, nx, u, nd, du, ok, nf, mf);
         }
         __builtin_amdgcn_wave_barrier();  // the next task rewrites this wave's LDS rows
     }
@@ -547,16 +584,8 @@ __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
         const int L = c->level;
         // pull levels: level_a (and level_c when sliced) slots; push: one slot set per view
         const int na = (c->dir == DIR_PULL ? (a.sliced ? 2 : 1) : fa.nviews) * a.G;
-        long long nf = 0, mf = 0, ex = 0;
-        for (int i = threadIdx.x; i < na + a.G; i += SMALL_BLK) {
-            const LevelPart p = fa.part[i < na ? i : 2 * a.G + (i - na)];
-            nf += p.nf;
-            mf += p.mf;
-            ex += p.ex;
-        }
-        nf = block_sum<SMALL_BLK>(nf);
-        mf = block_sum<SMALL_BLK>(mf);
-        ex = block_sum<SMALL_BLK>(ex);
+        long long nf, mf, ex;
+        sum_level_parts<SMALL_BLK>(fa, na, a.G, nf, mf, ex);
         if (threadIdx.x == 0) {
             c->edges_examined += ex;
             sssp_close(a, L, nf, mf);
