@@ -296,12 +296,14 @@ __global__ __launch_bounds__(SMALL_BLK) void bfs_small(BfsArgs a) {
                     int u[4];
                     uint32_t old[4];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) u[j] = p + j < e ? adj[p + j] : -1;
+                    for (int j = 0; j < 4; ++j) u[j] = adj[p + j < e ? p + j : p];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) u[j] = p + j < e ? u[j] : -1;
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        old[j] = u[j] >= 0 ? __hip_atomic_fetch_or(&a.visited[u[j] >> 5], 1u << (u[j] & 31),
-                                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-                                           : ~0u;
+                        if (u[j] >= 0)
+                            old[j] = __hip_atomic_fetch_or(&a.visited[u[j] >> 5], 1u << (u[j] & 31),
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const bool newly = u[j] >= 0 && !((old[j] >> (u[j] & 31)) & 1u);
