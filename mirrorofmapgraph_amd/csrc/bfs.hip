@@ -105,6 +105,8 @@ __global__ void bfs_init(BfsArgs a) {
 }
 
 __global__ void bfs_seed(BfsArgs a, int src, int iter_limit, int can_bu, int alpha, int beta, long long m_views) {
+    clear_parts(a.fa);
+    if (threadIdx.x != 0) return;
     Ctl *c = a.fa.ctl;
     const uint32_t bit = 1u << (src & 31);
     a.depth[src] = 0;
@@ -142,10 +144,8 @@ __global__ __launch_bounds__(SMALL_BLK) void bfs_small(BfsArgs a) {
     __shared__ long long s_mu;
     if (c->pending) {
         const int L = c->level;
-        const bool bu = c->dir == DIR_BU;
-        const int na = (bu ? 1 : fa.nviews) * a.G;
         long long nf, mf, ex;
-        sum_level_parts<SMALL_BLK>(fa, na, a.G, nf, mf, ex);
+        take_level_parts<SMALL_BLK>(fa, nf, mf, ex);
         if (threadIdx.x == 0) {
             c->edges_examined += ex;
             bfs_close(a, L, nf, mf);
@@ -444,7 +444,7 @@ __global__ __launch_bounds__(BLK) void bfs_level_a(BfsArgs a, int view) {
             }
         }
     }
-    write_part<BLK>(fa, view * a.G + blockIdx.x, pol.nf, pol.mf, 0);
+    write_part<BLK>(fa, pol.nf, pol.mf, 0);
 }
 
 // Edge chunks (top-down) or long in-edge scans (bottom-up) of the level.
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(BLK) void bfs_level_b(BfsArgs a) {
             }
         }
     }
-    write_part<BLK>(fa, 2 * a.G + blockIdx.x, pol.nf, pol.mf, 0);
+    write_part<BLK>(fa, pol.nf, pol.mf, 0);
     if (blockIdx.x == 0 && threadIdx.x == 0) c->pending = 1;
 }
 
@@ -596,7 +596,7 @@ extern "C" int mg_bfs(mg_graph *g, int32_t src, int32_t *depth_out, const mg_opt
 
     MG_HIP(hipEventRecord(g->ev0, g->stream));
     hipLaunchKernelGGL(bfs_init, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, g->stream, a);
-    hipLaunchKernelGGL(bfs_seed, dim3(1), dim3(1), 0, g->stream, a, src, o.iter_num, can_bu, o.alpha, o.beta,
+    hipLaunchKernelGGL(bfs_seed, dim3(1), dim3(PART_GROUPS), 0, g->stream, a, src, o.iter_num, can_bu, o.alpha, o.beta,
                        m_views);
     MG_HIP(hipGetLastError());
     auto step = [&]() -> int {

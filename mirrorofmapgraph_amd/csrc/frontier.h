@@ -123,57 +123,42 @@ __device__ T block_min(T x) {
     return t;
 }
 
-// Writes this workgroup's level counters into slot `slot`.
+// Level counters: every workgroup of a level's kernels adds its sums into slot
+// blockIdx % PART_GROUPS (skipped when all are zero); the next close reads the slots
+// and clears them, so one workgroup reads PART_GROUPS slots rather than one per
+// workgroup of the level (a few thousand, written on every XCD).  The seed kernels
+// clear them before a search.
+constexpr int PART_GROUPS = 256;
 template <int BLOCK>
-__device__ __forceinline__ void write_part(const FrontierArgs &fa, int slot, long long nf, long long mf,
-                                           long long ex) {
+__device__ __forceinline__ void write_part(const FrontierArgs &fa, long long nf, long long mf, long long ex) {
     nf = block_sum<BLOCK>(nf);
     mf = block_sum<BLOCK>(mf);
     ex = block_sum<BLOCK>(ex);
     if (threadIdx.x == 0) {
-        LevelPart p;
-        p.nf = nf;
-        p.mf = mf;
-        p.ex = ex;
-        p.pad = 0;
-        fa.part[slot] = p;
+        LevelPart *p = &fa.part[blockIdx.x % PART_GROUPS];
+        if (nf) atomicAdd((unsigned long long *)&p->nf, (unsigned long long)nf);
+        if (mf) atomicAdd((unsigned long long *)&p->mf, (unsigned long long)mf);
+        if (ex) atomicAdd((unsigned long long *)&p->ex, (unsigned long long)ex);
     }
 }
 
-// Sums the partial slots [0, nslots) (one thread; the previous kernels finished).
-__device__ __forceinline__ LevelPart sum_parts(const FrontierArgs &fa, int nslots) {
-    LevelPart s{0, 0, 0, 0};
-    for (int i = 0; i < nslots; ++i) {
-        s.nf += fa.part[i].nf;
-        s.mf += fa.part[i].mf;
-        s.ex += fa.part[i].ex;
-    }
-    return s;
+// Thread t < PART_GROUPS of a seed kernel clears slot t.
+__device__ __forceinline__ void clear_parts(const FrontierArgs &fa) {
+    if (threadIdx.x < PART_GROUPS) fa.part[threadIdx.x] = LevelPart{0, 0, 0, 0};
 }
 
-// Sums the counters a level left in level_a's slots [0, na) and level_b's slots
-// [2G, 3G) with the whole workgroup.  The slots come from other XCDs, so every load
-// misses this CU's caches: each thread keeps PART_U of them in flight.
-constexpr int PART_U = 8;
+// Sums the level counters (whole workgroup, BLOCK >= PART_GROUPS) and clears them.
 template <int BLOCK>
-__device__ __forceinline__ void sum_level_parts(const FrontierArgs &fa, int na, int G, long long &nf,
-                                                long long &mf, long long &ex) {
-    const int tot = na + G;
+__device__ __forceinline__ void take_level_parts(const FrontierArgs &fa, long long &nf, long long &mf,
+                                                 long long &ex) {
+    static_assert(BLOCK >= PART_GROUPS, "one slot per thread");
     nf = mf = ex = 0;
-    for (int i0 = threadIdx.x; i0 < tot; i0 += PART_U * BLOCK) {
-        LevelPart p[PART_U];
-#pragma unroll
-        for (int k = 0; k < PART_U; ++k) {
-            const int i = i0 + k * BLOCK < tot ? i0 + k * BLOCK : i0;  // i0 < tot
-            p[k] = fa.part[i < na ? i : 2 * G + (i - na)];
-        }
-#pragma unroll
-        for (int k = 0; k < PART_U; ++k)
-            if (i0 + k * BLOCK < tot) {
-                nf += p[k].nf;
-                mf += p[k].mf;
-                ex += p[k].ex;
-            }
+    if (threadIdx.x < PART_GROUPS) {
+        const LevelPart p = fa.part[threadIdx.x];
+        nf = p.nf;
+        mf = p.mf;
+        ex = p.ex;
+        fa.part[threadIdx.x] = LevelPart{0, 0, 0, 0};
     }
     nf = block_sum<BLOCK>(nf);
     mf = block_sum<BLOCK>(mf);

@@ -527,6 +527,8 @@ __device__ void sssp_close(const SsspArgs &a, int L, long long nf, long long mf)
 }
 
 __global__ void sssp_seed(SsspArgs a, int src, int iter_limit, int can_pull, long long m) {
+    clear_parts(a.fa);
+    if (threadIdx.x != 0) return;
     Ctl *c = a.fa.ctl;
     a.dist[src] = 0;
     a.fa.bm[0][src >> 5] = 1u << (src & 31);
@@ -582,10 +584,8 @@ __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
     __shared__ long long s_m, s_thr;
     if (c->pending) {
         const int L = c->level;
-        // pull levels: level_a (and level_c when sliced) slots; push: one slot set per view
-        const int na = (c->dir == DIR_PULL ? (a.sliced ? 2 : 1) : fa.nviews) * a.G;
         long long nf, mf, ex;
-        sum_level_parts<SMALL_BLK>(fa, na, a.G, nf, mf, ex);
+        take_level_parts<SMALL_BLK>(fa, nf, mf, ex);
         if (threadIdx.x == 0) {
             c->edges_examined += ex;
             sssp_close(a, L, nf, mf);
@@ -923,17 +923,15 @@ __global__ __launch_bounds__(BLK) void sssp_level_a(SsspArgs a, int view) {
     if (view == 0) clear_prev_sweep(fa, L, blockIdx.x * BLK + threadIdx.x, gridDim.x * BLK);
     if (dir == DIR_PUSH) {
         if (range_push_level(a, c)) {
-            write_part<BLK>(fa, view * a.G + blockIdx.x, 0, 0, 0);  // the level is sssp_level_b's
-            return;
+            return;  // the level is sssp_level_b's
         }
         SsspPush pol(a);
         expand_lanes<BLK>(fa, pol, view, L, fa.bm[L % 3]);
-        write_part<BLK>(fa, view * a.G + blockIdx.x, pol.nf, pol.mf, pol.ex);
+        write_part<BLK>(fa, pol.nf, pol.mf, pol.ex);
         return;
     }
     if (c->dense && a.dpull) {
-        write_part<BLK>(fa, blockIdx.x, 0, 0, 0);  // the level is sssp_level_b's
-        return;
+        return;  // the level is sssp_level_b's
     }
     if (a.sliced) {
         // XCD-sliced pull: this workgroup scans, for every vertex, the part of its
@@ -959,7 +957,7 @@ __global__ __launch_bounds__(BLK) void sssp_level_a(SsspArgs a, int view) {
                 pm[v] = (int)best;  // long ranges stay INT_MAX until their tasks
             }
         }
-        write_part<BLK>(fa, blockIdx.x, 0, 0, ex);
+        write_part<BLK>(fa, 0, 0, ex);
         return;
     }
     // pull, vertices of in-degree <= LANE_T, one lane each in vertex order; each
@@ -991,7 +989,7 @@ __global__ __launch_bounds__(BLK) void sssp_level_a(SsspArgs a, int view) {
         const uint32_t mine = (lane < 32) ? (uint32_t)fm : (uint32_t)(fm >> 32);
         if ((lane & 31) == 0 && mine) nx[v >> 5] = mine;
     }
-    write_part<BLK>(fa, blockIdx.x, nf, mf, ex);
+    write_part<BLK>(fa, nf, mf, ex);
 }
 
 __global__ __launch_bounds__(BLK) void sssp_level_b(SsspArgs a) {
@@ -1058,7 +1056,7 @@ __global__ __launch_bounds__(BLK) void sssp_level_b(SsspArgs a) {
             }
         }
     }
-    write_part<BLK>(fa, 2 * a.G + blockIdx.x, nf, mf, ex);
+    write_part<BLK>(fa, nf, mf, ex);
     if (blockIdx.x == 0 && threadIdx.x == 0) c->pending = 1;
 }
 
@@ -1089,7 +1087,7 @@ __global__ __launch_bounds__(BLK) void sssp_level_c(SsspArgs a) {
         const uint32_t mine = (lane < 32) ? (uint32_t)fm : (uint32_t)(fm >> 32);
         if ((lane & 31) == 0 && mine) nx[v >> 5] = mine;
     }
-    write_part<BLK>(fa, a.G + blockIdx.x, nf, mf, 0);
+    write_part<BLK>(fa, nf, mf, 0);
 }
 
 }  // namespace mg
@@ -1154,7 +1152,7 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
     for (int i = 0; i < 3; ++i)
         if ((rc = launch_fill(g, ws.bm[i], 0u, ws.nwords))) return rc;
     if ((rc = launch_fill(g, ws.visited, 0u, ws.nwords))) return rc;  // far bitmap
-    hipLaunchKernelGGL(sssp_seed, dim3(1), dim3(1), 0, g->stream, a, src, o.iter_num, can_pull, (long long)g->m);
+    hipLaunchKernelGGL(sssp_seed, dim3(1), dim3(PART_GROUPS), 0, g->stream, a, src, o.iter_num, can_pull, (long long)g->m);
     MG_HIP(hipGetLastError());
     auto step = [&]() -> int {
         hipLaunchKernelGGL(sssp_small, dim3(1), dim3(SMALL_BLK), 0, g->stream, a);
