@@ -136,6 +136,7 @@ __global__ void bfs_seed(BfsArgs a, int src, int iter_limit, int can_bu, int alp
 
 __global__ __launch_bounds__(SMALL_BLK) void bfs_small(BfsArgs a) {
     Ctl *c = a.fa.ctl;
+    if (threadIdx.x == 0 && !c->done) c->steps++;  // read by the host to size the next search's launches
     const FrontierArgs &fa = a.fa;
     __shared__ int s_q[SMALL_Q];
     __shared__ int s_big[SMALL_BIG];

@@ -56,7 +56,7 @@ struct Ctl {
     int level;          // level being processed
     int done;           // 1 once the search has converged or hit iter_limit
     int dir;            // DIR_TD or DIR_BU for the current level
-    int blocks_done;    // last-workgroup-done counter of the closing kernel
+    int steps;          // BFS / SSSP: steps whose first kernel found the search running
     int qsize[3];       // sizes of the three rotating frontier queues
     int nchunks;        // edge chunks queued this level
     int nhard;          // bottom-up vertices handed to whole waves this level
@@ -208,8 +208,10 @@ constexpr int DENSE_V = 256;
 enum { KIND_BFS = 0, KIND_SSSP = 1, KIND_PR = 2, KIND_GAS = 3, KIND_BATCH = 4, KIND_BATCH_SSSP = 5, NKINDS = 8 };
 
 struct StepGraph {
-    hipGraphExec_t exec = nullptr;
+    hipGraphExec_t exec = nullptr;                        // STEPS_PER_GRAPH steps
+    hipGraphExec_t part[STEPS_PER_GRAPH] = {};  // part[i]: i steps (1 <= i < STEPS_PER_GRAPH; captured on first use)
     uint64_t key = ~0ULL;
+    int64_t last_steps = 0;  // steps the previous search with this key ran (0: not counted)
 };
 
 template <class Step>
@@ -263,12 +265,10 @@ int mg::drive(mg_graph *g, Step step, int64_t max_steps, int kind, uint64_t key,
     }
     if (kind >= 0 && kind < NKINDS && graphs_enabled()) {
         StepGraph &sg = g->steps->sg[kind];
-        if (!sg.exec || sg.key != key) {
-            if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
-            sg.exec = nullptr;
+        auto capture = [&](int nsteps, hipGraphExec_t *out) -> int {
             MG_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
             int rc = MG_OK;
-            for (int i = 0; i < STEPS_PER_GRAPH && rc == MG_OK; ++i) rc = step();
+            for (int i = 0; i < nsteps && rc == MG_OK; ++i) rc = step();
             hipGraph_t graph = nullptr;
             hipError_t e = hipStreamEndCapture(g->stream, &graph);
             if (rc) {
@@ -276,21 +276,62 @@ int mg::drive(mg_graph *g, Step step, int64_t max_steps, int kind, uint64_t key,
                 return rc;
             }
             if (e != hipSuccess) return set_error(MG_ERR_DEVICE, "stream capture failed: %s", hipGetErrorString(e));
-            e = hipGraphInstantiate(&sg.exec, graph, nullptr, nullptr, 0);
+            e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             if (e != hipSuccess) {
-                sg.exec = nullptr;
+                *out = nullptr;
                 return set_error(MG_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
             }
+            return MG_OK;
+        };
+        if (!sg.exec || sg.key != key) {
+            if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+            for (hipGraphExec_t &x : sg.part) {
+                if (x) (void)hipGraphExecDestroy(x);
+                x = nullptr;
+            }
+            sg.exec = nullptr;
+            sg.last_steps = 0;
+            int rc = capture(STEPS_PER_GRAPH, &sg.exec);
+            if (rc) return rc;
             sg.key = key;
         }
+        auto finish = [&]() {
+            // kinds whose steps count themselves: the next search starts with this many
+            sg.last_steps = (kind == KIND_BFS || kind == KIND_SSSP) ? h_ctl->steps : 0;
+        };
         int64_t queued = 0, k = 1;
+        // A search like the previous one (same graph, kind and options) ran last_steps
+        // steps: queue exactly that many (full graphs, then one graph of the rest) before
+        // the first look at the control block, so no empty steps trail the search.
+        if (sg.last_steps > 0) {
+            int64_t left = std::min<int64_t>(sg.last_steps, max_steps);
+            for (; left >= STEPS_PER_GRAPH; left -= STEPS_PER_GRAPH, queued += STEPS_PER_GRAPH)
+                MG_HIP(hipGraphLaunch(sg.exec, g->stream));
+            if (left > 0) {  // one graph of the remaining steps (each graph boundary costs ~10 us)
+                if (!sg.part[left]) {
+                    int rc = capture((int)left, &sg.part[left]);
+                    if (rc) return rc;
+                }
+                MG_HIP(hipGraphLaunch(sg.part[left], g->stream));
+                queued += left;
+            }
+            MG_HIP(hipMemcpyAsync(h_ctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
+            MG_HIP(hipStreamSynchronize(g->stream));
+            if (h_ctl->done) {
+                finish();
+                return MG_OK;
+            }
+        }
         while (queued < max_steps) {
             for (int i = 0; i < k && queued < max_steps; ++i, queued += STEPS_PER_GRAPH)
                 MG_HIP(hipGraphLaunch(sg.exec, g->stream));
             MG_HIP(hipMemcpyAsync(h_ctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
             MG_HIP(hipStreamSynchronize(g->stream));
-            if (h_ctl->done) return MG_OK;
+            if (h_ctl->done) {
+                finish();
+                return MG_OK;
+            }
             if (k < 16) k *= 2;
         }
         return MG_OK;

@@ -319,8 +319,11 @@ int mg_graph_destroy(mg_graph *g) {
         gas_release(g);
         batch_release(g);
         ppr_release(g);
-        for (auto &s : g->steps->sg)
+        for (auto &s : g->steps->sg) {
             if (s.exec) (void)hipGraphExecDestroy(s.exec);
+            for (hipGraphExec_t x : s.part)
+                if (x) (void)hipGraphExecDestroy(x);
+        }
         delete g->steps;
         g->steps = nullptr;
     }

@@ -501,7 +501,7 @@ __device__ __forceinline__ void ctl_reset(Ctl *c, int n, int iter_limit, int fro
     c->level = 0;
     c->done = iter_limit <= 0 || frontier0 == 0;
     c->dir = DIR_TD;
-    c->blocks_done = 0;
+    c->steps = 0;
     c->qsize[0] = frontier0;
     c->qsize[1] = 0;
     c->qsize[2] = 0;

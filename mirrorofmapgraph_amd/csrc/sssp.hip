@@ -573,6 +573,7 @@ constexpr int SSSP_SMALL_E = 16384;
 constexpr int SMALL_QPT = SMALL_Q / SMALL_BLK;  // queue entries per thread (contiguous)
 __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
     Ctl *c = a.fa.ctl;
+    if (threadIdx.x == 0 && !c->done) c->steps++;  // read by the host to size the next search's launches
     const FrontierArgs &fa = a.fa;
     __shared__ int s_q[SMALL_Q];             // next-frontier appends (and the entry frontier)
     __shared__ int s_hash[SMALL_HASH];
