@@ -93,25 +93,25 @@ __device__ void bfs_close(const BfsArgs &a, int L, long long nf, long long mf) {
     advance_level(c, L, nf, mf);
 }
 
-__global__ void bfs_init(BfsArgs a) {
+// Starts a search in one launch: depths (0 at src, -1 elsewhere), the visited and
+// frontier bitmaps (src marked in visited and frontier 0), the level counters, and the
+// control block (workgroup 0).  Every word is written once, so no ordering between
+// workgroups is needed.
+__global__ void bfs_begin(BfsArgs a, int src, int iter_limit, int can_bu, int alpha, int beta, long long m_views) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
-    for (int i = tid; i < a.fa.n; i += nt) a.depth[i] = -1;
+    for (int i = tid; i < a.fa.n; i += nt) a.depth[i] = i == src ? 0 : -1;
+    const uint32_t bit = 1u << (src & 31);
     for (int i = tid; i < a.fa.nwords; i += nt) {
-        a.visited[i] = 0;
-        a.fa.bm[0][i] = 0;
+        const uint32_t b = i == (src >> 5) ? bit : 0u;
+        a.visited[i] = b;
+        a.fa.bm[0][i] = b;
         a.fa.bm[1][i] = 0;
         a.fa.bm[2][i] = 0;
     }
-}
-
-__global__ void bfs_seed(BfsArgs a, int src, int iter_limit, int can_bu, int alpha, int beta, long long m_views) {
+    if (blockIdx.x != 0) return;
     clear_parts(a.fa);
     if (threadIdx.x != 0) return;
     Ctl *c = a.fa.ctl;
-    const uint32_t bit = 1u << (src & 31);
-    a.depth[src] = 0;
-    a.visited[src >> 5] = bit;
-    a.fa.bm[0][src >> 5] = bit;
     a.fa.q[0][0] = src;
     ctl_reset(c, a.fa.n, iter_limit, 1);
     const long long d = view_degree(a.fa, src);
@@ -596,9 +596,8 @@ extern "C" int mg_bfs(mg_graph *g, int32_t src, int32_t *depth_out, const mg_opt
     const long long m_views = g->m * a.fa.nviews;
 
     MG_HIP(hipEventRecord(g->ev0, g->stream));
-    hipLaunchKernelGGL(bfs_init, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, g->stream, a);
-    hipLaunchKernelGGL(bfs_seed, dim3(1), dim3(PART_GROUPS), 0, g->stream, a, src, o.iter_num, can_bu, o.alpha, o.beta,
-                       m_views);
+    hipLaunchKernelGGL(bfs_begin, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, g->stream, a, src, o.iter_num, can_bu,
+                       o.alpha, o.beta, m_views);
     MG_HIP(hipGetLastError());
     auto step = [&]() -> int {
         hipLaunchKernelGGL(bfs_small, dim3(1), dim3(SMALL_BLK), 0, g->stream, a);

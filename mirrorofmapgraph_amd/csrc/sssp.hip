@@ -526,12 +526,23 @@ __device__ void sssp_close(const SsspArgs &a, int L, long long nf, long long mf)
     advance_level(c, L, nf, mf);
 }
 
-__global__ void sssp_seed(SsspArgs a, int src, int iter_limit, int can_pull, long long m) {
+// Starts a search in one launch: distances (0 at src, INT_MAX elsewhere), the frontier
+// bitmaps (src marked in frontier 0) and the far bitmap, the level counters, and the
+// control block (workgroup 0).  Every word is written once, so no ordering between
+// workgroups is needed.
+__global__ void sssp_begin(SsspArgs a, int src, int iter_limit, int can_pull, long long m) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+    for (int v = tid; v < a.fa.n; v += nt) a.dist[v] = v == src ? 0 : INT_MAX;
+    for (int i = tid; i < a.fa.nwords; i += nt) {
+        a.fa.bm[0][i] = i == (src >> 5) ? 1u << (src & 31) : 0u;
+        a.fa.bm[1][i] = 0;
+        a.fa.bm[2][i] = 0;
+        a.farbm[i] = 0;
+    }
+    if (blockIdx.x != 0) return;
     clear_parts(a.fa);
     if (threadIdx.x != 0) return;
     Ctl *c = a.fa.ctl;
-    a.dist[src] = 0;
-    a.fa.bm[0][src >> 5] = 1u << (src & 31);
     a.fa.q[0][0] = src;
     ctl_reset(c, a.fa.n, iter_limit, 1);
     c->thr = a.delta;  // the first bucket: [0, delta)
@@ -1149,11 +1160,8 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
     }
 
     MG_HIP(hipEventRecord(g->ev0, g->stream));
-    if ((rc = launch_fill(g, ws.vals, INT_MAX, g->n))) return rc;
-    for (int i = 0; i < 3; ++i)
-        if ((rc = launch_fill(g, ws.bm[i], 0u, ws.nwords))) return rc;
-    if ((rc = launch_fill(g, ws.visited, 0u, ws.nwords))) return rc;  // far bitmap
-    hipLaunchKernelGGL(sssp_seed, dim3(1), dim3(PART_GROUPS), 0, g->stream, a, src, o.iter_num, can_pull, (long long)g->m);
+    hipLaunchKernelGGL(sssp_begin, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, g->stream, a, src, o.iter_num, can_pull,
+                       (long long)g->m);
     MG_HIP(hipGetLastError());
     auto step = [&]() -> int {
         hipLaunchKernelGGL(sssp_small, dim3(1), dim3(SMALL_BLK), 0, g->stream, a);
