@@ -195,6 +195,19 @@ def test_repeated_searches_reuse_state(mg, oracle, device, s16):
         assert dg.device_bytes() == b0
 
 
+def test_searches_of_different_depths_in_a_row(mg, oracle, device, s16):
+    """A search queues as many steps as the previous one with the same options ran
+    before it reads the control block; deeper, shallower and empty searches in a row
+    must still run to the end."""
+    g = mg.grid_graph(128, seed=5, weight_max=16)
+    iso = int(np.argmin(s16.degrees()))
+    for h, srcs in ((g, [0, 64 * 128 + 64, 0, 128 * 128 - 1, 5]), (s16, [0, iso, 0, 777, iso, 3])):
+        with mg.Graph(h, symmetric=True) as dg:
+            for s in srcs:
+                assert np.array_equal(dg.bfs(s), oracle.bfs(h.row_offsets, h.col_idx, s)), s
+                assert np.array_equal(dg.sssp(s), oracle.sssp(h.row_offsets, h.col_idx, h.values, s)), s
+
+
 def test_device_csc_matches_host(mg, device):
     g = mg.rmat_graph(12, seed=2, symmetric=False)
     with mg.Graph(g, build_csc=True) as dg:
