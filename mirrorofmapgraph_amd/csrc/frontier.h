@@ -126,8 +126,8 @@ __device__ T block_min(T x) {
 // Level counters: every workgroup of a level's kernels adds its sums into slot
 // blockIdx % PART_GROUPS (skipped when all are zero); the next close reads the slots
 // and clears them, so one workgroup reads PART_GROUPS slots rather than one per
-// workgroup of the level (a few thousand, written on every XCD).  The seed kernels
-// clear them before a search.
+// workgroup of the level (a few thousand, written on every XCD).  The kernel that
+// starts a search clears them.
 constexpr int PART_GROUPS = 256;
 template <int BLOCK>
 __device__ __forceinline__ void write_part(const FrontierArgs &fa, long long nf, long long mf, long long ex) {
@@ -142,7 +142,7 @@ __device__ __forceinline__ void write_part(const FrontierArgs &fa, long long nf,
     }
 }
 
-// Thread t < PART_GROUPS of a seed kernel clears slot t.
+// Thread t < PART_GROUPS of workgroup 0 of a search's first kernel clears slot t.
 __device__ __forceinline__ void clear_parts(const FrontierArgs &fa) {
     if (threadIdx.x < PART_GROUPS) fa.part[threadIdx.x] = LevelPart{0, 0, 0, 0};
 }
