@@ -70,10 +70,7 @@ struct BfsPolicy {
 #pragma unroll
             for (int k = 0; k < 4; ++k)
                 if (u[k] >= 0 && !((w[k] >> (u[k] & 31)) & 1u)) need |= 1u << k;
-            claim_batch<4>(fa, a.visited, uu, need, nf, mf, [&](int x) {
-                a.depth[x] = L + 1;
-                atomicOr(&nx[x >> 5], 1u << (x & 31));
-            });
+            claim_batch<4, true>(fa, a.visited, uu, need, nf, mf, [&](int x) { a.depth[x] = L + 1; }, nx);
         }
     }
 };

@@ -100,6 +100,17 @@ __device__ __forceinline__ int seg_min_scan(int x, int lane, int start) {
     return x;
 }
 
+// Inclusive OR over lanes [start, lane] of this lane's run (every lane of the wave
+// active; start <= lane).
+__device__ __forceinline__ uint32_t seg_or_scan(uint32_t x, int lane, int start) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane - d >= start) x |= y;
+    }
+    return x;
+}
+
 __device__ __forceinline__ bool test_bit(const uint32_t *bm, int v) {
     return (bm[v >> 5] >> (v & 31)) & 1u;
 }

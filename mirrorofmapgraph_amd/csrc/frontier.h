@@ -184,24 +184,67 @@ __device__ __forceinline__ bool mark_next(const FrontierArgs &fa, int L, int u) 
 
 // Sets the bits of bitmap bm for the targets u[k] flagged in `need` (bit k) and,
 // for each bit this lane set first, calls on_first(u) and counts it in nf with its
-// out-degree in mf.  u[k] must be a vertex for every k.  The lane's test-and-sets go
-// out together, then the degree loads (clamped to vertex 0's shared line where not
-// needed): old[k] is left unassigned where no atomic is issued (it is read only
-// under the same flag), so no merge of the atomic's result waits for it inside the
-// branch.
-template <int K, class F>
+// out-degree in mf; mirror (if not null) gets the same newly set bits.  u[k] must
+// be a vertex for every k.  The lane's test-and-sets go out together, then the
+// degree loads (clamped to vertex 0's shared line where not needed): old[k] is left
+// unassigned where no atomic is issued (it is read only under the same flag), so no
+// merge of the atomic's result waits for it inside the branch.
+// RUNS (for dense claims such as a BFS level from a hub): with the whole wave active,
+// lanes whose targets share a bitmap word in a run of consecutive lanes (sorted lists)
+// OR their bits together and the run's last lane issues one atomic for the run; each
+// lane then reads its bit in the returned word (and in the bits of the lanes before
+// it, for repeated targets).  Where claims are sparse the per-lane form is cheaper.
+template <int K, bool RUNS, class F>
 __device__ __forceinline__ void claim_batch(const FrontierArgs &fa, uint32_t *bm, const int (&u)[K], uint32_t need,
-                                            long long &nf, long long &mf, F on_first) {
+                                            long long &nf, long long &mf, F on_first, uint32_t *mirror = nullptr) {
     if (!wave_ballot(need != 0)) return;
-    uint32_t old[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-        if ((need >> k) & 1u) old[k] = atomicOr(&bm[u[k] >> 5], 1u << (u[k] & 31));
     bool first[K];
+    if (RUNS && wave_ballot(true) == ~0ull) {
+        const int lane = lane_id();
+        const uint64_t upto = lane == WAVE - 1 ? ~0ull : ((2ull << lane) - 1);  // lanes 0..lane
+        uint32_t mine[K], incl[K], prev[K], old[K];
+        int tail[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            mine[k] = ((need >> k) & 1u) << (u[k] & 31);
+            const int w = u[k] >> 5;
+            const int wp = __shfl_up(w, 1);
+            const uint64_t heads = wave_ballot(lane == 0 || wp != w);
+            const int start = WAVE - 1 - __builtin_clzll(heads & upto);
+            const uint64_t later = heads & ~upto;
+            tail[k] = later ? __builtin_ctzll(later) - 1 : WAVE - 1;
+            incl[k] = seg_or_scan(mine[k], lane, start);
+            const uint32_t pv = __shfl_up(incl[k], 1);
+            prev[k] = lane > start ? pv : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            old[k] = 0;  // read by __shfl below in every lane: defined everywhere
+            if (lane == tail[k] && incl[k]) old[k] = atomicOr(&bm[u[k] >> 5], incl[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t o = __shfl(old[k], tail[k]);
+            first[k] = mine[k] && !(o & mine[k]) && !(prev[k] & mine[k]);
+            if (mirror && lane == tail[k] && incl[k]) {
+                const uint32_t fresh = incl[k] & ~o;  // bits this run set first
+                if (fresh) atomicOr(&mirror[u[k] >> 5], fresh);
+            }
+        }
+    } else {
+        uint32_t old[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if ((need >> k) & 1u) old[k] = atomicOr(&bm[u[k] >> 5], 1u << (u[k] & 31));
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            first[k] = ((need >> k) & 1u) ? !((old[k] >> (u[k] & 31)) & 1u) : false;
+            if (mirror && first[k]) atomicOr(&mirror[u[k] >> 5], 1u << (u[k] & 31));
+        }
+    }
     int o0[K], o1[K], p0[K], p1[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        first[k] = ((need >> k) & 1u) ? !((old[k] >> (u[k] & 31)) & 1u) : false;
         const int d = first[k] ? u[k] : 0;
         o0[k] = fa.off[0][d];
         o1[k] = fa.off[0][d + 1];
