@@ -193,13 +193,15 @@ __device__ __forceinline__ bool mark_next(const FrontierArgs &fa, int L, int u) 
 // lanes whose targets share a bitmap word in a run of consecutive lanes (sorted lists)
 // OR their bits together and the run's last lane issues one atomic for the run; each
 // lane then reads its bit in the returned word (and in the bits of the lanes before
-// it, for repeated targets).  Where claims are sparse the per-lane form is cheaper.
+// it, for repeated targets).  Where claims are sparse (fewer than RUNS_MIN lanes with
+// one) the per-lane form is cheaper.
+constexpr int RUNS_MIN = 16;  // lanes with claims below which a wave takes the per-lane form
 template <int K, bool RUNS, class F>
 __device__ __forceinline__ void claim_batch(const FrontierArgs &fa, uint32_t *bm, const int (&u)[K], uint32_t need,
                                             long long &nf, long long &mf, F on_first, uint32_t *mirror = nullptr) {
     if (!wave_ballot(need != 0)) return;
     bool first[K];
-    if (RUNS && wave_ballot(true) == ~0ull) {
+    if (RUNS && wave_ballot(true) == ~0ull && __popcll(wave_ballot(need != 0)) >= RUNS_MIN) {
         const int lane = lane_id();
         const uint64_t upto = lane == WAVE - 1 ? ~0ull : ((2ull << lane) - 1);  // lanes 0..lane
         uint32_t mine[K], incl[K], prev[K], old[K];

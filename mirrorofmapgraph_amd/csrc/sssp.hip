@@ -73,7 +73,7 @@ __device__ __forceinline__ void relax_k(const SsspArgs &a, uint32_t *nx, const i
 #pragma unroll
     for (int k = 0; k < K; ++k)
         if ((need >> k) & 1u) atomicMin(&a.dist[u[k]], (int)nd[k]);
-    claim_batch<K, false>(a.fa, nx, u, need, nf, mf, [](int) {});
+    claim_batch<K, true>(a.fa, nx, u, need, nf, mf, [](int) {});
 }
 
 struct SsspPush {
