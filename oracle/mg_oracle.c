@@ -15,8 +15,13 @@
  * csgraph routines (breadth-first order, Dijkstra, connected components) on random
  * graphs, and against brute force on tiny graphs; the two SSSP variants are checked
  * against each other.  PageRank variants are pinned by closed forms (star, 2-cycle,
- * chain), by a float64 power iteration with numpy, and by the fixed point of the
- * Gauss-Seidel sweep.
+ * chain), by a float64 power iteration with numpy, and by hand-derived known answers
+ * of the program's activation rule and of the Gauss-Seidel sweep order.
+ *
+ * The Jacobi PageRank loops run their per-vertex sums in parallel (OpenMP) over
+ * destination vertices: each vertex's sum stays sequential in CSC order and the L1
+ * change is summed in vertex order afterwards, so the output does not depend on the
+ * thread count.
  */
 #include <limits.h>
 #include <math.h>
@@ -226,6 +231,30 @@ ORACLE_API int oracle_cc(int n, const int *row_offsets, const int *col_idx, cons
     return sweeps;
 }
 
+/* Same labels as oracle_cc (min vertex id of each weakly connected component) by
+ * union-find over the CSR edges: each union hangs the larger root under the smaller,
+ * so a root is always its component's smallest id.  O(m alpha(n)) instead of the
+ * sweep count times m; used as the checker on the largest graphs and pinned against
+ * oracle_cc and scipy on smaller ones. */
+static int uf_find(int *parent, int v) {
+    while (parent[v] != v) {
+        parent[v] = parent[parent[v]];
+        v = parent[v];
+    }
+    return v;
+}
+
+ORACLE_API void oracle_cc_union_find(int n, const int *row_offsets, const int *col_idx, int *label) {
+    for (int i = 0; i < n; i++) label[i] = i;
+    for (int u = 0; u < n; u++)
+        for (int j = row_offsets[u]; j < row_offsets[u + 1]; j++) {
+            int a = uf_find(label, u), b = uf_find(label, col_idx[j]);
+            if (a < b) label[b] = a;
+            else if (b < a) label[a] = b;
+        }
+    for (int v = 0; v < n; v++) label[v] = uf_find(label, v);
+}
+
 /* ------------------------------------------------------------------ PageRank
  * Out-degree as CPUPR computes it: adjacent_difference of row_offsets
  * (PageRank.cu:121-125; PageRank.h:147-153 on the device). */
@@ -332,26 +361,31 @@ ORACLE_API int oracle_pr_jacobi_f32(int n, const int *row_offsets, const int *co
                                     int iters, float teleport, double tol, float *rank, double *l1) {
     int *deg = out_degrees(n, row_offsets);
     float *contrib = (float *)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+    double *dv = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
     const float base = teleport, damping = 1.0f - teleport;
     for (int i = 0; i < n; i++) rank[i] = base;
     int it = 0;
     double delta = 0;
     while (it < iters) {
+#pragma omp parallel for schedule(static)
         for (int u = 0; u < n; u++) contrib[u] = rank[u] / (float)deg[u];
-        delta = 0;
+#pragma omp parallel for schedule(dynamic, 4096)
         for (int v = 0; v < n; v++) {
             float s = 0.0f;
             for (int j = col_offsets[v]; j < col_offsets[v + 1]; ++j) s = s + contrib[row_idx[j]];
             float nv = base + damping * s;
-            delta += fabs((double)nv - (double)rank[v]);
+            dv[v] = fabs((double)nv - (double)rank[v]);
             rank[v] = nv;
         }
+        delta = 0;
+        for (int v = 0; v < n; v++) delta += dv[v];
         it++;
         if (delta < tol * (double)n) break;
     }
     if (l1) *l1 = delta;
     free(deg);
     free(contrib);
+    free(dv);
     return it;
 }
 
@@ -360,26 +394,31 @@ ORACLE_API int oracle_pr_jacobi_f64(int n, const int *row_offsets, const int *co
                                     int iters, double teleport, double tol, double *rank, double *l1) {
     int *deg = out_degrees(n, row_offsets);
     double *contrib = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double *dv = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
     const double base = teleport, damping = 1.0 - teleport;
     for (int i = 0; i < n; i++) rank[i] = base;
     int it = 0;
     double delta = 0;
     while (it < iters) {
+#pragma omp parallel for schedule(static)
         for (int u = 0; u < n; u++) contrib[u] = deg[u] ? rank[u] / (double)deg[u] : 0.0;
-        delta = 0;
+#pragma omp parallel for schedule(dynamic, 4096)
         for (int v = 0; v < n; v++) {
             double s = 0.0;
             for (int j = col_offsets[v]; j < col_offsets[v + 1]; ++j) s += contrib[row_idx[j]];
             double nv = base + damping * s;
-            delta += fabs(nv - rank[v]);
+            dv[v] = fabs(nv - rank[v]);
             rank[v] = nv;
         }
+        delta = 0;
+        for (int v = 0; v < n; v++) delta += dv[v];
         it++;
         if (delta < tol * (double)n) break;
     }
     if (l1) *l1 = delta;
     free(deg);
     free(contrib);
+    free(dv);
     return it;
 }
 

@@ -6,7 +6,12 @@
   Dijkstra and connected components.
 * The two SSSP restatements against each other, and brute force on tiny graphs.
 * PageRank: closed forms (2-cycle, star, chain), an exact float64 power iteration
-  with numpy, and the Gauss-Seidel fixed point.
+  with numpy, the Gauss-Seidel fixed point, and hand-derived known answers that fix
+  the program's activation rule (next frontier = out-neighbours of vertices that
+  moved by >= eps, Jacobi within an iteration) and the in-place sweep order of CPUPR.
+  Activating the changed vertices themselves, never deactivating, updating the
+  program's ranks in place, or sweeping CPUPR in Jacobi order each fail one of them.
+* The union-find CC checker against the CPUCC restatement and scipy.
 """
 import numpy as np
 import pytest
@@ -120,6 +125,25 @@ def test_cc_matches_scipy(mg, oracle, seed, symmetric):
     assert np.array_equal(lab, expect)
 
 
+@pytest.mark.parametrize("seed", [6, 7, 8])
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_cc_union_find_matches_cpucc(mg, oracle, seed, symmetric):
+    # the union-find checker used at scale 22 against the CPUCC restatement and scipy
+    g = random_graph(mg, 3000, 2500, seed, symmetric=symmetric)
+    lab = oracle.cc_union_find(g.row_offsets, g.col_idx)
+    assert np.array_equal(lab, oracle.cc(g.row_offsets, g.col_idx))
+    ncomp, comp = cg.connected_components(to_scipy(g), directed=True, connection="weak")
+    assert len(np.unique(lab)) == ncomp
+    for name in ("small.mtx", "trivialGraph.mtx", "chesapeake.mtx"):
+        f = load_mtx(mg, name)
+        assert np.array_equal(oracle.cc_union_find(f.row_offsets, f.col_idx), oracle.cc(f.row_offsets, f.col_idx))
+
+
+def test_cc_union_find_rmat(mg, oracle):
+    g = mg.rmat_graph(14, 8, seed=4)
+    assert np.array_equal(oracle.cc_union_find(g.row_offsets, g.col_idx), oracle.cc(g.row_offsets, g.col_idx))
+
+
 def test_bfs_brute_force_tiny(mg, oracle):
     rng = np.random.default_rng(11)
     for trial in range(20):
@@ -207,14 +231,83 @@ def test_pr_gauss_seidel_fixed_point(mg, oracle):
     assert np.max(np.abs(r - ref)) < 0.1
 
 
+def test_pr_gauss_seidel_sweep_order(oracle):
+    """CPUPR (PageRank.cu:136-156) updates ranks in place in vertex order, so a sweep
+    already sees the new ranks of lower ids.  Hand-derived sweeps:
+    chain 0 -> 1 -> 2: sweep 1 gives r1 = 0.15 + 0.85*0.15 = 0.2775 and then
+    r2 = 0.15 + 0.85*0.2775 = 0.385875 (a Jacobi sweep would give r2 = 0.2775); sweep 2
+    moves nothing, so 2 sweeps.
+    chain 2 -> 1 -> 0: sweep 1 reads the old ranks of higher ids: r0 = r1 = 0.2775,
+    r2 = 0.15; sweep 2: r0 = 0.385875 (moved by 0.108), sweep 3 moves nothing."""
+    g = _csr([(0, 1), (1, 2)], 3)
+    r1, s1 = oracle.pr_gauss_seidel(g.row_offsets, g.col_idx, max_sweeps=1)
+    assert s1 == 1 and np.allclose(r1, [0.15, 0.2775, 0.385875], rtol=0, atol=1e-6)
+    r, s = oracle.pr_gauss_seidel(g.row_offsets, g.col_idx)
+    assert s == 2 and np.allclose(r, [0.15, 0.2775, 0.385875], rtol=0, atol=1e-6)
+    g = _csr([(2, 1), (1, 0)], 3)
+    r1, _ = oracle.pr_gauss_seidel(g.row_offsets, g.col_idx, max_sweeps=1)
+    assert np.allclose(r1, [0.2775, 0.2775, 0.15], rtol=0, atol=1e-6)
+    r, s = oracle.pr_gauss_seidel(g.row_offsets, g.col_idx)
+    assert s == 3 and np.allclose(r, [0.385875, 0.2775, 0.15], rtol=0, atol=1e-6)
+
+
+def _activation_graph():
+    """a=0 -> z=1; z -> y=2 and z -> sinks 4..15 (out-degree 13); y -> x=3."""
+    edges = [(0, 1), (1, 2), (2, 3)] + [(1, s) for s in range(4, 16)]
+    return _csr(edges, 16)
+
+
+def test_pr_gas_activation_known_answers(oracle):
+    """The GPU PageRank program (PageRank.h:233-283; next frontier = out-neighbours of
+    the vertices whose rank moved by >= eps; Jacobi within an iteration), derived by hand
+    on _activation_graph with t = 0.15:
+    iteration 1 (all active): z = 0.15 + 0.85*0.15 = 0.2775 (moved, changed);
+      y = sinks = 0.15 + 0.85*0.15/13 = 0.1598077 (moved 0.0098 < 0.01: unchanged);
+      x = 0.15 + 0.85*0.15 = 0.2775 (changed; no out-edges); a = 0.15 (unchanged).
+      Next frontier = out(z) + out(x) = {y, sinks}.
+    iteration 2: y = sinks = 0.15 + 0.85*0.2775/13 = 0.1681442 (moved 0.0083: unchanged);
+      frontier empty: 2 iterations.  x keeps the stale 0.2775: y never moved by 0.01 at
+      once, so x is never recomputed from y's new rank (a converged Jacobi run would give
+      0.15 + 0.85*0.1681442).
+    Activating the changed vertices themselves, never deactivating, or updating in place
+    all give other ranks here."""
+    g = _activation_graph()
+    r, it = oracle.pr_gas(g.row_offsets, g.col_idx)
+    y2 = 0.15 + 0.85 * 0.2775 / 13
+    exp = np.array([0.15, 0.2775, y2, 0.2775] + [y2] * 12)
+    assert it == 2
+    assert np.allclose(r, exp, rtol=0, atol=1e-6), r
+    r1, it1 = oracle.pr_gas(g.row_offsets, g.col_idx, iter_num=1)
+    y1 = 0.15 + 0.85 * 0.15 / 13
+    assert it1 == 1
+    assert np.allclose(r1, [0.15, 0.2775, y1, 0.2775] + [y1] * 12, rtol=0, atol=1e-6)
+
+
+def test_pr_gas_activation_threshold(oracle):
+    """Same graph with eps = 0.005, by hand: iteration 1 as above but y and the sinks
+    (moved 0.0098) are changed too, so x joins the frontier.  Iteration 2: y = sinks =
+    0.1681442 (moved 0.0083, changed), x = 0.15 + 0.85*0.1598077 = 0.2858365 (moved
+    0.0083, changed).  Iteration 3: frontier = out(y) = {x}: x = 0.15 + 0.85*0.1681442 =
+    0.2929226 (moved 0.0071, changed; no out-edges).  Frontier empty: 3 iterations."""
+    g = _activation_graph()
+    r, it = oracle.pr_gas(g.row_offsets, g.col_idx, eps=0.005)
+    y2 = 0.15 + 0.85 * 0.2775 / 13
+    assert it == 3
+    exp = np.array([0.15, 0.2775, y2, 0.15 + 0.85 * y2] + [y2] * 12)
+    assert np.allclose(r, exp, rtol=0, atol=1e-6), r
+
+
 def test_pr_gas_semantics(mg, oracle):
     g = random_graph(mg, 300, 1200, 23, symmetric=False)
     rg, it = oracle.pr_gas(g.row_offsets, g.col_idx)
-    # the program stops when no vertex moves by >= 0.01: one more Jacobi step from
-    # rg moves nothing by 0.01 or more on any vertex with in-edges that changed
     assert it > 1
+    # every rank is within the activation threshold's reach of the fixed point
     ref, _, _ = oracle.pr_jacobi_f64(g.row_offsets, g.col_idx, iters=500)
     assert np.max(np.abs(rg - ref)) < 0.2
+    # at the stop no vertex whose in-neighbours changed in the last iteration moves by
+    # 0.01 or more: running once more from the final ranks leaves the frontier empty
+    r_more, it_more = oracle.pr_gas(g.row_offsets, g.col_idx, iter_num=it + 5)
+    assert it_more == it and np.array_equal(r_more, rg)
     # iter_num caps the iterations; one iteration equals one all-vertex Jacobi step
     r1, it1 = oracle.pr_gas(g.row_offsets, g.col_idx, iter_num=1)
     j1, _, _ = oracle.pr_jacobi_f32(g.row_offsets, g.col_idx, iters=1)
