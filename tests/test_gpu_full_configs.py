@@ -1,0 +1,200 @@
+"""GPU parity at the full sizes of the benchmark workloads.
+
+Each search runs on the device through the C-ABI and is compared with the CPU
+oracle on the same seeded graph:
+* R-MAT scale 16 (symmetrized): BFS from vertex 0.
+* R-MAT scale 22 (symmetrized, weights 1..64): SSSP from vertex 0, CC and BFS from 0.
+* R-MAT scale 22 (directed, dangling vertices kept): all-vertex PageRank, 50
+  iterations, FP32 -- normalised max-abs (p = r / n) <= 1e-6 against the FP32
+  Jacobi oracle and against the float64 oracle.
+* 2048 x 2048 grid (5% of edges deleted, weights 1..16): BFS and SSSP from the
+  corner, CC.
+* 256 batched BFS and SSSP sources on the 2048 x 2048 grid: sampled rows against
+  the oracle, every sampled row's distance certificate, and rows against the
+  single-source search.
+* Serving over the six resident graphs (R-MAT 16/18/20/22, grids 1024^2/2048^2):
+  a mixed stream of BFS / SSSP / personalized PageRank queries; sampled results are
+  identical to the query run alone.
+BFS depths, SSSP distances and CC labels are bit-exact."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+INT_MAX = 2**31 - 1
+PR_TOL = 1e-6  # max-abs in normalised units p = r / n
+
+
+@pytest.fixture(scope="module")
+def s22u(mg):
+    return mg.rmat_graph(22, 16, seed=42, weight_max=64)
+
+
+@pytest.fixture(scope="module")
+def g2048(mg):
+    return mg.grid_graph(2048, seed=42, weight_max=16)
+
+
+def _bfs_certificate(g, d, src):
+    """Exact BFS depths on a symmetric graph: d[src] = 0, adjacent reached vertices
+    differ by at most 1, components are reached whole, and every vertex at depth
+    k > 0 has a neighbour at k - 1."""
+    ro, ci = g.row_offsets, g.col_idx
+    rows = np.repeat(np.arange(g.n, dtype=np.int64), np.diff(ro))
+    d = d.astype(np.int64)
+    assert d[src] == 0
+    du, dv = d[rows], d[ci]
+    both = (du >= 0) & (dv >= 0)
+    assert np.all(np.abs(du[both] - dv[both]) <= 1)
+    assert np.all((du >= 0) == (dv >= 0))
+    nz = np.diff(ro) > 0
+    mn = np.full(g.n, 1 << 40)
+    mn[nz] = np.minimum.reduceat(np.where(dv >= 0, dv, 1 << 40), ro[:-1][nz])
+    assert np.all(mn[d > 0] == d[d > 0] - 1)
+
+
+def _sssp_certificate(g, d, src):
+    """Exact shortest distances (positive weights): d[src] = 0, no edge shortens a
+    distance, and every reached vertex other than the source has a tight in-edge."""
+    ro, ci, w = g.row_offsets, g.col_idx, g.values.astype(np.int64)
+    rows = np.repeat(np.arange(g.n, dtype=np.int64), np.diff(ro))
+    d = d.astype(np.int64)
+    assert d[src] == 0
+    dv, du = d[rows], d[ci]
+    ok = dv != INT_MAX
+    assert np.all(du[ok] <= dv[ok] + w[ok])
+    tight = np.zeros(g.n, bool)
+    tight[ci[ok & (du == dv + w)]] = True
+    reached = d != INT_MAX
+    reached[src] = False
+    assert np.all(tight[reached])
+
+
+def test_rmat16_bfs_from_0_full_config(mg, oracle, device):
+    g = mg.rmat_graph(16, 16, seed=42)
+    with mg.Graph(g, symmetric=True) as dg:
+        got, st = dg.bfs(0, stats=True)
+    exp = oracle.bfs(g.row_offsets, g.col_idx, 0)
+    assert np.array_equal(got, exp)
+    assert st.edges_traversed == oracle.teps_edges(g.row_offsets, exp)
+
+
+def test_rmat22_sssp_cc_bfs_from_0(mg, oracle, device, s22u):
+    g = s22u
+    print(f"rmat22 symmetric: n={g.n} m={g.m}", flush=True)
+    with mg.Graph(g, symmetric=True) as dg:
+        d, st = dg.sssp(0, stats=True)
+        exp = oracle.sssp(g.row_offsets, g.col_idx, g.values, 0)
+        bad = np.nonzero(d != exp)[0]
+        assert bad.size == 0, f"SSSP mismatch at {bad[:10].tolist()}"
+        assert st.edges_traversed == oracle.teps_edges(g.row_offsets, exp, INT_MAX)
+        assert st.vertices_reached == int((exp != INT_MAX).sum())
+        lab, stc = dg.cc(stats=True)
+        exp_cc = oracle.cc_union_find(g.row_offsets, g.col_idx)
+        bad = np.nonzero(lab != exp_cc)[0]
+        assert bad.size == 0, f"CC mismatch at {bad[:10].tolist()}"
+        b, stb = dg.bfs(0, stats=True)
+        exp_b = oracle.bfs(g.row_offsets, g.col_idx, 0)
+        assert np.array_equal(b, exp_b)
+        assert stb.edges_traversed == oracle.teps_edges(g.row_offsets, exp_b)
+        assert stb.bottom_up_levels > 0
+        # the same searches again (the step-count hint of a repeated search) and from
+        # other sources, including an isolated vertex
+        assert np.array_equal(dg.sssp(0), exp)
+        iso = int(np.argmin(g.degrees()))
+        for s in [int(x) for x in mg.pick_sources(g, 2, seed=5)] + [iso]:
+            assert np.array_equal(dg.sssp(s), oracle.sssp(g.row_offsets, g.col_idx, g.values, s)), s
+            assert np.array_equal(dg.bfs(s), oracle.bfs(g.row_offsets, g.col_idx, s)), s
+
+
+def test_rmat22_directed_pagerank_50_iterations(mg, oracle, device):
+    g = mg.rmat_graph(22, 16, seed=42, symmetric=False)
+    print(f"rmat22 directed: n={g.n} m={g.m}", flush=True)
+    with mg.Graph(g, build_csc=True) as dg:
+        got, st = dg.pagerank(mg.options(pr_mode=mg.MG_PR_ALL, iter_num=50, pr_tol=1e-6), stats=True)
+    exp, it, _ = oracle.pr_jacobi_f32(g.row_offsets, g.col_idx, iters=50, tol=1e-6)
+    assert it == 50 and st.iterations == 50  # the L1 stop at 1e-6 * n never triggers in FP32
+    err32 = float(np.max(np.abs(got.astype(np.float64) - exp))) / g.n
+    ref64, _, _ = oracle.pr_jacobi_f64(g.row_offsets, g.col_idx, iters=50)
+    err64 = float(np.max(np.abs(got.astype(np.float64) - ref64))) / g.n
+    oerr64 = float(np.max(np.abs(exp.astype(np.float64) - ref64))) / g.n
+    print(f"pagerank s22d: normalised max-abs vs FP32 oracle {err32:.3e}, vs FP64 oracle {err64:.3e} "
+          f"(FP32 oracle vs FP64 oracle {oerr64:.3e})", flush=True)
+    assert err32 <= PR_TOL
+    assert err64 <= PR_TOL
+
+
+def test_grid2048_bfs_sssp_cc_from_corner(mg, oracle, device, g2048):
+    g = g2048
+    with mg.Graph(g, symmetric=True) as dg:
+        d, st = dg.bfs(0, stats=True)
+        exp = oracle.bfs(g.row_offsets, g.col_idx, 0)
+        assert np.array_equal(d, exp)
+        assert st.iterations >= exp.max() > 4000
+        s, sts = dg.sssp(0, stats=True)
+        exp_s = oracle.sssp(g.row_offsets, g.col_idx, g.values, 0)
+        assert np.array_equal(s, exp_s)
+        assert sts.edges_traversed == oracle.teps_edges(g.row_offsets, exp_s, INT_MAX)
+        assert np.array_equal(dg.cc(), oracle.cc_union_find(g.row_offsets, g.col_idx))
+        # the opposite corner and the centre
+        for src in (g.n - 1, 1024 * 2048 + 1024):
+            assert np.array_equal(dg.sssp(src), oracle.sssp(g.row_offsets, g.col_idx, g.values, src)), src
+
+
+def test_grid2048_batched_256_sources(mg, oracle, device, g2048):
+    g = g2048
+    srcs = mg.pick_sources(g, 256, seed=42)
+    with mg.Graph(g, symmetric=True) as dg:
+        got, st = dg.bfs_batch(srcs, stats=True)
+        gs, sts = dg.sssp_batch(srcs, stats=True)
+        for j in (0, 255):
+            assert np.array_equal(got[j], dg.bfs(int(srcs[j]))), j
+            assert np.array_equal(gs[j], dg.sssp(int(srcs[j]))), j
+    for j in (1, 100, 200):
+        assert np.array_equal(got[j], oracle.bfs(g.row_offsets, g.col_idx, int(srcs[j]))), j
+        assert np.array_equal(gs[j], oracle.sssp(g.row_offsets, g.col_idx, g.values, int(srcs[j]))), j
+    for j in range(3, 256, 32):
+        _bfs_certificate(g, got[j], int(srcs[j]))
+        _sssp_certificate(g, gs[j], int(srcs[j]))
+    deg = np.diff(g.row_offsets).astype(np.int64)
+    assert st.vertices_reached == int((got >= 0).sum())
+    assert st.edges_traversed == sum(int(deg[r >= 0].sum()) for r in got)
+    assert sts.vertices_reached == int((gs != INT_MAX).sum())
+
+
+def test_serving_six_resident_graphs(mg, device):
+    hosts = [mg.rmat_graph(s, 16, seed=42 + s, weight_max=64) for s in (16, 18, 20, 22)]
+    hosts += [mg.grid_graph(w, seed=42 + w, weight_max=16) for w in (1024, 2048)]
+    dgs = [mg.Graph(h, symmetric=True) for h in hosts]
+    rng = np.random.default_rng(7)
+    nq = 240
+    gsel = rng.integers(0, len(hosts), nq)
+    kinds = rng.choice([mg.MG_Q_BFS, mg.MG_Q_SSSP, mg.MG_Q_PPR], p=[0.6, 0.3, 0.1], size=nq)
+    pools = [mg.pick_sources(h, 64, seed=11 + i) for i, h in enumerate(hosts)]
+    srcs = [int(pools[gi][i % 64]) for i, gi in enumerate(gsel)]
+    checked = list(range(0, nq, 6))  # results kept for these queries, discarded for the rest
+    try:
+        with mg.Server() as srv:
+            ids = [srv.add_graph(d) for d in dgs]
+            outs = {}
+            qids = []
+            for i in range(nq):
+                out = None
+                if i in checked:
+                    out = mg.PinnedArray(hosts[gsel[i]].n, np.float32 if kinds[i] == mg.MG_Q_PPR else np.int32)
+                    outs[i] = out
+                qids.append(srv.submit(ids[gsel[i]], int(kinds[i]), srcs[i], out))
+            infos = [srv.wait(q, 600000) for q in qids]
+            st = srv.stats()
+        assert st["completed"] == nq and all(i["status"] == 0 for i in infos)
+        for i in checked:
+            dg, kind, src = dgs[gsel[i]], int(kinds[i]), srcs[i]
+            if kind == mg.MG_Q_BFS:
+                assert np.array_equal(outs[i].array, dg.bfs(src)), i
+            elif kind == mg.MG_Q_SSSP:
+                assert np.array_equal(outs[i].array, dg.sssp(src)), i
+            else:
+                alone = dg.ppr_batch([src])[0]
+                assert np.array_equal(outs[i].array.view(np.int32), alone.view(np.int32)), i
+    finally:
+        for d in dgs:
+            d.close()
