@@ -570,7 +570,6 @@ __device__ __forceinline__ void ctl_reset(Ctl *c, int n, int iter_limit, int fro
     c->edges_examined = 0;
     c->dsum = 0;
     c->mf_cur = 0;
-    c->dense = 0;
 }
 
 // Advances the control block past level L given the next frontier's size nf and
@@ -602,11 +601,6 @@ int ensure_pull_tasks(mg_graph *g, const int *d_in_off);
 // {-1, va, vb} = the vertices [va, vb) (va a multiple of 32, vb too or n, at most
 // DENSE_V of them) whose in-lists are at most DENSE_BIG long, about DENSE_T edges in all.
 int ensure_dense_tasks(mg_graph *g, const int *d_in_off);
-// Per-slot owner array of the in-edge view (d_in_off) for edge-centric pulls.
-int ensure_edge_dst(mg_graph *g, const int *d_in_off);
-// Builds the XCD-sliced pull structures for the in-edge view (d_in_off, d_in_adj);
-// ws.slice_ok reports whether every in-list is sorted (slicing needs it).
-int ensure_pull_slices(mg_graph *g, const int *d_in_off, const int *d_in_adj);
 // Prints the per-level trace of the search that just finished to stderr (the
 // reference's verbose per-iteration frontier sizes).
 int print_trace(mg_graph *g, const char *what, const char *dir0, const char *dir1);

@@ -75,37 +75,6 @@ FrontierArgs make_frontier_args(mg_graph *g, int directed) {
     return fa;
 }
 
-// Builds (once per graph and in-edge view) the TASK-edge pieces of every in-edge list
-// longer than LANE_T, for pull steps; the host walks the offsets once.
-// Owner vertex of every in-edge slot (the COO row of the in-edge view), built on
-// the host from the offsets, for pulls that walk edges rather than vertices.
-int ensure_edge_dst(mg_graph *g, const int *d_in_off) {
-    Workspace &ws = g->ws;
-    if (ws.edge_dst_view == d_in_off) return MG_OK;
-    const int n = g->n;
-    const int64_t m = g->m;
-    std::vector<int32_t> off((size_t)n + 1);
-    MG_HIP(hipMemcpy(off.data(), d_in_off, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
-    std::vector<int32_t> dst((size_t)std::max<int64_t>(m, 1));
-    const int nt = std::max(1, std::min(host_threads(), 64));
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t)
-        th.emplace_back([&, t] {
-            const int lo = (int)((int64_t)n * t / nt), hi = (int)((int64_t)n * (t + 1) / nt);
-            for (int v = lo; v < hi; ++v)
-                for (int e = off[v]; e < off[v + 1]; ++e) dst[e] = v;
-        });
-    for (auto &x : th) x.join();
-    free_dev(ws.edge_dst);
-    ws.edge_dst = nullptr;
-    ws.edge_dst_view = nullptr;
-    int rc = alloc_i32(&ws.edge_dst, dst.size());
-    if (rc) return rc;
-    if (m) MG_HIP(hipMemcpy(ws.edge_dst, dst.data(), 4 * (size_t)m, hipMemcpyHostToDevice));
-    ws.edge_dst_view = d_in_off;
-    return MG_OK;
-}
-
 // Static pull tasks: in-lists longer than `longer` cut into PULL_TASK-slot pieces.
 static int build_pull_tasks(mg_graph *g, const int *d_in_off, int longer, ChunkRec **tasks, int *ntasks,
                             const int32_t **view) {
@@ -173,98 +142,6 @@ int ensure_dense_tasks(mg_graph *g, const int *d_in_off) {
     if (!t.empty()) MG_HIP(hipMemcpy(ws.dense_tasks, t.data(), sizeof(ChunkRec) * t.size(), hipMemcpyHostToDevice));
     ws.dense_ntasks = (int)t.size();
     ws.dense_view = d_in_off;
-    return MG_OK;
-}
-
-// XCD-sliced pull.  Source vertices are cut into PULL_SLICES slices of slice_len
-// (a multiple of 64, so frontier bitmap words never straddle slices).  For every
-// vertex the sorted in-list splits into PULL_SLICES contiguous ranges, one per
-// source slice; ranges longer than LANE_T become TASK-edge tasks of their slice.
-int ensure_pull_slices(mg_graph *g, const int *d_in_off, const int *d_in_adj) {
-    Workspace &ws = g->ws;
-    if (ws.slice_view == d_in_off) return MG_OK;
-    const int n = g->n;
-    const int64_t m = g->m;
-    std::vector<int32_t> off((size_t)n + 1), adj((size_t)std::max<int64_t>(m, 1));
-    MG_HIP(hipMemcpy(off.data(), d_in_off, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
-    if (m) MG_HIP(hipMemcpy(adj.data(), d_in_adj, 4 * (size_t)m, hipMemcpyDeviceToHost));
-    const int SL = (int)((((int64_t)n + PULL_SLICES - 1) / PULL_SLICES + 63) / 64 * 64);
-    std::vector<int32_t> cut((size_t)(PULL_SLICES - 1) * n);
-    const int nt = std::max(1, std::min(host_threads(), 64));
-    std::vector<std::vector<ChunkRec>> tasks((size_t)nt * PULL_SLICES);
-    std::vector<char> sorted_ok(nt, 1);
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t)
-        th.emplace_back([&, t] {
-            const int lo = (int)((int64_t)n * t / nt), hi = (int)((int64_t)n * (t + 1) / nt);
-            for (int v = lo; v < hi; ++v) {
-                const int b = off[v], e = off[v + 1];
-                for (int p = b + 1; p < e; ++p)
-                    if (adj[p] < adj[p - 1]) {
-                        sorted_ok[t] = 0;
-                        break;
-                    }
-                int c[PULL_SLICES + 1];
-                c[0] = b;
-                c[PULL_SLICES] = e;
-                int p = b;
-                for (int x = 1; x < PULL_SLICES; ++x) {
-                    const int64_t lim = (int64_t)x * SL;
-                    while (p < e && adj[p] < lim) ++p;
-                    c[x] = p;
-                    cut[(size_t)(x - 1) * n + v] = p;
-                }
-                for (int x = 0; x < PULL_SLICES; ++x)
-                    if (c[x + 1] - c[x] > LANE_T)
-                        for (int q = c[x]; q < c[x + 1]; q += TASK)
-                            tasks[(size_t)t * PULL_SLICES + x].push_back(ChunkRec{v, q, std::min(q + TASK, c[x + 1]), x});
-            }
-        });
-    for (auto &x : th) x.join();
-    ws.slice_ok = true;
-    for (char ok : sorted_ok) ws.slice_ok = ws.slice_ok && ok;
-    if (!ws.slice_ok) {
-        ws.slice_view = d_in_off;  // unsorted lists: the plain pull stays
-        return MG_OK;
-    }
-    std::vector<ChunkRec> all, segs;
-    int32_t sto[PULL_SLICES + 1];
-    for (int x = 0; x < PULL_SLICES; ++x) {
-        sto[x] = (int32_t)all.size();
-        for (int t = 0; t < nt; ++t) {
-            const auto &v = tasks[(size_t)t * PULL_SLICES + x];
-            all.insert(all.end(), v.begin(), v.end());
-        }
-    }
-    sto[PULL_SLICES] = (int32_t)all.size();
-    // the pieces of one (vertex, slice) range are consecutive: one segment each
-    for (size_t i = 0; i < all.size();) {
-        size_t j = i + 1;
-        while (j < all.size() && all[j].v == all[i].v && all[j].pad == all[i].pad) ++j;
-        segs.push_back(ChunkRec{all[i].v, (int)i, (int)(j - i), all[i].pad});
-        i = j;
-    }
-    free_dev(ws.slice_off);
-    free_dev(ws.slice_min);
-    free_dev(ws.slice_tasks);
-    free_dev(ws.slice_segs);
-    free_dev(ws.slice_part);
-    ws.slice_off = ws.slice_min = nullptr;
-    ws.slice_tasks = ws.slice_segs = nullptr;
-    ws.slice_part = nullptr;
-    ws.slice_view = nullptr;
-    int rc;
-    if ((rc = alloc_i32(&ws.slice_off, cut.size())) || (rc = alloc_i32(&ws.slice_min, (size_t)PULL_SLICES * n)) ||
-        (rc = alloc_chunks(&ws.slice_tasks, all.size())) || (rc = alloc_chunks(&ws.slice_segs, segs.size())) ||
-        (rc = alloc_f32(&ws.slice_part, all.size())))
-        return rc;
-    if (!cut.empty()) MG_HIP(hipMemcpy(ws.slice_off, cut.data(), 4 * cut.size(), hipMemcpyHostToDevice));
-    if (!all.empty()) MG_HIP(hipMemcpy(ws.slice_tasks, all.data(), sizeof(ChunkRec) * all.size(), hipMemcpyHostToDevice));
-    if (!segs.empty()) MG_HIP(hipMemcpy(ws.slice_segs, segs.data(), sizeof(ChunkRec) * segs.size(), hipMemcpyHostToDevice));
-    ws.nslice_segs = (int)segs.size();
-    for (int x = 0; x <= PULL_SLICES; ++x) ws.slice_task_off[x] = sto[x];
-    ws.slice_len = SL;
-    ws.slice_view = d_in_off;
     return MG_OK;
 }
 

@@ -9,8 +9,8 @@
 //   * push  (sparse frontiers) relaxes the frontier's out-edges straight into dist
 //           with atomicMin and marks improved targets in the next-frontier bitmap;
 //   * pull  (dense frontiers, when an in-edge view exists) lets every vertex take
-//           the minimum of dist[u] + w over its in-edges from frontier vertices:
-//           no atomics, edges streamed once, the minimum reduced in LDS per tile.
+//           the minimum of dist[u] + w over all its in-edges (see dense_pull):
+//           edges streamed once, the minimum reduced in registers and LDS.
 // Both are Bellman-Ford relaxations; any vertex whose distance drops during a level
 // is in the next frontier, so the fixed point is the same.  Integer shortest
 // distances are unique, so the result equals the reference's: dist[src] = 0,
@@ -29,23 +29,11 @@ struct SsspArgs {
     FrontierArgs fa;
     int *dist;
     const int *w;
-    const ChunkRec *pull;  // static pull tasks (in-edge lists longer than LANE_T)
-    int npull;
-    int G;                 // a multiple of PULL_SLICES
-    // XCD-sliced pull (sliced = 1): workgroup b gathers from source slice b % 8 only
-    int sliced;
-    const int *slice_off;        // (PULL_SLICES - 1) x n cut points in the in-lists
-    int *slice_min;              // PULL_SLICES x n partial minima
-    const ChunkRec *slice_tasks; // long ranges, grouped by slice
-    int sto[PULL_SLICES + 1];    // slice task offsets
-    // dense pull levels: the dense tasks, all in one launch (null: dense levels use the
-    // LANE_T split above)
+    int G;                 // workgroups of the whole-GPU level kernels
+    // pull levels: the dense tasks, all in one launch
     const ChunkRec *dpull;
     int ndpull;
     int push_range;  // symmetric graph with dense tasks: wide push levels use range_push
-    // dense pull: a pull level whose frontier holds more than dense_pct % of the edges
-    // reads every in-neighbour's distance without the frontier test (> 100: never)
-    int dense_pct;
     // buckets of the small-frontier loop: width, far list (n entries) and its bitmap
     long long delta;
     int *far;
@@ -136,18 +124,16 @@ struct SsspPush {
     }
 };
 
-// Pull: the best dist[u] + w over in-edge slots [beg, end) from frontier vertices u.
-// Loads go out PULL_U at a time: neighbour ids, then their frontier bits, then the
-// distances and weights of the frontier ones (three dependent round trips per batch).
-// Dense levels (ALL) skip the frontier test and take every in-neighbour: a vertex u
-// outside the frontier has relaxed its out-edges with its present distance already
-// (its last change put it in a frontier), so its candidate dist[u] + w cannot beat
-// dist[v] and the minimum is unchanged; unreached u give INT_MAX + w.  That trades
-// a bitmap gather per edge for a distance gather per non-frontier edge, a gain once
-// the frontier holds most edges.
+// Pull: the best dist[u] + w over all in-edge slots [beg, end), PULL_U loads in
+// flight (neighbour ids, then their distances and weights).  No frontier test: a
+// vertex u outside the frontier has relaxed its out-edges with its present distance
+// already (its last change put it in a frontier), so its candidate dist[u] + w cannot
+// beat dist[v] and the minimum is unchanged; unreached u give INT_MAX + w.  That
+// trades a bitmap gather per edge for a distance gather per non-frontier edge, a gain
+// since pull levels run only when the frontier holds most edges.
 constexpr int PULL_U = 8;
-template <bool ALL, bool EID>
-__device__ __forceinline__ long long pull_best_t(const SsspArgs &a, const uint32_t *fr, int beg, int end, int step) {
+template <bool EID>
+__device__ __forceinline__ long long pull_best(const SsspArgs &a, int beg, int end, int step) {
     long long best = INT_MAX;
     const int *adj = a.fa.adj[2];
     const int *eid = a.fa.eid[2];
@@ -171,33 +157,16 @@ __device__ __forceinline__ long long pull_best_t(const SsspArgs &a, const uint32
 #pragma unroll
             for (int k = 0; k < PULL_U; ++k) wt[k] = a.w[sl[k]];
         }
-        long long cand[PULL_U];
-        if (ALL) {
-            int du[PULL_U];
+        int du[PULL_U];
 #pragma unroll
-            for (int k = 0; k < PULL_U; ++k) du[k] = a.dist[u[k]];
+        for (int k = 0; k < PULL_U; ++k) du[k] = a.dist[u[k]];
 #pragma unroll
-            for (int k = 0; k < PULL_U; ++k) cand[k] = ok[k] ? (long long)du[k] + wt[k] : (long long)INT_MAX;
-        } else {
-            uint32_t wd[PULL_U];
-#pragma unroll
-            for (int k = 0; k < PULL_U; ++k) wd[k] = fr[u[k] >> 5];
-#pragma unroll
-            for (int k = 0; k < PULL_U; ++k)
-                cand[k] = ok[k] && ((wd[k] >> (u[k] & 31)) & 1u) ? (long long)a.dist[u[k]] + wt[k] : (long long)INT_MAX;
+        for (int k = 0; k < PULL_U; ++k) {
+            const long long cand = ok[k] ? (long long)du[k] + wt[k] : (long long)INT_MAX;
+            best = cand < best ? cand : best;
         }
-#pragma unroll
-        for (int k = 0; k < PULL_U; ++k) best = cand[k] < best ? cand[k] : best;
     }
     return best;
-}
-template <bool ALL>
-__device__ __forceinline__ long long pull_best(const SsspArgs &a, const uint32_t *fr, int beg, int end, int step) {
-    return a.fa.eid[2] ? pull_best_t<ALL, true>(a, fr, beg, end, step) : pull_best_t<ALL, false>(a, fr, beg, end, step);
-}
-__device__ __forceinline__ long long pull_best(const SsspArgs &a, const uint32_t *fr, int beg, int end, int step,
-                                               bool all) {
-    return all ? pull_best<true>(a, fr, beg, end, step) : pull_best<false>(a, fr, beg, end, step);
 }
 
 // Dense pull level: every vertex takes min(dist[u] + w) over all its in-edges (no
@@ -377,7 +346,7 @@ __device__ __forceinline__ void dense_pull(const SsspArgs &a, int L, DenseLds &l
             continue;
         }
         ex += (r.end - r.beg + WAVE - 1 - lane) / WAVE;
-        long long best = pull_best_t<true, EID>(a, nullptr, r.beg + lane, r.end, WAVE);
+        long long best = pull_best<EID>(a, r.beg + lane, r.end, WAVE);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             const long long y = __shfl_xor(best, o);
@@ -522,7 +491,6 @@ __device__ void sssp_close(const SsspArgs &a, int L, long long nf, long long mf)
         else if (dir == DIR_PULL && mf * c->beta < m) dir = DIR_PUSH;
     }
     c->dir = dir;
-    c->dense = dir == DIR_PULL && mf * 100 > c->m_unexplored * a.dense_pct;
     advance_level(c, L, nf, mf);
 }
 
@@ -915,7 +883,6 @@ __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
             const int dir = s_canpull && mft * s_alpha > s_m ? DIR_PULL : DIR_PUSH;
             c->edges_examined += ex;
             c->dir = dir;
-            c->dense = dir == DIR_PULL && mft * 100 > s_m * a.dense_pct;
             c->small_levels += s_levels;
             c->thr = s_thr;
             advance_level(c, L, nft, mft);
@@ -942,66 +909,7 @@ __global__ __launch_bounds__(BLK) void sssp_level_a(SsspArgs a, int view) {
         write_part<BLK>(fa, pol.nf, pol.mf, pol.ex);
         return;
     }
-    if (c->dense && a.dpull) {
-        return;  // the level is sssp_level_b's
-    }
-    if (a.sliced) {
-        // XCD-sliced pull: this workgroup scans, for every vertex, the part of its
-        // in-list whose sources lie in slice x = blockIdx % 8, so the dist[] and
-        // frontier-bit gathers stay inside one 1/8 of the arrays, resident in this
-        // XCD's L2.  Ranges longer than LANE_T are sssp_level_b tasks.
-        const uint32_t *fr = fa.bm[L % 3];
-        const int *off = fa.off[2];
-        const int x = blockIdx.x % PULL_SLICES, nbx = gridDim.x / PULL_SLICES;
-        int *pm = a.slice_min + (int64_t)x * fa.n;
-        const int *c0 = x == 0 ? nullptr : a.slice_off + (int64_t)(x - 1) * fa.n;
-        const int *c1 = x == PULL_SLICES - 1 ? nullptr : a.slice_off + (int64_t)x * fa.n;
-        long long ex = 0;
-        for (int base = (blockIdx.x / PULL_SLICES) * BLK; base < fa.n; base += nbx * BLK) {
-            const int v = base + (int)threadIdx.x;
-            if (v < fa.n) {
-                const int beg = c0 ? c0[v] : off[v], end = c1 ? c1[v] : off[v + 1];
-                long long best = INT_MAX;
-                if (end - beg <= LANE_T) {
-                    ex += end - beg;
-                    best = pull_best<false>(a, fr, beg, end, 1);
-                }
-                pm[v] = (int)best;  // long ranges stay INT_MAX until their tasks
-            }
-        }
-        write_part<BLK>(fa, 0, 0, ex);
-        return;
-    }
-    // pull, vertices of in-degree <= LANE_T, one lane each in vertex order; each
-    // wave owns the two bitmap words of its 64 vertices.  Longer in-edge lists are
-    // the static tasks of sssp_level_b.
-    const uint32_t *fr = fa.bm[L % 3];
-    uint32_t *nx = fa.bm[(L + 1) % 3];
-    const int *off = fa.off[2];
-    const bool all = c->dense;
-    long long nf = 0, mf = 0, ex = 0;
-    const int lane = lane_id();
-    for (int base = blockIdx.x * BLK; base < fa.n; base += gridDim.x * BLK) {
-        const int v = base + (int)threadIdx.x;
-        bool found = false;
-        if (v < fa.n) {
-            const int beg = off[v], end = off[v + 1];
-            if (end - beg <= LANE_T) {
-                ex += end - beg;
-                const long long best = pull_best(a, fr, beg, end, 1, all);
-                if (best < a.dist[v]) {
-                    a.dist[v] = (int)best;
-                    found = true;
-                    ++nf;
-                    mf += view_degree(fa, v);
-                }
-            }
-        }
-        const uint64_t fm = wave_ballot(found);
-        const uint32_t mine = (lane < 32) ? (uint32_t)fm : (uint32_t)(fm >> 32);
-        if ((lane & 31) == 0 && mine) nx[v >> 5] = mine;
-    }
-    write_part<BLK>(fa, nf, mf, ex);
+    // a pull level is sssp_level_b's (the dense tasks, one launch)
 }
 
 __global__ __launch_bounds__(BLK) void sssp_level_b(SsspArgs a) {
@@ -1019,92 +927,73 @@ __global__ __launch_bounds__(BLK) void sssp_level_b(SsspArgs a) {
         nf = pol.nf;
         mf = pol.mf;
         ex = pol.ex;
-    } else if (c->dense && a.dpull) {
-        if (fa.eid[2]) dense_pull<true>(a, L, lds, nf, mf, ex);
-        else dense_pull<false>(a, L, lds, nf, mf, ex);
-    } else if (a.sliced) {
-        // long (vertex, slice) ranges of this workgroup's slice: wave minimum, one
-        // atomicMin into the slice's partial
-        const uint32_t *fr = fa.bm[L % 3];
-        const int lane = lane_id();
-        const int x = blockIdx.x % PULL_SLICES;
-        const int wave = (blockIdx.x / PULL_SLICES) * (BLK / WAVE) + (int)threadIdx.x / WAVE;
-        const int nwaves = (gridDim.x / PULL_SLICES) * (BLK / WAVE);
-        int *pm = a.slice_min + (int64_t)x * fa.n;
-        for (int t = a.sto[x] + wave; t < a.sto[x + 1]; t += nwaves) {
-            const ChunkRec r = a.slice_tasks[t];
-            ex += (r.end - r.beg + WAVE - 1 - lane) / WAVE;
-            long long best = pull_best<false>(a, fr, r.beg + lane, r.end, WAVE);
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const long long y = __shfl_xor(best, o);
-                best = y < best ? y : best;
-            }
-            if (lane == 0 && best < INT_MAX) atomicMin(&pm[r.v], (int)best);
-        }
+    } else if (fa.eid[2]) {
+        dense_pull<true>(a, L, lds, nf, mf, ex);
     } else {
-        // static in-edge tasks: wave minimum, one atomicMin per task
-        const uint32_t *fr = fa.bm[L % 3];
-        const bool all = c->dense;
-        const ChunkRec *tasks = a.pull;
-        const int ntasks = a.npull;
-        const int lane = lane_id();
-        const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
-        for (int t = wave; t < ntasks; t += nwaves) {
-            const ChunkRec r = tasks[t];
-            ex += (r.end - r.beg + WAVE - 1 - lane) / WAVE;
-            long long best = pull_best(a, fr, r.beg + lane, r.end, WAVE, all);
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const long long y = __shfl_xor(best, o);
-                best = y < best ? y : best;
-            }
-            if (lane == 0 && best < a.dist[r.v]) {
-                const int old = atomicMin(&a.dist[r.v], (int)best);
-                if (best < old && set_bit_first(fa.bm[(L + 1) % 3], r.v)) {
-                    ++nf;
-                    mf += view_degree(fa, r.v);
-                }
-            }
-        }
+        dense_pull<false>(a, L, lds, nf, mf, ex);
     }
     write_part<BLK>(fa, nf, mf, ex);
     if (blockIdx.x == 0 && threadIdx.x == 0) c->pending = 1;
 }
 
-// Sliced pull levels only: dist[v] = min(dist[v], the slices' partial minima);
-// improved vertices form the next frontier (each wave writes its 64 vertices' words).
-__global__ __launch_bounds__(BLK) void sssp_level_c(SsspArgs a) {
-    Ctl *c = a.fa.ctl;
-    const FrontierArgs &fa = a.fa;
-    if (c->done || c->dir != DIR_PULL || !a.sliced) return;
-    uint32_t *nx = fa.bm[(c->level + 1) % 3];
-    const int lane = lane_id();
-    long long nf = 0, mf = 0;
-    for (int base = blockIdx.x * BLK; base < fa.n; base += gridDim.x * BLK) {
-        const int v = base + (int)threadIdx.x;
-        bool found = false;
-        if (v < fa.n) {
-            int best = a.slice_min[v];
-#pragma unroll
-            for (int x = 1; x < PULL_SLICES; ++x) best = min(best, a.slice_min[(int64_t)x * fa.n + v]);
-            if (best < a.dist[v]) {
-                a.dist[v] = best;
-                found = true;
-                ++nf;
-                mf += view_degree(fa, v);
-            }
-        }
-        const uint64_t fm = wave_ballot(found);
-        const uint32_t mine = (lane < 32) ? (uint32_t)fm : (uint32_t)(fm >> 32);
-        if ((lane & 31) == 0 && mine) nx[v >> 5] = mine;
-    }
-    write_part<BLK>(fa, nf, mf, 0);
-}
-
 }  // namespace mg
 
 using namespace mg;
+
+namespace mg {
+
+// One SSSP search with the current queues (MG_ERR_OVERFLOW when the edge-chunk queue
+// overflowed; the caller regrows it and runs again).
+static int sssp_run(mg_graph *g, int src, int *dist_out, const mg_options &o, mg_stats *stats,
+                    const CallTimer &timer) {
+    int rc;
+    Workspace &ws = g->ws;
+    SsspArgs a{};
+    a.fa = make_frontier_args(g, o.directed);
+    a.dist = ws.vals;
+    a.w = g->w;
+    a.G = grid_blocks_cap(std::max<int64_t>(g->n / 4, g->m / 64), BLK, BIG_GRID);
+    // bucket width of the small-frontier loop: the option, else four maximum edge weights
+    // (a frontier vertex's improvements then stay within the next few buckets; wider
+    // buckets mean fewer, fuller rounds); sssp_delta < 0: unbucketed
+    a.delta = o.sssp_delta < 0 ? LLONG_MAX / 4
+                               : o.sssp_delta > 0 ? (long long)o.sssp_delta
+                                                  : std::max<long long>(1, 4LL * g->max_weight);
+    a.far = ws.hard;
+    a.farbm = ws.visited;
+    const int can_pull = a.fa.off[2] != nullptr && a.fa.nviews == 1 && o.engine != MG_ENGINE_GAS;
+    if (can_pull) {
+        if ((rc = ensure_dense_tasks(g, a.fa.off[2]))) return rc;
+        a.dpull = ws.dense_tasks;
+        a.ndpull = ws.dense_ntasks;
+        // the tasks cut the in-lists; on a symmetric graph those are the out-lists too
+        a.push_range = a.fa.off[2] == a.fa.off[0] && a.fa.nviews == 1 && a.fa.eid[0] == nullptr;
+    }
+
+    MG_HIP(hipEventRecord(g->ev0, g->stream));
+    hipLaunchKernelGGL(sssp_begin, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, g->stream, a, src, o.iter_num, can_pull,
+                       (long long)g->m);
+    MG_HIP(hipGetLastError());
+    auto step = [&]() -> int {
+        hipLaunchKernelGGL(sssp_small, dim3(1), dim3(SMALL_BLK), 0, g->stream, a);
+        for (int v = 0; v < a.fa.nviews; ++v)
+            hipLaunchKernelGGL(sssp_level_a, dim3(a.G), dim3(BLK), 0, g->stream, a, v);
+        hipLaunchKernelGGL(sssp_level_b, dim3(a.G), dim3(BLK), 0, g->stream, a);
+        MG_HIP(hipGetLastError());
+        return MG_OK;
+    };
+    // Unbucketed Bellman-Ford with non-negative weights settles within n rounds and a
+    // step runs at least one round; the bucketed narrow loop runs many rounds per step.
+    // Twice that bound leaves room; a search that still runs then is reported, not cut.
+    const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, 2 * (int64_t)g->n + 2) + 3;
+    if ((rc = drive(g, step, max_steps, KIND_SSSP, (uint64_t)o.directed * 4 + (uint64_t)(a.dpull != nullptr) + ((uint64_t)a.delta << 3))))
+        return rc;
+    rc = finish_search(g, stats, ws.vals, INT_MAX, dist_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
+    if (rc == MG_OK && o.verbose) rc = print_trace(g, "sssp", "push", "pull");
+    return rc;
+}
+
+}  // namespace mg
 
 extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_options *opt, mg_stats *stats) {
     CallTimer timer;
@@ -1119,64 +1008,13 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
     if (src < 0 || src >= g->n)
         return set_error(MG_ERR_INVALID_ARG, "mg_sssp: source %d outside [0,%d)", src, g->n);
     if (o.engine == MG_ENGINE_GAS) return gas_sssp(g, src, dist_out, o, stats, timer);
-    Workspace &ws = g->ws;
-    SsspArgs a{};
-    a.fa = make_frontier_args(g, o.directed);
-    a.dist = ws.vals;
-    a.w = g->w;
-    a.G = std::max(PULL_SLICES,
-                   grid_blocks_cap(std::max<int64_t>(g->n / 4, g->m / 64), BLK, BIG_GRID) / PULL_SLICES * PULL_SLICES);
-    a.dense_pct = sssp_dense_pct();
-    // bucket width of the small-frontier loop: the option, else four maximum edge weights
-    // (a frontier vertex's improvements then stay within the next few buckets; wider
-    // buckets mean fewer, fuller rounds); sssp_delta < 0: unbucketed
-    a.delta = o.sssp_delta < 0 ? LLONG_MAX / 4
-                               : o.sssp_delta > 0 ? (long long)o.sssp_delta
-                                                  : std::max<long long>(1, 4LL * g->max_weight);
-    a.far = ws.hard;
-    a.farbm = ws.visited;
-    const int can_pull = a.fa.off[2] != nullptr && a.fa.nviews == 1 && o.engine != MG_ENGINE_GAS;
-    if (can_pull) {
-        if ((rc = ensure_pull_tasks(g, a.fa.off[2]))) return rc;
-        a.pull = ws.pull_tasks;
-        a.npull = ws.pull_ntasks;
-        if (pull_slicing_enabled()) {
-            if ((rc = ensure_pull_slices(g, a.fa.off[2], a.fa.adj[2]))) return rc;
-            if (ws.slice_ok) {
-                a.sliced = 1;
-                a.slice_off = ws.slice_off;
-                a.slice_min = ws.slice_min;
-                a.slice_tasks = ws.slice_tasks;
-                for (int x = 0; x <= PULL_SLICES; ++x) a.sto[x] = ws.slice_task_off[x];
-            }
-        }
-        if (!a.sliced && a.dense_pct <= 100) {
-            if ((rc = ensure_dense_tasks(g, a.fa.off[2]))) return rc;
-            a.dpull = ws.dense_tasks;
-            a.ndpull = ws.dense_ntasks;
-            // the tasks cut the in-lists; on a symmetric graph those are the out-lists too
-            a.push_range = a.fa.off[2] == a.fa.off[0] && a.fa.nviews == 1 && a.fa.eid[0] == nullptr;
-        }
+    if ((rc = size_chunk_queue(g, o))) return rc;
+    int regrows = 0;
+    while ((rc = sssp_run(g, src, dist_out, o, stats, timer)) == MG_ERR_OVERFLOW && regrows < MAX_REGROWS) {
+        ++regrows;
+        if ((rc = grow_chunk_queue(g))) return rc;
+        clear_error();
     }
-
-    MG_HIP(hipEventRecord(g->ev0, g->stream));
-    hipLaunchKernelGGL(sssp_begin, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, g->stream, a, src, o.iter_num, can_pull,
-                       (long long)g->m);
-    MG_HIP(hipGetLastError());
-    auto step = [&]() -> int {
-        hipLaunchKernelGGL(sssp_small, dim3(1), dim3(SMALL_BLK), 0, g->stream, a);
-        for (int v = 0; v < a.fa.nviews; ++v)
-            hipLaunchKernelGGL(sssp_level_a, dim3(a.G), dim3(BLK), 0, g->stream, a, v);
-        hipLaunchKernelGGL(sssp_level_b, dim3(a.G), dim3(BLK), 0, g->stream, a);
-        if (a.sliced) hipLaunchKernelGGL(sssp_level_c, dim3(a.G), dim3(BLK), 0, g->stream, a);
-        MG_HIP(hipGetLastError());
-        return MG_OK;
-    };
-    // Bellman-Ford with non-negative weights settles within n rounds.
-    const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n + 1) + 3;
-    if ((rc = drive(g, step, max_steps, KIND_SSSP, (uint64_t)o.directed * 4 + (uint64_t)a.sliced * 2 + (uint64_t)(a.dpull != nullptr) + ((uint64_t)a.delta << 3))))
-        return rc;
-    rc = finish_search(g, stats, ws.vals, INT_MAX, dist_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
-    if (rc == MG_OK && o.verbose) rc = print_trace(g, "sssp", "push", "pull");
+    if (rc == MG_OK && stats) stats->overflow_regrows = regrows;
     return rc;
 }
