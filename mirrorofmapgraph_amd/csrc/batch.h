@@ -134,7 +134,9 @@ struct BatchDev {
     uint32_t *ring = nullptr;                    // SSSP bucket ring (SB_RING bitmaps)
     int *hard = nullptr;                         // BFS pull: long in-lists
     ChunkRec *tasks = nullptr;
-    int64_t task_cap = 0;
+    int64_t task_cap = 0;      // wave-task records allocated
+    int64_t task_need = 0;     // records one level can produce at most (the default capacity)
+    int64_t task_floor = 0;    // capacity reached by regrowing (kept for later calls)
     uint4 *V = nullptr, *F = nullptr, *N = nullptr, *FAR = nullptr;
     int *D = nullptr;                            // SSSP distances, n x KS
     int *rows = nullptr;                         // S x n results
@@ -211,6 +213,11 @@ int ppr_iters_of(const mg_options &o);
 int check_batch_sources(const char *fn, const mg_graph *g, const int32_t *srcs, int32_t k);
 // The graph's own batch context (allocated once at MG_BATCH_MAX slots).
 int graph_batch_dev(mg_graph *g, bool sssp, BatchDev **out);
+// Sizes the wave-task queue of the graph's batch context for options o (as
+// size_chunk_queue: need * max_queue_sizing / 1.5, or the regrown size), and grows it
+// 4x after an overflow; both drop the graph's captured step graphs when they resize.
+int size_task_queue(mg_graph *g, BatchDev &b, const mg_options &o);
+int grow_task_queue(mg_graph *g, BatchDev &b);
 // Ends a batched call: device time, overflow check, stats, result delivery.
 int finish_batch(mg_graph *g, BatchDev &b, mg_stats *stats, void *out, size_t bytes, bool out_on_device,
                  const CallTimer &timer);

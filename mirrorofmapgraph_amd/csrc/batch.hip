@@ -522,10 +522,11 @@ int batch_dev_alloc(BatchDev &b, mg_graph *g, bool sssp, int slots) {
     b.KS = (slots + 15) / 16 * 16;
     b.nwords = (int)(((n + 63) / 64) * 2);
     b.G = sssp ? grid_blocks(std::max<int64_t>(g->n / 4, g->m / 16), BLK) : grid_blocks(std::max<int64_t>(g->n / 4, g->m / 64), BLK);
-    b.task_cap = sssp ? g->m / SB_TASK + (int64_t)n + 64 : 2 * (g->m / TASK) + 64;
+    b.task_need = sssp ? g->m / SB_TASK + (int64_t)n + 64 : 2 * (g->m / TASK) + 64;
+    b.task_cap = b.task_need;
     hipError_t e = hipSuccess;
     auto A = [&](void **p, size_t bytes) {
-        if (e == hipSuccess) e = hipMalloc(p, bytes ? bytes : 16);
+        if (e == hipSuccess) e = alloc_refused() ? hipErrorOutOfMemory : hipMalloc(p, bytes ? bytes : 16);
     };
     A((void **)&b.ctl, sizeof(Ctl));
     A((void **)&b.sc, sizeof(SlotCtl));
@@ -559,6 +560,32 @@ int batch_dev_alloc(BatchDev &b, mg_graph *g, bool sssp, int slots) {
 }
 
 int64_t batch_dev_size(const BatchDev *b) { return b->bytes; }
+
+static int set_task_cap(mg_graph *g, BatchDev &b, int64_t cap) {
+    if (cap == b.task_cap) return MG_OK;
+    MG_HIP(hipStreamSynchronize(g->stream));  // no queued step may still use the old queue
+    invalidate_steps(g);
+    b.bytes -= (int64_t)sizeof(ChunkRec) * b.task_cap;
+    free_dev(b.tasks);
+    b.tasks = nullptr;
+    b.task_cap = 0;
+    if (alloc_refused() || hipMalloc((void **)&b.tasks, sizeof(ChunkRec) * (size_t)cap) != hipSuccess) {
+        b.tasks = nullptr;
+        return set_error(MG_ERR_OOM, "batched task queue of %lld records: allocation failed", (long long)cap);
+    }
+    b.task_cap = cap;
+    b.bytes += (int64_t)sizeof(ChunkRec) * cap;
+    return MG_OK;
+}
+
+int size_task_queue(mg_graph *g, BatchDev &b, const mg_options &o) {
+    return set_task_cap(g, b, std::max(queue_capacity(b.task_need, o.max_queue_sizing), b.task_floor));
+}
+
+int grow_task_queue(mg_graph *g, BatchDev &b) {
+    b.task_floor = 4 * b.task_cap;
+    return set_task_cap(g, b, b.task_floor);
+}
 
 void batch_release(mg_graph *g) {
     for (BatchDev *&p : g->steps->bdev) {
@@ -646,12 +673,24 @@ extern "C" int mg_bfs_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t
         sl.slot[s] = s;
         sl.src[s] = srcs[s];
     }
-    MG_HIP(hipEventRecord(g->ev0, g->stream));
-    if ((rc = bfs_batch_reset(*b, g, g->stream, o.iter_num, batch_alpha(o), batch_beta(o), 0))) return rc;
-    if ((rc = bfs_batch_seed(*b, g, g->stream, sl))) return rc;
-    auto step = [&]() -> int { return bfs_batch_step(*b, g, g->stream); };
-    // the step graph reads everything from device memory: one capture per graph
-    const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n) + 3;
-    if ((rc = drive(g, step, max_steps, KIND_BATCH, 0, b->ctl, b->h_ctl))) return rc;
-    return finish_batch(g, *b, stats, depth_out, 4 * (size_t)k * g->n, o.out_on_device, timer);
+    auto run = [&]() -> int {
+        int rc;
+        MG_HIP(hipEventRecord(g->ev0, g->stream));
+        if ((rc = bfs_batch_reset(*b, g, g->stream, o.iter_num, batch_alpha(o), batch_beta(o), 0))) return rc;
+        if ((rc = bfs_batch_seed(*b, g, g->stream, sl))) return rc;
+        auto step = [&]() -> int { return bfs_batch_step(*b, g, g->stream); };
+        // the step graph reads everything from device memory: one capture per graph
+        const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n) + 3;
+        if ((rc = drive(g, step, max_steps, KIND_BATCH, 0, b->ctl, b->h_ctl))) return rc;
+        return finish_batch(g, *b, stats, depth_out, 4 * (size_t)k * g->n, o.out_on_device, timer);
+    };
+    if ((rc = size_task_queue(g, *b, o))) return rc;
+    int regrows = 0;
+    while ((rc = run()) == MG_ERR_OVERFLOW && regrows < MAX_REGROWS) {
+        ++regrows;
+        if ((rc = grow_task_queue(g, *b))) return rc;
+        clear_error();
+    }
+    if (rc == MG_OK && stats) stats->overflow_regrows = regrows;
+    return rc;
 }

@@ -634,15 +634,27 @@ extern "C" int mg_sssp_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_
         sl.slot[s] = all.slot[s] = s;
         sl.src[s] = srcs[s];
     }
-    MG_HIP(hipEventRecord(g->ev0, g->stream));
-    if ((rc = sssp_batch_reset(*b, g, g->stream, o.iter_num, delta))) return rc;
-    if ((rc = sssp_batch_seed(*b, g, g->stream, sl, delta, true))) return rc;
-    auto step = [&]() -> int { return sssp_batch_step(*b, g, g->stream, delta); };
-    if ((rc = drive(g, step, (int64_t)o.iter_num + 3, KIND_BATCH_SSSP, mix64((uint64_t)delta), b->ctl, b->h_ctl)))
-        return rc;
-    if ((rc = sssp_batch_extract(*b, g, g->stream, all))) return rc;
-    if ((rc = batch_row_stats(*b, g, g->stream, all, INT_MAX))) return rc;
-    hipLaunchKernelGGL(sb_sum_stats, dim3(1), dim3(SMALL_BLK), 0, g->stream, b->ctl, (const SlotCtl *)b->sc, k);
-    MG_HIP(hipGetLastError());
-    return finish_batch(g, *b, stats, dist_out, 4 * (size_t)k * g->n, o.out_on_device, timer);
+    auto run = [&]() -> int {
+        int rc;
+        MG_HIP(hipEventRecord(g->ev0, g->stream));
+        if ((rc = sssp_batch_reset(*b, g, g->stream, o.iter_num, delta))) return rc;
+        if ((rc = sssp_batch_seed(*b, g, g->stream, sl, delta, true))) return rc;
+        auto step = [&]() -> int { return sssp_batch_step(*b, g, g->stream, delta); };
+        if ((rc = drive(g, step, (int64_t)o.iter_num + 3, KIND_BATCH_SSSP, mix64((uint64_t)delta), b->ctl, b->h_ctl)))
+            return rc;
+        if ((rc = sssp_batch_extract(*b, g, g->stream, all))) return rc;
+        if ((rc = batch_row_stats(*b, g, g->stream, all, INT_MAX))) return rc;
+        hipLaunchKernelGGL(sb_sum_stats, dim3(1), dim3(SMALL_BLK), 0, g->stream, b->ctl, (const SlotCtl *)b->sc, k);
+        MG_HIP(hipGetLastError());
+        return finish_batch(g, *b, stats, dist_out, 4 * (size_t)k * g->n, o.out_on_device, timer);
+    };
+    if ((rc = size_task_queue(g, *b, o))) return rc;
+    int regrows = 0;
+    while ((rc = run()) == MG_ERR_OVERFLOW && regrows < MAX_REGROWS) {
+        ++regrows;
+        if ((rc = grow_task_queue(g, *b))) return rc;
+        clear_error();
+    }
+    if (rc == MG_OK && stats) stats->overflow_regrows = regrows;
+    return rc;
 }

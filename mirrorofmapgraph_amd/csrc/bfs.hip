@@ -562,14 +562,13 @@ extern "C" int mg_bfs_pred(mg_graph *g, int32_t src, int32_t *depth_out, int32_t
     return rc;
 }
 
-extern "C" int mg_bfs(mg_graph *g, int32_t src, int32_t *depth_out, const mg_options *opt, mg_stats *stats) {
-    CallTimer timer;
-    mg_options o;
-    int rc = begin_search(g, opt, &o);
-    if (rc) return rc;
-    std::lock_guard<std::mutex> lock(g->mu);
-    if (src < 0 || src >= g->n) return set_error(MG_ERR_INVALID_ARG, "mg_bfs: source %d outside [0,%d)", src, g->n);
-    if (o.engine == MG_ENGINE_GAS) return gas_bfs(g, src, depth_out, o, stats, timer);
+namespace mg {
+
+// One BFS search with the current queues (MG_ERR_OVERFLOW when the edge-chunk queue
+// overflowed; the caller regrows it and runs again).
+static int bfs_run(mg_graph *g, int src, int *depth_out, const mg_options &o, mg_stats *stats,
+                   const CallTimer &timer) {
+    int rc;
     Workspace &ws = g->ws;
     BfsArgs a{};
     a.fa = make_frontier_args(g, o.directed);
@@ -608,5 +607,26 @@ extern "C" int mg_bfs(mg_graph *g, int32_t src, int32_t *depth_out, const mg_opt
     if ((rc = drive(g, step, max_steps, KIND_BFS, (uint64_t)o.directed))) return rc;
     rc = finish_search(g, stats, ws.vals, -1, depth_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
     if (rc == MG_OK && o.verbose) rc = print_trace(g, "bfs", "top-down", "bottom-up");
+    return rc;
+}
+
+}  // namespace mg
+
+extern "C" int mg_bfs(mg_graph *g, int32_t src, int32_t *depth_out, const mg_options *opt, mg_stats *stats) {
+    CallTimer timer;
+    mg_options o;
+    int rc = begin_search(g, opt, &o);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g->mu);
+    if (src < 0 || src >= g->n) return set_error(MG_ERR_INVALID_ARG, "mg_bfs: source %d outside [0,%d)", src, g->n);
+    if (o.engine == MG_ENGINE_GAS) return gas_bfs(g, src, depth_out, o, stats, timer);
+    if ((rc = size_chunk_queue(g, o))) return rc;
+    int regrows = 0;
+    while ((rc = bfs_run(g, src, depth_out, o, stats, timer)) == MG_ERR_OVERFLOW && regrows < MAX_REGROWS) {
+        ++regrows;
+        if ((rc = grow_chunk_queue(g))) return rc;
+        clear_error();
+    }
+    if (rc == MG_OK && stats) stats->overflow_regrows = regrows;
     return rc;
 }

@@ -81,14 +81,15 @@ struct Ctl {
     long long thr;            // bucketed SSSP: distances below thr are in the current bucket
     long long far_hi;         // bucketed SSSP: highest bucket holding a waiting distance
     int pair_end;             // batched SSSP: pair slots below this are valid this iteration
-    int dense;                // SSSP pull level over every in-neighbour (no frontier test)
 };
 
 // Pooled per-search device state, grown on demand and reused across searches.
 struct Workspace {
     int n = 0;
     int nwords = 0;           // 32-bit words of one vertex bitmap (rounded to even)
-    int64_t chunk_cap = 0;
+    int64_t chunk_cap = 0;    // edge-chunk records allocated
+    int64_t chunk_need = 0;   // records one level can produce at most (the default capacity)
+    int64_t chunk_floor = 0;  // capacity reached by regrowing (kept for later searches)
     int32_t *vals = nullptr;  // depth / dist / label
     long long *trace = nullptr;  // per level: next frontier size | direction << 40, its out-edges
     int32_t *q[3] = {nullptr, nullptr, nullptr};
@@ -113,11 +114,6 @@ struct Workspace {
     int32_t *pr_pact = nullptr;                       // per-chunk "an in-neighbour changed" flags
     int pr_nmid = 0, pr_nchunks = 0, pr_nbig = 0;
     const int32_t *pr_view = nullptr;                 // in-edge offsets the buckets were built for
-    // edge-centric PageRank: in-lists crossing PULL_CHUNK boundaries
-    int32_t *pr_span = nullptr;       // per crossing list: vertex, first chunk, last chunk (3 ints)
-    int pr_nspan_short = 0, pr_nspan = 0;  // short spans (<= PR_SPAN_SHORT chunks) first, then long
-    float *pr_head = nullptr, *pr_tail = nullptr;  // per chunk: partial of its first / last list
-    const int32_t *pr_span_view = nullptr;
     // Static pull tasks: PULL_TASK-edge pieces of the in-edge lists longer than LANE_T
     ChunkRec *pull_tasks = nullptr;
     int pull_ntasks = 0;
@@ -125,22 +121,6 @@ struct Workspace {
     ChunkRec *dense_tasks = nullptr;  // dense SSSP pull tasks (pieces and vertex ranges)
     int dense_ntasks = 0;
     const int32_t *dense_view = nullptr;
-    // XCD-sliced pull: in-lists cut by source slice (PULL_SLICES slices of slice_len
-    // vertices, slice x gathered by the workgroups on XCD x)
-    int32_t *slice_off = nullptr;      // (PULL_SLICES - 1) x n interior cut points
-    int32_t *slice_min = nullptr;      // PULL_SLICES x n per-slice partial results
-    ChunkRec *slice_tasks = nullptr;   // TASK-edge pieces of long (vertex, slice) ranges, grouped by slice
-    ChunkRec *slice_segs = nullptr;    // per long range: {vertex, first piece, pieces, slice}
-    float *slice_part = nullptr;       // per piece partial (PageRank)
-    int nslice_segs = 0;
-    int32_t slice_task_off[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    int32_t *d_slice_task_off = nullptr;
-    int slice_len = 0;
-    const int32_t *slice_view = nullptr;  // in-edge offsets the slices were built for (nullptr: none)
-    bool slice_ok = false;                // lists sorted, so slicing applies
-    // Edge-centric pull: the vertex owning each in-edge slot of the in-edge view
-    int32_t *edge_dst = nullptr;
-    const int32_t *edge_dst_view = nullptr;
 };
 
 }  // namespace mg
@@ -167,6 +147,26 @@ namespace mg {
 
 // Grows the workspace for graph g; returns an MG_* status.
 int ensure_workspace(mg_graph *g);
+// Fault injection for tests: with MG_FAIL_ALLOC_AT=k in the environment, the k-th
+// device allocation of the library counted from the moment the variable was set (or
+// last changed) is refused, so every cleanup path behind an MG_ERR_OOM can be driven
+// from the host.  Each allocation site calls this first; false when unset.
+bool alloc_refused();
+// Queue sizing.  A queue whose records one level can produce at most `need` holds
+// need * max_queue_sizing / 1.5 records (the reference's max_queue_sizing, default
+// 1.5, scales its frontier queues the same way, csr_problem.cuh:627-827), at least
+// 64; a queue that overflows is regrown 4x and the search replayed, and the grown
+// size stays for later searches on the graph.
+constexpr int MAX_REGROWS = 12;
+int64_t queue_capacity(int64_t need, double max_queue_sizing);
+// Sizes the edge-chunk queue of the BFS / SSSP levels for options o (reallocating it,
+// and dropping the captured step graphs that point at it, when the size changes).
+int size_chunk_queue(mg_graph *g, const mg_options &o);
+// After an overflow: 4x the capacity (kept for later searches).
+int grow_chunk_queue(mg_graph *g);
+// Destroys every captured step graph of g (their kernel arguments hold buffer
+// pointers; they are captured again on next use).
+void invalidate_steps(mg_graph *g);
 int ensure_pr_workspace(mg_graph *g);
 int alloc_i32(int32_t **p, size_t n);
 int alloc_f32(float **p, size_t n);
@@ -196,8 +196,6 @@ int deliver(mg_graph *g, void *dst, const void *src, size_t bytes, bool out_on_d
 // every later search with the same key.
 constexpr int STEPS_PER_GRAPH = 8;
 constexpr int TRACE_CAP = 16384;  // levels recorded in the per-level trace (verbose)
-constexpr int PULL_SLICES = 8;  // one source slice per XCD (workgroup b runs on XCD b % 8)
-constexpr int PULL_CHUNK = 256;  // edge-centric pull: in-edge slots per wave task (4 per lane)
 constexpr int PULL_TASK = 512;   // static pull tasks: in-edges per wave (8 per lane in flight)
 // Dense SSSP pull tasks: in-lists longer than DENSE_BIG are cut into DENSE_T-slot
 // pieces; the other vertices are grouped into ranges of whole bitmap words (at most
@@ -214,19 +212,18 @@ struct StepGraph {
     int64_t last_steps = 0;  // steps the previous search with this key ran (0: not counted)
 };
 
+// Returns MG_OK once the device sets ctl->done; a search still running after
+// max_steps steps is an error (every iteration limit sets done on the device, so
+// this never ends a normal search early).
 template <class Step>
 int drive(mg_graph *g, Step step, int64_t max_steps, int kind = -1, uint64_t key = 0, Ctl *ctl = nullptr,
           Ctl *h_ctl = nullptr);
+inline int not_converged(int64_t max_steps) {
+    return set_error(MG_ERR_DEVICE, "the search did not finish within %lld steps", (long long)max_steps);
+}
+// Captured step graphs are on unless MG_NO_GRAPH is set to a non-zero value (then
+// every step is launched directly; read once per process).
 bool graphs_enabled();
-// MG_SLICED_PULL=1 turns on the XCD-sliced pull of SSSP and PageRank (off by
-// default: measured slower than the unsliced pull at scale 22).
-bool pull_slicing_enabled();
-// MG_EDGE_PULL=1 selects edge-centric pull levels (off by default: measured slower
-// than the vertex-centric pull at scale 22).
-bool edge_pull_enabled();
-// SSSP pull levels whose frontier holds more than this percentage of the edges skip
-// the frontier test (MG_SSSP_DENSE_PCT, default 0: every pull level; above 100 never).
-int sssp_dense_pct();
 
 // Generic GAS enactor over the ported programs (engine = MG_ENGINE_GAS).
 struct GasStateBase;
@@ -334,7 +331,7 @@ int mg::drive(mg_graph *g, Step step, int64_t max_steps, int kind, uint64_t key,
             }
             if (k < 16) k *= 2;
         }
-        return MG_OK;
+        return not_converged(max_steps);
     }
     int64_t queued = 0, k = 2;
     while (queued < max_steps) {
@@ -347,5 +344,5 @@ int mg::drive(mg_graph *g, Step step, int64_t max_steps, int kind, uint64_t key,
         if (h_ctl->done) return MG_OK;
         if (k < 64) k *= 2;
     }
-    return MG_OK;
+    return not_converged(max_steps);
 }

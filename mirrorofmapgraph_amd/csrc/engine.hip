@@ -10,6 +10,7 @@
 #include <climits>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "engine.h"
@@ -27,9 +28,29 @@ int grid_blocks_cap(int64_t work_items, int block, int cap) {
     return (int)b;
 }
 
+bool alloc_refused() {
+    static std::mutex mu;
+    static std::string armed;
+    static long long seen = 0;
+    const char *e = getenv("MG_FAIL_ALLOC_AT");
+    std::lock_guard<std::mutex> lock(mu);
+    const std::string cur = e ? e : "";
+    if (cur != armed) {
+        armed = cur;
+        seen = 0;
+    }
+    if (cur.empty()) return false;
+    const long long k = atoll(cur.c_str());
+    return k > 0 && ++seen == k;
+}
+
 template <class T>
 static int dalloc(T **p, size_t count) {
     if (count == 0) count = 1;
+    if (alloc_refused()) {
+        *p = nullptr;
+        return set_error(MG_ERR_OOM, "device allocation of %zu bytes refused (MG_FAIL_ALLOC_AT)", count * sizeof(T));
+    }
     hipError_t e = hipMalloc((void **)p, count * sizeof(T));
     if (e != hipSuccess) {
         *p = nullptr;
@@ -65,19 +86,9 @@ static void free_workspace(Workspace &ws) {
     dfree(ws.pr_part);
     dfree(ws.pr_pact);
     dfree(ws.pr_bigv);
-    dfree(ws.pr_span);
-    dfree(ws.pr_head);
-    dfree(ws.pr_tail);
     dfree(ws.trace);
     dfree(ws.pull_tasks);
     dfree(ws.dense_tasks);
-    dfree(ws.slice_off);
-    dfree(ws.slice_min);
-    dfree(ws.slice_tasks);
-    dfree(ws.edge_dst);
-    dfree(ws.slice_segs);
-    dfree(ws.slice_part);
-    dfree(ws.d_slice_task_off);
     ws = Workspace();
 }
 
@@ -89,7 +100,8 @@ int ensure_workspace(mg_graph *g) {
     ws.nwords = ((n + 63) / 64) * 2;
     // records per level: TASK-edge pieces of lists longer than LANE_T (CHUNK of
     // lists longer than CHUNK_T in the tile path), over both push views
-    ws.chunk_cap = 2 * (g->m / TASK + g->m / (LANE_T + 1)) + 2 * (g->m / CHUNK + g->m / CHUNK_T) + 64;
+    ws.chunk_need = 2 * (g->m / TASK + g->m / (LANE_T + 1)) + 2 * (g->m / CHUNK + g->m / CHUNK_T) + 64;
+    ws.chunk_cap = ws.chunk_need;
     int rc = 0;
     if ((rc = dalloc(&ws.vals, n)) || (rc = dalloc(&ws.q[0], n)) || (rc = dalloc(&ws.q[1], n)) ||
         (rc = dalloc(&ws.q[2], n)) || (rc = dalloc(&ws.hard, n)) || (rc = dalloc(&ws.visited, ws.nwords)) ||
@@ -114,9 +126,57 @@ int ensure_pr_workspace(mg_graph *g) {
     int rc = 0;
     const int n = g->n;
     if ((rc = dalloc(&ws.rank, n)) || (rc = dalloc(&ws.contrib[0], n)) || (rc = dalloc(&ws.contrib[1], n)) ||
-        (rc = dalloc(&ws.partials, 8192)))
+        (rc = dalloc(&ws.partials, 8192))) {
+        // all or nothing: ws.rank marks the PageRank state as present
+        dfree(ws.rank);
+        dfree(ws.contrib[0]);
+        dfree(ws.contrib[1]);
+        dfree(ws.partials);
         return rc;
+    }
     return MG_OK;
+}
+
+int64_t queue_capacity(int64_t need, double max_queue_sizing) {
+    if (!(max_queue_sizing > 0)) return need;  // 0, negative or NaN: the default
+    const double c = (double)need * (max_queue_sizing / 1.5);
+    return c < 64 ? 64 : c > 4e18 ? (int64_t)4e18 : (int64_t)c;
+}
+
+void invalidate_steps(mg_graph *g) {
+    if (!g->steps) return;
+    for (StepGraph &s : g->steps->sg) {
+        if (s.exec) (void)hipGraphExecDestroy(s.exec);
+        for (hipGraphExec_t &x : s.part) {
+            if (x) (void)hipGraphExecDestroy(x);
+            x = nullptr;
+        }
+        s = StepGraph();
+    }
+}
+
+static int set_chunk_cap(mg_graph *g, int64_t cap) {
+    Workspace &ws = g->ws;
+    if (cap == ws.chunk_cap) return MG_OK;
+    MG_HIP(hipStreamSynchronize(g->stream));  // no queued step may still use the old queue
+    invalidate_steps(g);
+    dfree(ws.chunks);
+    ws.chunk_cap = 0;
+    int rc = dalloc(&ws.chunks, (size_t)cap);
+    if (rc) return rc;
+    ws.chunk_cap = cap;
+    return MG_OK;
+}
+
+int size_chunk_queue(mg_graph *g, const mg_options &o) {
+    const Workspace &ws = g->ws;
+    return set_chunk_cap(g, std::max(queue_capacity(ws.chunk_need, o.max_queue_sizing), ws.chunk_floor));
+}
+
+int grow_chunk_queue(mg_graph *g) {
+    Workspace &ws = g->ws;
+    ws.chunk_floor = 4 * ws.chunk_cap;
+    return set_chunk_cap(g, ws.chunk_floor);
 }
 
 int alloc_i32(int32_t **p, size_t n) { return dalloc(p, n); }
@@ -132,30 +192,6 @@ bool graphs_enabled() {
     static const bool on = [] {
         const char *e = getenv("MG_NO_GRAPH");
         return !(e && *e && *e != '0');
-    }();
-    return on;
-}
-
-bool edge_pull_enabled() {
-    static const bool on = [] {
-        const char *e = getenv("MG_EDGE_PULL");
-        return e && *e && *e != '0';
-    }();
-    return on;
-}
-
-int sssp_dense_pct() {
-    static const int pct = [] {
-        const char *e = getenv("MG_SSSP_DENSE_PCT");
-        return e && *e ? atoi(e) : 0;
-    }();
-    return pct;
-}
-
-bool pull_slicing_enabled() {
-    static const bool on = [] {
-        const char *e = getenv("MG_SLICED_PULL");
-        return e && *e && *e != '0';
     }();
     return on;
 }

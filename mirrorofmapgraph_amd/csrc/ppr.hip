@@ -216,7 +216,7 @@ int ppr_dev_alloc(PprDev &p, mg_graph *g) {
     p.npart = npart;
     hipError_t e = hipSuccess;
     auto A = [&](void **q, size_t bytes) {
-        if (e == hipSuccess) e = hipMalloc(q, bytes ? bytes : 16);
+        if (e == hipSuccess) e = alloc_refused() ? hipErrorOutOfMemory : hipMalloc(q, bytes ? bytes : 16);
     };
     A((void **)&p.X[0], 4 * (size_t)PPR_SLOTS * n);
     A((void **)&p.X[1], 4 * (size_t)PPR_SLOTS * n);

@@ -296,3 +296,137 @@ def test_bfs_predecessors_fixtures(mg, oracle, device):
             for src in range(min(g.n, 4)):
                 d, p = dg.bfs_pred(src)
                 assert np.array_equal(p, _pred_reference(g, d)), (name, src)
+
+
+# ---------------------------------------------------------------- uncaptured steps
+_UNCAPTURED_CHILD = r"""
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+sys.path.insert(0, sys.argv[1] + "/tests")
+import mirrorofmapgraph_amd as mg
+import oracle
+mg.check(mg.lib().mg_device_init(0))
+rm = mg.rmat_graph(14, 16, seed=5, weight_max=64)
+grid = mg.grid_graph(128, seed=6, weight_max=16)
+for g in (rm, grid):
+    with mg.Graph(g, symmetric=True) as dg:
+        for s in (0, int(mg.pick_sources(g, 1, seed=3)[0])):
+            assert np.array_equal(dg.bfs(s), oracle.bfs(g.row_offsets, g.col_idx, s)), ("bfs", s)
+            assert np.array_equal(dg.sssp(s), oracle.sssp(g.row_offsets, g.col_idx, g.values, s)), ("sssp", s)
+        assert np.array_equal(dg.cc(), oracle.cc(g.row_offsets, g.col_idx))
+        srcs = mg.pick_sources(g, 40, seed=9)
+        rows = dg.bfs_batch(srcs)
+        srows = dg.sssp_batch(srcs)
+        for j in (0, 17, 39):
+            assert np.array_equal(rows[j], oracle.bfs(g.row_offsets, g.col_idx, int(srcs[j]))), ("bfs_batch", j)
+            assert np.array_equal(srows[j], oracle.sssp(g.row_offsets, g.col_idx, g.values, int(srcs[j]))), j
+gd = mg.rmat_graph(12, 16, seed=5, symmetric=False)
+with mg.Graph(gd, build_csc=True) as dg:
+    r = dg.pagerank(mg.options(pr_mode=mg.MG_PR_ALL, iter_num=20))
+exp, _, _ = oracle.pr_jacobi_f32(gd.row_offsets, gd.col_idx, iters=20)
+assert float(np.max(np.abs(r - exp))) / gd.n <= 1e-6
+print("uncaptured ok", flush=True)
+"""
+
+
+def test_uncaptured_steps_match_oracle(mg, device):
+    """MG_NO_GRAPH=1 launches every step directly instead of replaying captured HIP
+    graphs (the path the server also takes when capture fails); the results must be
+    the same.  The switch is read once per process, so it runs in a child."""
+    import os
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    root = str(Path(__file__).resolve().parents[1])
+    env = dict(os.environ, MG_NO_GRAPH="1")
+    r = subprocess.run([sys.executable, "-c", _UNCAPTURED_CHILD, root], env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0 and "uncaptured ok" in r.stdout, r.stdout[-2000:] + r.stderr[-4000:]
+
+
+# ---------------------------------------------------------------- overflow and allocation failures
+def test_queue_overflow_regrows_and_replays(mg, oracle, device):
+    """A tiny max_queue_sizing makes the edge-chunk queue (BFS / SSSP levels) and the
+    wave-task queue (batched levels) overflow; the engine regrows the queue 4x and replays
+    the search until it fits, counting the regrows in mg_stats.overflow_regrows, and the
+    grown size stays with the graph.  The directed graph has no in-edge view, so every
+    level pushes through the queues."""
+    g = mg.rmat_graph(16, 16, seed=42, symmetric=False, weight_max=64)
+    tiny = mg.options(max_queue_sizing=1e-4)
+    exp_b = oracle.bfs(g.row_offsets, g.col_idx, 0)
+    exp_s = oracle.sssp(g.row_offsets, g.col_idx, g.values, 0)
+    with mg.Graph(g) as dg:
+        d, st = dg.bfs(0, tiny, stats=True)
+        assert np.array_equal(d, exp_b) and st.overflow_regrows > 0
+        d, st = dg.bfs(0, tiny, stats=True)
+        assert np.array_equal(d, exp_b) and st.overflow_regrows == 0
+        d, st = dg.bfs(0, stats=True)
+        assert np.array_equal(d, exp_b) and st.overflow_regrows == 0
+    with mg.Graph(g) as dg:
+        s, st = dg.sssp(0, tiny, stats=True)
+        assert np.array_equal(s, exp_s) and st.overflow_regrows > 0
+    srcs = mg.pick_sources(g, 64, seed=1)
+    with mg.Graph(g) as dg:
+        rows, st = dg.bfs_batch(srcs, tiny, stats=True)
+        assert st.overflow_regrows > 0
+        srows, st2 = dg.sssp_batch(srcs, tiny, stats=True)
+        for j in (0, 31, 63):
+            assert np.array_equal(rows[j], oracle.bfs(g.row_offsets, g.col_idx, int(srcs[j]))), j
+            assert np.array_equal(srows[j], oracle.sssp(g.row_offsets, g.col_idx, g.values, int(srcs[j]))), j
+
+
+_ALLOC_FAIL_CHILD = r"""
+import os, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import mirrorofmapgraph_amd as mg
+import oracle
+mg.check(mg.lib().mg_device_init(0))
+g = mg.rmat_graph(11, 8, seed=3, weight_max=16)
+srcs = mg.pick_sources(g, 8, seed=2)
+exp = {"bfs": oracle.bfs(g.row_offsets, g.col_idx, 0), "sssp": oracle.sssp(g.row_offsets, g.col_idx, g.values, 0),
+       "cc": oracle.cc(g.row_offsets, g.col_idx)}
+pr_exp, _, _ = oracle.pr_jacobi_f32(g.row_offsets, g.col_idx, iters=10)
+refused = 0
+for k in range(1, 90):
+    os.environ["MG_FAIL_ALLOC_AT"] = str(k)  # the k-th device allocation from now on is refused
+    def attempt(fn):
+        global refused
+        try:
+            return fn()
+        except mg.MgError as e:
+            assert e.code == mg.MG_ERR_OOM, (k, e)
+            refused += 1
+            return fn()  # the hook fires once: the same call now succeeds
+    dg = attempt(lambda: mg.Graph(g, symmetric=True))
+    assert np.array_equal(attempt(lambda: dg.bfs(0)), exp["bfs"]), k
+    assert np.array_equal(attempt(lambda: dg.sssp(0)), exp["sssp"]), k
+    assert np.array_equal(attempt(lambda: dg.cc()), exp["cc"]), k
+    r = attempt(lambda: dg.pagerank(mg.options(pr_mode=mg.MG_PR_ALL, iter_num=10)))
+    assert float(np.max(np.abs(r - pr_exp))) / g.n <= 1e-6, k
+    rows = attempt(lambda: dg.bfs_batch(srcs))
+    assert np.array_equal(rows[0], oracle.bfs(g.row_offsets, g.col_idx, int(srcs[0]))), k
+    srows = attempt(lambda: dg.sssp_batch(srcs))
+    assert np.array_equal(srows[1], oracle.sssp(g.row_offsets, g.col_idx, g.values, int(srcs[1]))), k
+    attempt(lambda: dg.ppr_batch(srcs[:4]))
+    dg.close()
+del os.environ["MG_FAIL_ALLOC_AT"]
+assert refused > 30, refused
+print("alloc failures ok", refused, flush=True)
+"""
+
+
+def test_allocation_failures_are_status_codes(mg, device):
+    """With the MG_FAIL_ALLOC_AT hook the k-th device allocation is refused, for every k
+    across graph creation and every search kind: each call fails with MG_ERR_OOM (never
+    a crash or a wrong result), leaves the graph usable, and the same call then succeeds.
+    Runs in a child process so the hook cannot touch other tests."""
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    root = str(Path(__file__).resolve().parents[1])
+    r = subprocess.run([sys.executable, "-c", _ALLOC_FAIL_CHILD, root], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0 and "alloc failures ok" in r.stdout, r.stdout[-2000:] + r.stderr[-4000:]
