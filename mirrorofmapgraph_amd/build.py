@@ -19,6 +19,8 @@ LIB = PKG / "libmapgraph_hip.so"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 HIPCC = str(ROCM / "bin" / "hipcc")
 ARCH = os.environ.get("MG_OFFLOAD_ARCH", "gfx950")
+DROPIN_SRC = PKG.parent / "tests" / "dropin" / "mapgraph_dropin.cpp"
+DROPIN_BIN = PKG.parent / "tests" / "dropin" / "mapgraph_dropin"
 
 # -packed-fp32-ops: no packed FP32 instructions (v_pk_*_f32) are generated.
 # -ffp-contract=off: float expressions round as written (PageRank parity).
@@ -77,6 +79,11 @@ def build(jobs: int = 8, force: bool = True, verbose: bool = False) -> Path:
     if drv.exists():
         _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", str(PKG.parent / "include"), str(drv), "-L", str(PKG),
               "-lmapgraph_hip", "-Wl,-rpath,$ORIGIN", "-o", str(PKG / "mapgraph_run")])
+    # the C++ drop-in test driver: a MapGraph-style program over the header-only
+    # CsrProblem / EnactorVertexCentric API (compiled by hipcc: the enactor's kernels)
+    dropin = DROPIN_SRC
+    if dropin.exists():
+        _run([HIPCC, *[f for f in HIP_FLAGS if f != "-fPIC"], str(dropin), "-o", str(DROPIN_BIN)])
     return LIB
 
 
