@@ -223,9 +223,12 @@ def batched_part(mg, oracle, seed: int, reps: int, log) -> dict:
     and the 2048x2048 grid: one batched call each against the same 256 sources run one by one (plain, every
     source timed).  Every batched row is compared with its plain search, and sampled rows with the oracle."""
     out = {}
-    for name, host in [("rmat20", mg.rmat_graph(20, 16, seed=seed, weight_max=64)),
-                       ("grid2048", mg.grid_graph(2048, seed=seed, weight_max=16))]:
-        srcs = mg.pick_sources(host, 256, seed=seed)
+    hosts = [("rmat20", mg.rmat_graph(20, 16, seed=seed, weight_max=64)),
+             ("grid2048", mg.grid_graph(2048, seed=seed, weight_max=16))]
+    sources = {name: mg.pick_sources(host, 256, seed=seed) for name, host in hosts}
+    out["both_graphs_one_search"] = multi_graph_part(mg, hosts, sources, reps, log)
+    for name, host in hosts:
+        srcs = sources[name]
         with mg.Graph(host, symmetric=True) as dg:
             res = {"n": host.n, "m": host.m, "sources": 256}
             for kind, batch, plain, orc in [
@@ -250,6 +253,37 @@ def batched_part(mg, oracle, seed: int, reps: int, log) -> dict:
                 log(f"batched {name} {kind}: {tb:.2f} ms vs plain {tp:.1f} ms ({tp / tb:.1f}x)")
         out[name] = res
     return out
+
+
+def multi_graph_part(mg, hosts, sources, reps: int, log) -> dict:
+    """Both batched graphs in one search: their disjoint union (mg_graph_union) with row j carrying the j-th
+    source of each graph (256 + 256 sources in 256 rows), against the two separate batched calls; every
+    graph's rows are compared with its separate call."""
+    dgs = [mg.Graph(h, symmetric=True) for _, h in hosts]
+    res = {"graphs": [name for name, _ in hosts], "sources_per_graph": 256}
+    try:
+        u = mg.Graph.union(dgs)
+        srcs = [sources[name] for name, _ in hosts]
+        with u:
+            for kind in ("bfs", "sssp"):
+                one = lambda: u.parts_batch(kind, srcs, stats=True, copy=False)
+                t_one, st = median_ms(one, reps)
+                parts = u.parts_batch(kind, srcs)
+                t_sep = 0.0
+                for i, dg in enumerate(dgs):
+                    fn = dg.bfs_batch if kind == "bfs" else dg.sssp_batch
+                    t, _ = median_ms(lambda: fn(srcs[i], stats=True, copy=False), reps)
+                    t_sep += t
+                    require_equal(f"{hosts[i][0]} {kind} rows of the one-search union vs the graph's own batch",
+                                  parts[i].ravel(), fn(srcs[i]).ravel())
+                del parts
+                res[kind] = {"one_search_ms": t_one, "separate_ms": t_sep, "speedup": t_sep / t_one,
+                             "gteps": st.edges_traversed / t_one / 1e6, "levels": st.iterations, "checked": True}
+                log(f"union rmat20+grid2048 {kind}: {t_one:.1f} ms in one search vs {t_sep:.1f} ms separately")
+    finally:
+        for d in dgs:
+            d.close()
+    return res
 
 
 def varied_sources_part(mg, oracle, dg, g, seed: int, log) -> dict:

@@ -81,6 +81,12 @@ int mg_graph_create(const int32_t *row_offsets, const int32_t *col_idx,
                     const int32_t *edge_values, int32_t n, int64_t m,
                     uint32_t flags, mg_graph **out);
 int mg_graph_destroy(mg_graph *g);
+/* The disjoint union of ngraphs resident graphs (of one device) as a new graph, built
+ * on the device: part i's vertices v become vertex_offsets[i] + v (vertex_offsets, if
+ * not NULL, gets ngraphs + 1 entries, the last one the union's n).  The union is
+ * weighted (symmetric) when every part is; it has no CSC view.  The parts stay
+ * independent graphs.  No counterpart in the reference (one graph per CsrProblem). */
+int mg_graph_union(int32_t ngraphs, mg_graph *const *parts, mg_graph **out, int32_t *vertex_offsets);
 /* n, m and the flags a graph was created with. */
 int mg_graph_info(const mg_graph *g, int32_t *n, int64_t *m, uint32_t *flags);
 /* Device bytes held by the graph (topology + pooled per-search state). */
@@ -121,11 +127,20 @@ typedef struct mg_options {
     int32_t batch_beta;     /* mg_bfs_batch pull -> push switch (frontier edges < m/batch_beta); 0 = 16  */
     int32_t sssp_delta;     /* SSSP bucket width: relax only distances below the current
                                bucket bound, raised by sssp_delta when no such distance is
-                               left (mg_sssp: in its small-frontier levels; the batched
-                               searches: always); 0 = auto (mg_sssp: 4 x the largest
-                               weight; batched: about 4 x the mean weight), -1 = unbucketed */
+                               left.  mg_sssp: > 0 buckets every level (whole-GPU levels
+                               push and carry waiting vertices along); 0 = auto buckets the
+                               single-workgroup levels at 4 x the largest weight and runs the
+                               whole-GPU levels unbucketed (dense levels pull); -1 =
+                               unbucketed.  Batched searches: the width of every level, 0 =
+                               about 4 x the mean weight, -1 = unbucketed                  */
     int32_t ppr_iters;      /* personalized PageRank iterations (0 = 20)                   */
-    int32_t reserved[4];
+    int32_t sssp_bucket_edges; /* mg_sssp: once the pending vertices of a whole-GPU level
+                               have fewer out-edges than this, the bucket bound is lifted
+                               and the rest runs as Bellman-Ford; 0 = auto (m / 8), -1 =
+                               never lifted                                                */
+    int32_t level_grid;     /* mg_bfs / mg_sssp: workgroups of the whole-GPU level kernels
+                               (1..65536; 0 = auto from n and m)                           */
+    int32_t reserved[2];
 } mg_options;
 
 /* Fills *o with the reference defaults. */
@@ -197,6 +212,20 @@ int mg_bfs_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t *depth_out
 int mg_sssp_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t *dist_out,
                   const mg_options *opt, mg_stats *stats);
 
+/* Batched searches with source sets: npairs (row, source) pairs, row_of[i] in [0, k),
+ * every row with at least one source.  Row r of the k x n result holds the depths
+ * (distances) from the set of its sources, as a search seeded with several sources
+ * (the reference's num_srcs / srcs, Algorithms/BFS/bfs.h:62-126, which runs them as
+ * one frontier).  On a disjoint union of graphs (mg_graph_union) a row may carry one
+ * source in each part: the parts' searches never meet, so each part's segment of the
+ * row equals the single-source search on that part -- several graphs of uneven size
+ * advance in one batched search.  Same options and requirements as mg_bfs_batch /
+ * mg_sssp_batch; mg_stats sums the rows' counts. */
+int mg_bfs_batch_sets(mg_graph *g, const int32_t *row_of, const int32_t *srcs, int32_t npairs, int32_t k,
+                      int32_t *depth_out, const mg_options *opt, mg_stats *stats);
+int mg_sssp_batch_sets(mg_graph *g, const int32_t *row_of, const int32_t *srcs, int32_t npairs, int32_t k,
+                       int32_t *dist_out, const mg_options *opt, mg_stats *stats);
+
 #define MG_PPR_MAX 64  /* sources per batched personalized PageRank call */
 
 /* Personalized PageRank from k <= MG_PPR_MAX sources, FP32 (new here; the
@@ -209,6 +238,36 @@ int mg_sssp_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t *dist_out
  * whatever other sources share the call. */
 int mg_ppr_batch(mg_graph *g, const int32_t *srcs, int32_t k, float *rank_out,
                  const mg_options *opt, mg_stats *stats);
+
+/* ---------------------------------------------------------------- autotuning */
+/* Picks, for one graph and one search kind, the mg_options fields that decide how the
+ * search maps onto the GPU (no counterpart in the reference, which fixes its kernel
+ * policies per SM version at compile time, GASengine/enactor_vertex_centric.cuh:
+ * 2910-2966, and takes one `threshold`, Algorithms/BFS/register.h:39):
+ *   MG_TUNE_BFS         alpha, beta, level_grid            (mg_bfs)
+ *   MG_TUNE_SSSP        sssp_delta, sssp_bucket_edges, level_grid   (mg_sssp)
+ *   MG_TUNE_BFS_BATCH   batch_alpha, batch_beta             (mg_bfs_batch)
+ *   MG_TUNE_SSSP_BATCH  sssp_delta                          (mg_sssp_batch)
+ * Coordinate descent from *start (NULL: the defaults) over candidate values, two
+ * passes: each setting is timed on the sample sources (device time, median of reps
+ * runs after a warm-up; single-source kinds: up to 64 sources, summed; batched kinds:
+ * one call with all of them, up to MG_BATCH_MAX) and kept when at least 3% faster.
+ * Every candidate's results must equal the starting setting's bit for bit, else
+ * MG_ERR_DEVICE.  *best receives the chosen options (the other fields as *start). */
+#define MG_TUNE_BFS        0
+#define MG_TUNE_SSSP       1
+#define MG_TUNE_BFS_BATCH  2
+#define MG_TUNE_SSSP_BATCH 3
+
+typedef struct mg_tune_report {
+    double  start_ms;         /* device ms of the starting setting (as timed by the tuner) */
+    double  best_ms;          /* device ms of the chosen setting                           */
+    int32_t candidates;       /* settings timed                                            */
+    int32_t reserved[7];
+} mg_tune_report;
+
+int mg_autotune(mg_graph *g, int32_t what, const int32_t *srcs, int32_t nsrc, int32_t reps,
+                const mg_options *start, mg_options *best, mg_tune_report *report);
 
 /* ---------------------------------------------------------------- serving */
 /* A request loop over resident graphs (new here: the reference runs one search at a

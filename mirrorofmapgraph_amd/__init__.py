@@ -52,7 +52,8 @@ EXPORTED = [
     "mg_options_default", "mg_bfs", "mg_bfs_pred", "mg_sssp", "mg_cc", "mg_pagerank", "mg_bfs_batch", "mg_sssp_batch", "mg_ppr_batch",
     "mg_rmat_num_edges", "mg_gen_rmat_edges", "mg_grid_num_edges", "mg_gen_grid_edges",
     "mg_csr_from_edges", "mg_edge_weights_hash", "mg_csc_from_csr", "mg_pick_sources",
-    "mg_mtx_read_header", "mg_mtx_read",
+    "mg_graph_union", "mg_bfs_batch_sets", "mg_sssp_batch_sets",
+    "mg_mtx_read_header", "mg_mtx_read", "mg_autotune",
     "mg_server_config_default", "mg_server_create", "mg_server_add_graph", "mg_server_submit", "mg_server_wait",
     "mg_server_now_ms", "mg_server_get_stats", "mg_server_destroy", "mg_host_alloc", "mg_host_free",
 ]
@@ -71,7 +72,8 @@ class Options(C.Structure):
         ("pr_mode", C.c_int32), ("pr_teleport", C.c_float), ("pr_tol", C.c_float),
         ("pr_active_eps", C.c_float), ("max_queue_sizing", C.c_double),
         ("alpha", C.c_int32), ("beta", C.c_int32), ("batch_alpha", C.c_int32), ("batch_beta", C.c_int32),
-        ("sssp_delta", C.c_int32), ("ppr_iters", C.c_int32), ("reserved", C.c_int32 * 4),
+        ("sssp_delta", C.c_int32), ("ppr_iters", C.c_int32), ("sssp_bucket_edges", C.c_int32),
+        ("level_grid", C.c_int32), ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -92,6 +94,21 @@ class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("max_slots", C.c_int32), ("ppr_iters", C.c_int32),
                 ("ppr_teleport", C.c_float), ("steps_per_round", C.c_int32), ("sssp_delta", C.c_int32),
                 ("reserved", C.c_int32 * 7)]
+
+
+MG_TUNE_BFS, MG_TUNE_SSSP, MG_TUNE_BFS_BATCH, MG_TUNE_SSSP_BATCH = 0, 1, 2, 3
+TUNE_KINDS = {"bfs": MG_TUNE_BFS, "sssp": MG_TUNE_SSSP, "bfs_batch": MG_TUNE_BFS_BATCH,
+              "sssp_batch": MG_TUNE_SSSP_BATCH}
+# the mg_options fields mg_autotune chooses
+TUNABLE = ("alpha", "beta", "level_grid", "sssp_delta", "sssp_bucket_edges", "batch_alpha", "batch_beta")
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("start_ms", C.c_double), ("best_ms", C.c_double), ("candidates", C.c_int32),
+                ("reserved", C.c_int32 * 7)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class QueryInfo(C.Structure):
@@ -163,10 +180,17 @@ def lib() -> C.CDLL:
         "mg_cc": [C.c_void_p, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
         "mg_pagerank": [C.c_void_p, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
         "mg_bfs_batch": [C.c_void_p, I32P, C.c_int32, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
+        "mg_bfs_batch_sets": [C.c_void_p, I32P, I32P, C.c_int32, C.c_int32, C.c_void_p, C.POINTER(Options),
+                              C.POINTER(Stats)],
+        "mg_sssp_batch_sets": [C.c_void_p, I32P, I32P, C.c_int32, C.c_int32, C.c_void_p, C.POINTER(Options),
+                               C.POINTER(Stats)],
+        "mg_graph_union": [C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), I32P],
         "mg_sssp_batch": [C.c_void_p, I32P, C.c_int32, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
         "mg_ppr_batch": [C.c_void_p, I32P, C.c_int32, C.c_void_p, C.POINTER(Options), C.POINTER(Stats)],
     }
     search.update({
+        "mg_autotune": [C.c_void_p, C.c_int32, I32P, C.c_int32, C.c_int32, C.POINTER(Options), C.POINTER(Options),
+                        C.POINTER(TuneReport)],
         "mg_server_config_default": [C.POINTER(ServerConfig)],
         "mg_server_create": [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)],
         "mg_server_add_graph": [C.c_void_p, C.c_void_p, I32P],
@@ -356,6 +380,24 @@ class Graph:
         self.m = host.m
         self.flags = flags
 
+    @classmethod
+    def union(cls, parts: list["Graph"]) -> "Graph":
+        """The disjoint union of resident graphs (mg_graph_union); .offsets[i] is part i's first vertex."""
+        L = lib()
+        hs = (C.c_void_p * len(parts))(*[p.handle for p in parts])
+        offs = np.empty(len(parts) + 1, np.int32)
+        h = C.c_void_p()
+        check(L.mg_graph_union(len(parts), hs, C.byref(h), _p(offs)))
+        g = cls.__new__(cls)
+        g.host = None
+        g._h = h
+        n, m, flags = C.c_int32(), C.c_int64(), C.c_uint32()
+        check(L.mg_graph_info(h, C.byref(n), C.byref(m), C.byref(flags)))
+        g.n, g.m, g.flags = n.value, m.value, flags.value
+        g.offsets = offs
+        g.part_n = [p.n for p in parts]
+        return g
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().mg_graph_destroy(self._h)
@@ -435,6 +477,52 @@ class Graph:
         s = np.ascontiguousarray(srcs, dtype=np.int32)
         out = np.empty((len(s), self.n), np.int32) if copy else None
         return self._run(lib().mg_sssp_batch, _p(s), len(s), opt=opt, out=out, stats=stats)
+
+    def bfs_batch_sets(self, rows_of, srcs, k: int, opt: Options | None = None, stats: bool = False,
+                       copy: bool = True):
+        """Depths from source sets: row r (of k) from every srcs[i] with rows_of[i] == r (mg_bfs_batch_sets)."""
+        r = np.ascontiguousarray(rows_of, dtype=np.int32)
+        s = np.ascontiguousarray(srcs, dtype=np.int32)
+        out = np.empty((k, self.n), np.int32) if copy else None
+        return self._run(lib().mg_bfs_batch_sets, _p(r), _p(s), len(s), k, opt=opt, out=out, stats=stats)
+
+    def sssp_batch_sets(self, rows_of, srcs, k: int, opt: Options | None = None, stats: bool = False,
+                        copy: bool = True):
+        """Distances from source sets (mg_sssp_batch_sets); see bfs_batch_sets."""
+        r = np.ascontiguousarray(rows_of, dtype=np.int32)
+        s = np.ascontiguousarray(srcs, dtype=np.int32)
+        out = np.empty((k, self.n), np.int32) if copy else None
+        return self._run(lib().mg_sssp_batch_sets, _p(r), _p(s), len(s), k, opt=opt, out=out, stats=stats)
+
+    def parts_batch(self, kind: str, srcs_per_part, opt: Options | None = None, stats: bool = False,
+                    copy: bool = True):
+        """On a union graph: batched BFS ('bfs') or SSSP ('sssp') from srcs_per_part[i] (sources of part i, local
+        ids) in one search; row j of every part carries that part's j-th source.  Returns per-part row arrays
+        (len(srcs_per_part[i]) x n_i), or None with copy=False, and the stats when asked."""
+        rows, src = [], []
+        for i, ss in enumerate(srcs_per_part):
+            for j, v in enumerate(ss):
+                rows.append(j)
+                src.append(int(self.offsets[i]) + int(v))
+        k = max(len(ss) for ss in srcs_per_part)
+        fn = self.bfs_batch_sets if kind == "bfs" else self.sssp_batch_sets
+        res = fn(rows, src, k, opt, stats=True, copy=copy)
+        out, st = res
+        parts = None
+        if copy:
+            parts = [out[: len(ss), int(self.offsets[i]): int(self.offsets[i]) + self.part_n[i]]
+                     for i, ss in enumerate(srcs_per_part)]
+        return (parts, st) if stats else parts
+
+    def autotune(self, what: str, srcs, reps: int = 3, start: Options | None = None) -> tuple[Options, dict]:
+        """mg_autotune for `what` in TUNE_KINDS on sample sources srcs: the chosen options and the report
+        (start_ms / best_ms: device ms of the starting and the chosen setting, candidates timed)."""
+        s = np.ascontiguousarray(srcs, dtype=np.int32)
+        best, rep = Options(), TuneReport()
+        st = start if start is not None else options()
+        check(lib().mg_autotune(self.handle, TUNE_KINDS[what], _p(s), len(s), reps, C.byref(st), C.byref(best),
+                                C.byref(rep)))
+        return best, rep.as_dict()
 
     def ppr_batch(self, srcs, opt: Options | None = None, stats: bool = False, copy: bool = True):
         """Personalized PageRank from every source in srcs (at most MG_PPR_MAX) as a float32 len(srcs) x n array."""

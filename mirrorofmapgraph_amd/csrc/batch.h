@@ -171,6 +171,14 @@ int bfs_batch_reset(BatchDev &b, mg_graph *g, hipStream_t st, int iter_limit, in
                     int rows_to_clear);
 // Seeds the listed slots: their old visited bits and rows are cleared first.
 int bfs_batch_seed(BatchDev &b, mg_graph *g, hipStream_t st, const SeedList &sl);
+// Seeds npairs (slot, source) pairs, several sources per slot allowed (the slot's
+// frontier 0 is the set of its sources); the listed slots are cleared first.
+int bfs_batch_seed_pairs(BatchDev &b, mg_graph *g, hipStream_t st, const int32_t *slot_of, const int32_t *srcs,
+                         int npairs);
+// Argument checks of the *_batch_sets entry points: rows in [0, k), sources in
+// [0, n), every row with at least one source.
+int check_batch_sets(const char *fn, const mg_graph *g, const int32_t *slot_of, const int32_t *srcs, int32_t npairs,
+                     int32_t k);
 int bfs_batch_step(BatchDev &b, mg_graph *g, hipStream_t st);
 int sssp_batch_reset(BatchDev &b, mg_graph *g, hipStream_t st, int iter_limit, long long delta);
 // fresh: the context was just reset (distance columns already clear).

@@ -444,6 +444,31 @@ int bfs_batch_seed(BatchDev &b, mg_graph *g, hipStream_t st, const SeedList &sl)
     return MG_OK;
 }
 
+int bfs_batch_seed_pairs(BatchDev &b, mg_graph *g, hipStream_t st, const int32_t *slot_of, const int32_t *srcs,
+                         int npairs) {
+    BatchArgs a = bfs_args(b, g);
+    uint32_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    SlotList ls{};
+    for (int i = 0; i < npairs; ++i) {
+        const int s = slot_of[i];
+        if (!((m[s >> 5] >> (s & 31)) & 1u)) ls.slot[ls.n++] = s;
+        m[s >> 5] |= 1u << (s & 31);
+    }
+    // the slots' old state is cleared once, then every pair sets its source (a slot's
+    // sources together form its frontier 0)
+    hipLaunchKernelGGL(batch_prepare, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, st, a, host_mask(m), ls);
+    for (int i0 = 0; i0 < npairs; i0 += MG_BATCH_MAX) {
+        SeedList sl{};
+        for (int i = i0; i < npairs && i < i0 + MG_BATCH_MAX; ++i) {
+            sl.slot[sl.n] = slot_of[i];
+            sl.src[sl.n++] = srcs[i];
+        }
+        hipLaunchKernelGGL(batch_seed, dim3(1), dim3(MG_BATCH_MAX), 0, st, a, sl);
+    }
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
 int bfs_batch_step(BatchDev &b, mg_graph *g, hipStream_t st) {
     BatchArgs a = bfs_args(b, g);
     hipLaunchKernelGGL(batch_close, dim3(1), dim3(SMALL_BLK), 0, st, a);
@@ -655,29 +680,36 @@ int batch_beta(const mg_options &o) { return o.batch_beta > 0 ? o.batch_beta : 1
 
 using namespace mg;
 
-extern "C" int mg_bfs_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t *depth_out, const mg_options *opt,
-                            mg_stats *stats) {
-    CallTimer timer;
-    mg_options o;
-    int rc = begin_search(g, opt, &o);
-    if (rc) return rc;
-    std::lock_guard<std::mutex> lock(g->mu);
-    if ((rc = check_batch_sources("mg_bfs_batch", g, srcs, k))) return rc;
-    if (o.directed == 0 && !(g->flags & MG_GRAPH_SYMMETRIC))
-        return set_error(MG_ERR_UNSUPPORTED, "mg_bfs_batch: directed=0 needs a symmetric graph");
+namespace mg {
+
+int check_batch_sets(const char *fn, const mg_graph *g, const int32_t *slot_of, const int32_t *srcs, int32_t npairs,
+                     int32_t k) {
+    if (k < 1 || k > MG_BATCH_MAX) return set_error(MG_ERR_INVALID_ARG, "%s: k must be in [1, %d]", fn, MG_BATCH_MAX);
+    if (npairs < 1 || !srcs || !slot_of) return set_error(MG_ERR_INVALID_ARG, "%s: no (slot, source) pairs", fn);
+    uint32_t seen[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < npairs; ++i) {
+        if (slot_of[i] < 0 || slot_of[i] >= k)
+            return set_error(MG_ERR_INVALID_ARG, "%s: pair %d names row %d outside [0,%d)", fn, i, slot_of[i], k);
+        if (srcs[i] < 0 || srcs[i] >= g->n)
+            return set_error(MG_ERR_INVALID_ARG, "%s: pair %d: source %d outside [0,%d)", fn, i, srcs[i], g->n);
+        seen[slot_of[i] >> 5] |= 1u << (slot_of[i] & 31);
+    }
+    for (int s = 0; s < k; ++s)
+        if (!((seen[s >> 5] >> (s & 31)) & 1u)) return set_error(MG_ERR_INVALID_ARG, "%s: row %d has no source", fn, s);
+    return MG_OK;
+}
+
+// Batched BFS with npairs (row, source) pairs; every row of [0, k) has a source.
+static int bfs_batch_run(mg_graph *g, const int32_t *slot_of, const int32_t *srcs, int32_t npairs, int32_t k,
+                         int32_t *depth_out, const mg_options &o, mg_stats *stats, const CallTimer &timer) {
+    int rc;
     BatchDev *b = nullptr;
     if ((rc = graph_batch_dev(g, false, &b))) return rc;
-    SeedList sl{};
-    sl.n = k;
-    for (int s = 0; s < k; ++s) {
-        sl.slot[s] = s;
-        sl.src[s] = srcs[s];
-    }
     auto run = [&]() -> int {
         int rc;
         MG_HIP(hipEventRecord(g->ev0, g->stream));
         if ((rc = bfs_batch_reset(*b, g, g->stream, o.iter_num, batch_alpha(o), batch_beta(o), 0))) return rc;
-        if ((rc = bfs_batch_seed(*b, g, g->stream, sl))) return rc;
+        if ((rc = bfs_batch_seed_pairs(*b, g, g->stream, slot_of, srcs, npairs))) return rc;
         auto step = [&]() -> int { return bfs_batch_step(*b, g, g->stream); };
         // the step graph reads everything from device memory: one capture per graph
         const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n) + 3;
@@ -693,4 +725,34 @@ extern "C" int mg_bfs_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t
     }
     if (rc == MG_OK && stats) stats->overflow_regrows = regrows;
     return rc;
+}
+
+}  // namespace mg
+
+extern "C" int mg_bfs_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t *depth_out, const mg_options *opt,
+                            mg_stats *stats) {
+    CallTimer timer;
+    mg_options o;
+    int rc = begin_search(g, opt, &o);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g->mu);
+    if ((rc = check_batch_sources("mg_bfs_batch", g, srcs, k))) return rc;
+    if (o.directed == 0 && !(g->flags & MG_GRAPH_SYMMETRIC))
+        return set_error(MG_ERR_UNSUPPORTED, "mg_bfs_batch: directed=0 needs a symmetric graph");
+    int32_t rows[MG_BATCH_MAX];
+    for (int s = 0; s < k; ++s) rows[s] = s;
+    return bfs_batch_run(g, rows, srcs, k, k, depth_out, o, stats, timer);
+}
+
+extern "C" int mg_bfs_batch_sets(mg_graph *g, const int32_t *row_of, const int32_t *srcs, int32_t npairs, int32_t k,
+                                 int32_t *depth_out, const mg_options *opt, mg_stats *stats) {
+    CallTimer timer;
+    mg_options o;
+    int rc = begin_search(g, opt, &o);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g->mu);
+    if ((rc = check_batch_sets("mg_bfs_batch_sets", g, row_of, srcs, npairs, k))) return rc;
+    if (o.directed == 0 && !(g->flags & MG_GRAPH_SYMMETRIC))
+        return set_error(MG_ERR_UNSUPPORTED, "mg_bfs_batch_sets: directed=0 needs a symmetric graph");
+    return bfs_batch_run(g, row_of, srcs, npairs, k, depth_out, o, stats, timer);
 }

@@ -612,33 +612,39 @@ long long sssp_delta(const mg_graph *g, const mg_options &o) {
 
 using namespace mg;
 
-extern "C" int mg_sssp_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t *dist_out, const mg_options *opt,
-                             mg_stats *stats) {
-    CallTimer timer;
-    mg_options o;
-    int rc = begin_search(g, opt, &o);
-    if (rc) return rc;
-    std::lock_guard<std::mutex> lock(g->mu);
-    if ((rc = check_batch_sources("mg_sssp_batch", g, srcs, k))) return rc;
-    if (!g->w) return set_error(MG_ERR_UNSUPPORTED, "mg_sssp_batch: the graph has no edge weights");
-    if (g->min_weight < 0) return set_error(MG_ERR_UNSUPPORTED, "mg_sssp_batch: negative edge weights");
+namespace mg {
+
+static int sssp_batch_checks(const char *fn, const mg_graph *g, const mg_options &o) {
+    if (!g->w) return set_error(MG_ERR_UNSUPPORTED, "%s: the graph has no edge weights", fn);
+    if (g->min_weight < 0) return set_error(MG_ERR_UNSUPPORTED, "%s: negative edge weights", fn);
     if (o.directed == 0 && !(g->flags & MG_GRAPH_SYMMETRIC))
-        return set_error(MG_ERR_UNSUPPORTED, "mg_sssp_batch: directed=0 needs a symmetric graph");
+        return set_error(MG_ERR_UNSUPPORTED, "%s: directed=0 needs a symmetric graph", fn);
+    return MG_OK;
+}
+
+// Batched SSSP with npairs (row, source) pairs; every row of [0, k) has a source.
+static int sssp_batch_run(mg_graph *g, const int32_t *slot_of, const int32_t *srcs, int32_t npairs, int32_t k,
+                          int32_t *dist_out, const mg_options &o, mg_stats *stats, const CallTimer &timer) {
+    int rc;
     BatchDev *b = nullptr;
     if ((rc = graph_batch_dev(g, true, &b))) return rc;
     const long long delta = sssp_delta(g, o);
-    SeedList sl{};
     SlotList all{};
-    sl.n = all.n = k;
-    for (int s = 0; s < k; ++s) {
-        sl.slot[s] = all.slot[s] = s;
-        sl.src[s] = srcs[s];
-    }
+    all.n = k;
+    for (int s = 0; s < k; ++s) all.slot[s] = s;
     auto run = [&]() -> int {
         int rc;
         MG_HIP(hipEventRecord(g->ev0, g->stream));
         if ((rc = sssp_batch_reset(*b, g, g->stream, o.iter_num, delta))) return rc;
-        if ((rc = sssp_batch_seed(*b, g, g->stream, sl, delta, true))) return rc;
+        // a fresh context: the distance columns are clear, every pair only sets its source
+        for (int i0 = 0; i0 < npairs; i0 += MG_BATCH_MAX) {
+            SeedList sl{};
+            for (int i = i0; i < npairs && i < i0 + MG_BATCH_MAX; ++i) {
+                sl.slot[sl.n] = slot_of[i];
+                sl.src[sl.n++] = srcs[i];
+            }
+            if ((rc = sssp_batch_seed(*b, g, g->stream, sl, delta, true))) return rc;
+        }
         auto step = [&]() -> int { return sssp_batch_step(*b, g, g->stream, delta); };
         if ((rc = drive(g, step, (int64_t)o.iter_num + 3, KIND_BATCH_SSSP, mix64((uint64_t)delta), b->ctl, b->h_ctl)))
             return rc;
@@ -657,4 +663,32 @@ extern "C" int mg_sssp_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_
     }
     if (rc == MG_OK && stats) stats->overflow_regrows = regrows;
     return rc;
+}
+
+}  // namespace mg
+
+extern "C" int mg_sssp_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t *dist_out, const mg_options *opt,
+                             mg_stats *stats) {
+    CallTimer timer;
+    mg_options o;
+    int rc = begin_search(g, opt, &o);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g->mu);
+    if ((rc = check_batch_sources("mg_sssp_batch", g, srcs, k))) return rc;
+    if ((rc = sssp_batch_checks("mg_sssp_batch", g, o))) return rc;
+    int32_t rows[MG_BATCH_MAX];
+    for (int s = 0; s < k; ++s) rows[s] = s;
+    return sssp_batch_run(g, rows, srcs, k, k, dist_out, o, stats, timer);
+}
+
+extern "C" int mg_sssp_batch_sets(mg_graph *g, const int32_t *row_of, const int32_t *srcs, int32_t npairs, int32_t k,
+                                  int32_t *dist_out, const mg_options *opt, mg_stats *stats) {
+    CallTimer timer;
+    mg_options o;
+    int rc = begin_search(g, opt, &o);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g->mu);
+    if ((rc = check_batch_sets("mg_sssp_batch_sets", g, row_of, srcs, npairs, k))) return rc;
+    if ((rc = sssp_batch_checks("mg_sssp_batch_sets", g, o))) return rc;
+    return sssp_batch_run(g, row_of, srcs, npairs, k, dist_out, o, stats, timer);
 }

@@ -588,7 +588,7 @@ static int bfs_run(mg_graph *g, int src, int *depth_out, const mg_options &o, mg
     a.depth = ws.vals;
     a.visited = ws.visited;
     a.hard = ws.hard;
-    a.G = grid_blocks(std::max<int64_t>(g->n / 4, g->m / 64), BLK);
+    a.G = o.level_grid > 0 ? o.level_grid : grid_blocks(std::max<int64_t>(g->n / 4, g->m / 64), BLK);
     const long long m_views = g->m * a.fa.nviews;
 
     MG_HIP(hipEventRecord(g->ev0, g->stream));
@@ -604,7 +604,7 @@ static int bfs_run(mg_graph *g, int src, int *depth_out, const mg_options &o, mg
         return MG_OK;
     };
     const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n) + 3;
-    if ((rc = drive(g, step, max_steps, KIND_BFS, (uint64_t)o.directed))) return rc;
+    if ((rc = drive(g, step, max_steps, KIND_BFS, mix64((uint64_t)o.directed + 2 * (uint64_t)a.G)))) return rc;
     rc = finish_search(g, stats, ws.vals, -1, depth_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
     if (rc == MG_OK && o.verbose) rc = print_trace(g, "bfs", "top-down", "bottom-up");
     return rc;

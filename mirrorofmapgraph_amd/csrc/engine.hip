@@ -206,10 +206,50 @@ int begin_search(mg_graph *g, const mg_options *opt, mg_options *resolved) {
         return set_error(MG_ERR_INVALID_ARG, "directed must be 0 or 1");
     if (resolved->engine != MG_ENGINE_AUTO && resolved->engine != MG_ENGINE_GAS)
         return set_error(MG_ERR_INVALID_ARG, "unknown engine %d", resolved->engine);
+    if (resolved->level_grid < 0 || resolved->level_grid > 65536)
+        return set_error(MG_ERR_INVALID_ARG, "level_grid must be in [0, 65536]");
     if (resolved->directed == 0 && !(g->flags & (MG_GRAPH_BUILD_CSC | MG_GRAPH_SYMMETRIC)))
         return set_error(MG_ERR_UNSUPPORTED, "directed=0 needs a graph created with MG_GRAPH_BUILD_CSC or MG_GRAPH_SYMMETRIC");
     MG_HIP(hipSetDevice(g->device));
     return ensure_workspace(g);
+}
+
+// A graph object on `device` with its stream and events and no arrays yet (the
+// caller fills them; mg_graph_destroy releases a partly built one).
+static int graph_shell(int device, int32_t n, int64_t m, uint32_t flags, mg_graph **out) {
+    *out = nullptr;
+    mg_graph *g = new (std::nothrow) mg_graph();
+    if (!g) return set_error(MG_ERR_OOM, "host allocation failed");
+    g->steps = new (std::nothrow) mg_graph_steps();
+    if (!g->steps) {
+        delete g;
+        return set_error(MG_ERR_OOM, "host allocation failed");
+    }
+    g->device = device;
+    g->n = n;
+    g->m = m;
+    g->flags = flags;
+    if (hipSetDevice(g->device) != hipSuccess) {
+        mg_graph_destroy(g);
+        return set_error(MG_ERR_DEVICE, "hipSetDevice failed");
+    }
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess) {
+        mg_graph_destroy(g);
+        return set_error(MG_ERR_DEVICE, "stream/event creation failed");
+    }
+    *out = g;
+    return MG_OK;
+}
+
+// Part p of a disjoint union: its offsets shifted by the edges before it, its column
+// ids by the vertices before it.
+__global__ void union_offsets(const int *ro, int n, int eoff, int *out) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) out[v] = ro[v] + eoff;
+}
+__global__ void union_columns(const int *col, int64_t m, int voff, int *out) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x)
+        out[e] = col[e] + voff;
 }
 
 int deliver(mg_graph *g, void *dst, const void *src, size_t bytes, bool out_on_device) {
@@ -288,26 +328,13 @@ int mg_graph_create(const int32_t *row_offsets, const int32_t *col_idx, const in
     int ndev = mg_device_count();
     if (ndev == 0) return set_error(MG_ERR_NO_DEVICE, "no HIP device visible");
 
-    mg_graph *g = new (std::nothrow) mg_graph();
-    if (!g) return set_error(MG_ERR_OOM, "host allocation failed");
-    g->steps = new (std::nothrow) mg_graph_steps();
-    if (!g->steps) {
-        delete g;
-        return set_error(MG_ERR_OOM, "host allocation failed");
-    }
-    g->device = t_device;
-    g->n = n;
-    g->m = m;
-    g->flags = flags;
-    int rc = MG_OK;
+    mg_graph *g = nullptr;
+    int rc = graph_shell(t_device, n, m, flags, &g);
+    if (rc) return rc;
     auto fail = [&](int code) {
         mg_graph_destroy(g);
         return code;
     };
-    if (hipSetDevice(g->device) != hipSuccess) return fail(set_error(MG_ERR_DEVICE, "hipSetDevice failed"));
-    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess)
-        return fail(set_error(MG_ERR_DEVICE, "stream/event creation failed"));
     if ((rc = dalloc(&g->row_off, (size_t)n + 1)) || (rc = dalloc(&g->col, (size_t)m))) return fail(rc);
     g->topo_bytes = 4 * ((int64_t)n + 1 + m);
     if (hipMemcpy(g->row_off, row_offsets, 4 * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
@@ -341,6 +368,73 @@ int mg_graph_create(const int32_t *row_offsets, const int32_t *col_idx, const in
             return fail(set_error(MG_ERR_DEVICE, "host-to-device copy of the CSC failed"));
         g->topo_bytes += 4 * ((int64_t)n + 1 + 2 * m);
     }
+    if ((rc = ensure_workspace(g))) return fail(rc);
+    *out = g;
+    return MG_OK;
+}
+
+int mg_graph_union(int32_t ngraphs, mg_graph *const *parts, mg_graph **out, int32_t *vertex_offsets) {
+    clear_error();
+    if (!out) return set_error(MG_ERR_INVALID_ARG, "mg_graph_union: out is null");
+    *out = nullptr;
+    if (ngraphs < 1 || !parts) return set_error(MG_ERR_INVALID_ARG, "mg_graph_union: no graphs");
+    int64_t N = 0, M = 0;
+    uint32_t flags = MG_GRAPH_WEIGHTED | MG_GRAPH_SYMMETRIC;
+    for (int i = 0; i < ngraphs; ++i) {
+        const mg_graph *p = parts[i];
+        if (!p) return set_error(MG_ERR_INVALID_ARG, "mg_graph_union: graph %d is null", i);
+        if (p->device != parts[0]->device)
+            return set_error(MG_ERR_INVALID_ARG, "mg_graph_union: graphs on different devices");
+        N += p->n;
+        M += p->m;
+        flags &= p->flags;
+    }
+    if (N >= INT32_MAX || M > INT32_MAX)
+        return set_error(MG_ERR_INVALID_ARG, "mg_graph_union: %lld vertices / %lld edges exceed int32", (long long)N,
+                         (long long)M);
+    mg_graph *g = nullptr;
+    int rc = graph_shell(parts[0]->device, (int32_t)N, M, flags, &g);
+    if (rc) return rc;
+    auto fail = [&](int code) {
+        mg_graph_destroy(g);
+        return code;
+    };
+    if ((rc = dalloc(&g->row_off, (size_t)N + 1)) || (rc = dalloc(&g->col, (size_t)M))) return fail(rc);
+    if ((flags & MG_GRAPH_WEIGHTED) && (rc = dalloc(&g->w, (size_t)M))) return fail(rc);
+    g->topo_bytes = 4 * (N + 1 + M) + ((flags & MG_GRAPH_WEIGHTED) ? 4 * M : 0);
+    int64_t voff = 0, eoff = 0;
+    double wsum = 0;
+    g->min_weight = INT32_MAX;
+    g->max_weight = INT32_MIN;
+    for (int i = 0; i < ngraphs; ++i) {
+        const mg_graph *p = parts[i];
+        if (vertex_offsets) vertex_offsets[i] = (int32_t)voff;
+        // the part's own stream is idle past this point: its arrays are read on g's stream
+        MG_HIP(hipStreamSynchronize(p->stream));
+        if (p->n > 0)
+            hipLaunchKernelGGL(union_offsets, dim3(grid_blocks(p->n, BLK)), dim3(BLK), 0, g->stream, p->row_off, p->n,
+                               (int)eoff, g->row_off + voff);
+        if (p->m > 0) {
+            hipLaunchKernelGGL(union_columns, dim3(grid_blocks(p->m, BLK)), dim3(BLK), 0, g->stream, p->col, p->m,
+                               (int)voff, g->col + eoff);
+            if (flags & MG_GRAPH_WEIGHTED)
+                MG_HIP(hipMemcpyAsync(g->w + eoff, p->w, 4 * (size_t)p->m, hipMemcpyDeviceToDevice, g->stream));
+        }
+        if (p->m > 0) {
+            g->min_weight = std::min(g->min_weight, p->min_weight);
+            g->max_weight = std::max(g->max_weight, p->max_weight);
+            wsum += p->mean_weight * (double)p->m;
+        }
+        voff += p->n;
+        eoff += p->m;
+    }
+    if (vertex_offsets) vertex_offsets[ngraphs] = (int32_t)voff;
+    const int32_t last = (int32_t)M;
+    MG_HIP(hipMemcpyAsync(g->row_off + N, &last, 4, hipMemcpyHostToDevice, g->stream));
+    MG_HIP(hipGetLastError());
+    MG_HIP(hipStreamSynchronize(g->stream));
+    if (M == 0) g->min_weight = g->max_weight = 0;
+    g->mean_weight = M ? wsum / (double)M : 0;
     if ((rc = ensure_workspace(g))) return fail(rc);
     *out = g;
     return MG_OK;

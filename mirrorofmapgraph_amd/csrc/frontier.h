@@ -21,6 +21,9 @@
 // spreads over the whole GPU (a 160K-edge hub becomes ~160 workgroup-chunks).
 #pragma once
 
+#include <type_traits>
+#include <utility>
+
 #include "devutil.h"
 #include "engine.h"
 
@@ -129,16 +132,24 @@ __device__ T block_min(T x) {
 // workgroup of the level (a few thousand, written on every XCD).  The kernel that
 // starts a search clears them.
 constexpr int PART_GROUPS = 256;
+// The slot's fourth field carries a minimum (the smallest pending distance of a
+// bucketed SSSP level) as the key PART_MIN_BASE - value under atomicMax, so a cleared
+// slot (0) means "none" for every user.
+constexpr long long PART_MIN_BASE = 1LL << 62;
 template <int BLOCK>
-__device__ __forceinline__ void write_part(const FrontierArgs &fa, long long nf, long long mf, long long ex) {
+__device__ __forceinline__ void write_part(const FrontierArgs &fa, long long nf, long long mf, long long ex,
+                                           long long mn = LLONG_MAX) {
     nf = block_sum<BLOCK>(nf);
     mf = block_sum<BLOCK>(mf);
     ex = block_sum<BLOCK>(ex);
+    mn = block_min<BLOCK>(mn);
     if (threadIdx.x == 0) {
         LevelPart *p = &fa.part[blockIdx.x % PART_GROUPS];
         if (nf) atomicAdd((unsigned long long *)&p->nf, (unsigned long long)nf);
         if (mf) atomicAdd((unsigned long long *)&p->mf, (unsigned long long)mf);
         if (ex) atomicAdd((unsigned long long *)&p->ex, (unsigned long long)ex);
+        if (mn < PART_MIN_BASE)
+            atomicMax((unsigned long long *)&p->pad, (unsigned long long)(PART_MIN_BASE - mn));
     }
 }
 
@@ -148,21 +159,25 @@ __device__ __forceinline__ void clear_parts(const FrontierArgs &fa) {
 }
 
 // Sums the level counters (whole workgroup, BLOCK >= PART_GROUPS) and clears them.
+// The minimum written with write_part comes back in *mn (LLONG_MAX when none).
 template <int BLOCK>
 __device__ __forceinline__ void take_level_parts(const FrontierArgs &fa, long long &nf, long long &mf,
-                                                 long long &ex) {
+                                                 long long &ex, long long *mn = nullptr) {
     static_assert(BLOCK >= PART_GROUPS, "one slot per thread");
     nf = mf = ex = 0;
+    long long m = LLONG_MAX;
     if (threadIdx.x < PART_GROUPS) {
         const LevelPart p = fa.part[threadIdx.x];
         nf = p.nf;
         mf = p.mf;
         ex = p.ex;
+        if (p.pad) m = PART_MIN_BASE - p.pad;
         fa.part[threadIdx.x] = LevelPart{0, 0, 0, 0};
     }
     nf = block_sum<BLOCK>(nf);
     mf = block_sum<BLOCK>(mf);
     ex = block_sum<BLOCK>(ex);
+    if (mn) *mn = block_min<BLOCK>(m);
 }
 
 // Clears bm[(L+2)%3] (frontier L-1) during level L by a sweep of 16-byte stores
@@ -382,6 +397,15 @@ __device__ void expand_bitmap(const FrontierArgs &fa, P &pol, int view, int L, c
     }
 }
 
+// Policies that postpone frontier vertices define bool defer(int v, int L): called
+// for every frontier vertex with edges in the view (once per expanded view) before
+// they are expanded; true skips the vertex (the policy carries it to the next
+// frontier itself).
+template <class P, class = void>
+struct policy_defers : std::false_type {};
+template <class P>
+struct policy_defers<P, std::void_t<decltype(std::declval<P &>().defer(0, 0))>> : std::true_type {};
+
 // Multi-workgroup push over the frontier bitmap `fr` without workgroup barriers in
 // the edge loop: each round a workgroup compacts BLOCK/2 bitmap words into LDS; a
 // lane walks the edges of each listed vertex of degree <= LANE_T itself, longer
@@ -415,6 +439,8 @@ __device__ void expand_lanes(const FrontierArgs &fa, P &pol, int view, int L, co
                 v = s_q[i];
                 beg = off[v];
                 deg = off[v + 1] - beg;
+                if constexpr (policy_defers<P>::value)
+                    if (deg > 0 && pol.defer(v, L)) deg = 0;
             }
             const int nt = deg > LANE_T ? (deg + TASK - 1) / TASK : 0;
             int ttot;

@@ -34,11 +34,18 @@ struct SsspArgs {
     const ChunkRec *dpull;
     int ndpull;
     int push_range;  // symmetric graph with dense tasks: wide push levels use range_push
-    // buckets of the small-frontier loop: width, far list (n entries) and its bitmap
+    // buckets: width, far list of the small-frontier loop (n entries) and its bitmap
     long long delta;
     int *far;
     uint32_t *farbm;
+    // bucketed whole-GPU levels: a frontier vertex at or above ctl->thr is carried to
+    // the next frontier unrelaxed; once the pending vertices' out-edges fall below
+    // bucket_edges the bound is lifted (UNBUCKETED) and the tail runs as Bellman-Ford
+    int bucketed;
+    long long bucket_edges;
 };
+
+constexpr long long UNBUCKETED = LLONG_MAX / 4;  // a bucket bound no distance reaches
 
 __device__ __forceinline__ long long edge_w(const SsspArgs &a, int view, int slot) {
     return a.w[edge_id(a.fa, view, slot)];
@@ -50,14 +57,19 @@ __device__ __forceinline__ long long edge_w(const SsspArgs &a, int view, int slo
 // must be a vertex even where ok[k] is false (callers load from clamped slots).  A
 // stale (larger) du[k] only causes a redundant relaxation: the target's distance
 // dropped this level either way, so it belongs to the next frontier.
+// mn takes the smallest improved distance (the next frontier's pending minimum, for
+// the bucket bound).
 template <int K>
 __device__ __forceinline__ void relax_k(const SsspArgs &a, uint32_t *nx, const int (&u)[K],
                                         const long long (&nd)[K], const int (&du)[K], const bool (&ok)[K],
-                                        long long &nf, long long &mf) {
+                                        long long &nf, long long &mf, long long &mn) {
     uint32_t need = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k)
-        if (ok[k] && nd[k] < (long long)du[k]) need |= 1u << k;
+        if (ok[k] && nd[k] < (long long)du[k]) {
+            need |= 1u << k;
+            mn = nd[k] < mn ? nd[k] : mn;
+        }
 #pragma unroll
     for (int k = 0; k < K; ++k)
         if ((need >> k) & 1u) atomicMin(&a.dist[u[k]], (int)nd[k]);
@@ -67,14 +79,32 @@ __device__ __forceinline__ void relax_k(const SsspArgs &a, uint32_t *nx, const i
 struct SsspPush {
     const SsspArgs &a;
     long long nf = 0, mf = 0, ex = 0;
-    __device__ explicit SsspPush(const SsspArgs &args) : a(args) {}
+    long long mn = LLONG_MAX;  // smallest distance entering the next frontier
+    long long thr;             // bucket bound of this level (UNBUCKETED: relax every frontier vertex)
+    __device__ explicit SsspPush(const SsspArgs &args)
+        : a(args), thr(args.bucketed ? args.fa.ctl->thr : UNBUCKETED) {}
     // device-coherent: the small-frontier loop reads distances lowered earlier in the launch
     __device__ int load(int v) const { return ld_dev(&a.dist[v]); }
+    // Bucketed levels: a frontier vertex at or above the bound waits in the next
+    // frontier, unrelaxed.  A stale (larger) distance read here only postpones a vertex
+    // that stays pending; the one that is relaxed reads its distance again (load).
+    __device__ bool defer(int v, int L) {
+        if (thr >= UNBUCKETED) return false;
+        const long long dv = a.dist[v];
+        if (dv < thr) return false;
+        mn = dv < mn ? dv : mn;
+        if (set_bit_first(a.fa.bm[(L + 1) % 3], v)) {
+            ++nf;
+            mf += view_degree(a.fa, v);
+        }
+        return true;
+    }
     // u's distance dropped (to at most nd) this level: mark it once.  A stale
     // (larger) dist[u] read only causes a redundant relaxation: dist[u] dropped
     // this level either way, so u belongs to the next frontier.
     __device__ bool relax(long long nd, int du, int u, int L) {
         if (nd >= (long long)du) return false;
+        mn = nd < mn ? nd : mn;
         atomicMin(&a.dist[u], (int)nd);
         if (!mark_next(a.fa, L, u)) return false;
         ++nf;
@@ -119,7 +149,7 @@ struct SsspPush {
                 nd[k] = (long long)d + wt[k];
                 ex += ok[k];
             }
-            relax_k<4>(a, nx, u, nd, du, ok, nf, mf);
+            relax_k<4>(a, nx, u, nd, du, ok, nf, mf, mn);
         }
     }
 };
@@ -373,7 +403,7 @@ __device__ __forceinline__ bool range_push_level(const SsspArgs &a, const Ctl *c
 }
 
 __device__ __forceinline__ void range_push(const SsspArgs &a, int L, DenseLds &lds, long long &nf, long long &mf,
-                                           long long &ex) {
+                                           long long &ex, long long &mn) {
     const FrontierArgs &fa = a.fa;
     const uint32_t *fr = fa.bm[L % 3];
     uint32_t *nx = fa.bm[(L + 1) % 3];
@@ -382,11 +412,25 @@ __device__ __forceinline__ void range_push(const SsspArgs &a, int L, DenseLds &l
     const int lane = lane_id(), wv = (int)threadIdx.x / WAVE;
     int *pre = lds.pre[wv], *begs = lds.beg[wv], *dsrc = lds.min[wv];
     const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
+    const long long thr = a.bucketed ? fa.ctl->thr : UNBUCKETED;
+    // a frontier vertex at or above the bucket bound waits in the next frontier (each
+    // piece of a long list checks it; the first to set the bit counts it)
+    auto defer = [&](int v, long long dv) {
+        mn = dv < mn ? dv : mn;
+        if (set_bit_first(nx, v)) {
+            ++nf;
+            mf += view_degree(fa, v);
+        }
+    };
     for (int t = wave; t < a.ndpull; t += nwaves) {
         const ChunkRec r = a.dpull[t];
         if (r.v >= 0) {
             if (!test_bit(fr, r.v)) continue;  // wave-uniform
             const long long dv = a.dist[r.v];
+            if (dv >= thr) {  // wave-uniform
+                if (lane == 0) defer(r.v, dv);
+                continue;
+            }
             for (int e = r.beg + lane; e < r.end; e += 4 * WAVE) {
                 int sl[4], u[4], wt[4], du[4];
 #pragma unroll
@@ -405,12 +449,12 @@ __device__ __forceinline__ void range_push(const SsspArgs &a, int L, DenseLds &l
                     nd[k] = dv + wt[k];
                     ex += ok[k];
                 }
-                relax_k<4>(a, nx, u, nd, du, ok, nf, mf);
+                relax_k<4>(a, nx, u, nd, du, ok, nf, mf, mn);
             }
             continue;
         }
         const int va = r.beg, vb = r.end;
-        int beg[DENSE_PER], len[DENSE_PER];
+        int beg[DENSE_PER], len[DENSE_PER], dvs[DENSE_PER];
 #pragma unroll
         for (int j = 0; j < DENSE_PER; ++j) {
             const int v = va + j * WAVE + lane;
@@ -418,7 +462,14 @@ __device__ __forceinline__ void range_push(const SsspArgs &a, int L, DenseLds &l
             const int o0 = off[vv], o1 = off[vv + 1];
             beg[j] = o0;
             len[j] = v < vb && o1 - o0 <= DENSE_BIG && test_bit(fr, v) ? o1 - o0 : 0;
+            dvs[j] = a.dist[vv];
         }
+#pragma unroll
+        for (int j = 0; j < DENSE_PER; ++j)
+            if (len[j] > 0 && dvs[j] >= thr) {
+                defer(va + j * WAVE + lane, dvs[j]);
+                len[j] = 0;
+            }
         int total = 0;
 #pragma unroll
         for (int j = 0; j < DENSE_PER; ++j) {
@@ -428,10 +479,9 @@ __device__ __forceinline__ void range_push(const SsspArgs &a, int L, DenseLds &l
                 const int y = __shfl_up(incl, d);
                 if (lane >= d) incl += y;
             }
-            const int v = va + j * WAVE + lane;
             pre[j * WAVE + lane + 1] = total + incl;
             begs[j * WAVE + lane] = beg[j];
-            dsrc[j * WAVE + lane] = len[j] ? a.dist[v] : 0;
+            dsrc[j * WAVE + lane] = len[j] ? dvs[j] : 0;
             total += __shfl(incl, WAVE - 1);
         }
         if (lane == 0) pre[0] = 0;
@@ -474,16 +524,26 @@ __device__ __forceinline__ void range_push(const SsspArgs &a, int L, DenseLds &l
                 nd[k] = (long long)dsrc[own[k]] + wt[k];
                 ex += ok[k];
             }
-            relax_k<DENSE_WIN>(a, nx, u, nd, du, ok, nf, mf);
+            relax_k<DENSE_WIN>(a, nx, u, nd, du, ok, nf, mf, mn);
         }
         __builtin_amdgcn_wave_barrier();  // the next task rewrites this wave's LDS rows
     }
 }
 
-__device__ void sssp_close(const SsspArgs &a, int L, long long nf, long long mf) {
+// Closes a whole-GPU level.  nf / mf: the next frontier's size and out-edges
+// (waiting vertices included); mn: its smallest distance.  Bucketed: the bound is
+// lifted once the pending vertices have fewer than bucket_edges out-edges (the tail of
+// low-degree vertices is cheaper to relax unbucketed than at one level per bucket,
+// each carrying the waiting vertices along); otherwise, when no pending vertex lies
+// below the bound, it moves to the bucket of the smallest one.
+__device__ void sssp_close(const SsspArgs &a, int L, long long nf, long long mf, long long mn) {
     Ctl *c = a.fa.ctl;
     int dir = c->dir;
     trace_level(a.fa, L, dir, nf, mf);
+    if (a.bucketed && c->thr < UNBUCKETED) {
+        if (mf < a.bucket_edges) c->thr = UNBUCKETED;
+        else if (mn >= c->thr) c->thr = (mn / a.delta + 1) * a.delta;
+    }
     if (dir == DIR_PULL) c->bu_levels++;
     if (c->can_bu) {
         const long long m = c->m_unexplored;  // total edges (set at seed)
@@ -564,11 +624,11 @@ __global__ __launch_bounds__(SMALL_BLK) void sssp_small(SsspArgs a) {
     __shared__ long long s_m, s_thr;
     if (c->pending) {
         const int L = c->level;
-        long long nf, mf, ex;
-        take_level_parts<SMALL_BLK>(fa, nf, mf, ex);
+        long long nf, mf, ex, mn;
+        take_level_parts<SMALL_BLK>(fa, nf, mf, ex, &mn);
         if (threadIdx.x == 0) {
             c->edges_examined += ex;
-            sssp_close(a, L, nf, mf);
+            sssp_close(a, L, nf, mf, mn);
             c->pending = 0;
             c->qvalid = 0;
         }
@@ -906,7 +966,7 @@ __global__ __launch_bounds__(BLK) void sssp_level_a(SsspArgs a, int view) {
         }
         SsspPush pol(a);
         expand_lanes<BLK>(fa, pol, view, L, fa.bm[L % 3]);
-        write_part<BLK>(fa, pol.nf, pol.mf, pol.ex);
+        write_part<BLK>(fa, pol.nf, pol.mf, pol.ex, pol.mn);
         return;
     }
     // a pull level is sssp_level_b's (the dense tasks, one launch)
@@ -917,22 +977,23 @@ __global__ __launch_bounds__(BLK) void sssp_level_b(SsspArgs a) {
     const FrontierArgs &fa = a.fa;
     if (c->done) return;
     const int L = c->level;
-    long long nf = 0, mf = 0, ex = 0;
+    long long nf = 0, mf = 0, ex = 0, mn = LLONG_MAX;
     __shared__ DenseLds lds;
     if (c->dir == DIR_PUSH && range_push_level(a, c)) {
-        range_push(a, L, lds, nf, mf, ex);
+        range_push(a, L, lds, nf, mf, ex, mn);
     } else if (c->dir == DIR_PUSH) {
         SsspPush pol(a);
         expand_tasks<BLK>(fa, pol, L);
         nf = pol.nf;
         mf = pol.mf;
         ex = pol.ex;
+        mn = pol.mn;
     } else if (fa.eid[2]) {
         dense_pull<true>(a, L, lds, nf, mf, ex);
     } else {
         dense_pull<false>(a, L, lds, nf, mf, ex);
     }
-    write_part<BLK>(fa, nf, mf, ex);
+    write_part<BLK>(fa, nf, mf, ex, mn);
     if (blockIdx.x == 0 && threadIdx.x == 0) c->pending = 1;
 }
 
@@ -941,6 +1002,23 @@ __global__ __launch_bounds__(BLK) void sssp_level_b(SsspArgs a) {
 using namespace mg;
 
 namespace mg {
+
+// Bucket width of mg_sssp for options o (sssp_delta > 0 as given, < 0 unbucketed,
+// 0 automatic: four maximum weights -- a frontier vertex's improvements then stay
+// within the next few buckets, and wider buckets mean fewer, fuller rounds).
+long long sssp_width(const mg_graph *g, const mg_options &o) {
+    if (o.sssp_delta < 0) return UNBUCKETED;
+    if (o.sssp_delta > 0) return o.sssp_delta;
+    return std::max<long long>(1, 4LL * g->max_weight);
+}
+
+// Whole-GPU levels are bucketed only for an explicit width: on R-MAT scale 22
+// (weights 1..64) bucket-ordered push levels relax each edge about once (1.0 x m
+// against 3 x m) but ran slower than unbucketed levels with dense pulls (2.7 ms at
+// width 1, 4.2 ms at width 4, against 2.3 ms), the pushes' atomics and the carried
+// waiting vertices costing more than the saved relaxations; the narrow levels are
+// always bucketed (grids: 1.12 G relaxations unbucketed, 133 M bucketed).
+bool sssp_bucketed_levels(const mg_options &o) { return o.sssp_delta > 0; }
 
 // One SSSP search with the current queues (MG_ERR_OVERFLOW when the edge-chunk queue
 // overflowed; the caller regrows it and runs again).
@@ -952,17 +1030,17 @@ static int sssp_run(mg_graph *g, int src, int *dist_out, const mg_options &o, mg
     a.fa = make_frontier_args(g, o.directed);
     a.dist = ws.vals;
     a.w = g->w;
-    a.G = grid_blocks_cap(std::max<int64_t>(g->n / 4, g->m / 64), BLK, BIG_GRID);
-    // bucket width of the small-frontier loop: the option, else four maximum edge weights
-    // (a frontier vertex's improvements then stay within the next few buckets; wider
-    // buckets mean fewer, fuller rounds); sssp_delta < 0: unbucketed
-    a.delta = o.sssp_delta < 0 ? LLONG_MAX / 4
-                               : o.sssp_delta > 0 ? (long long)o.sssp_delta
-                                                  : std::max<long long>(1, 4LL * g->max_weight);
+    a.G = o.level_grid > 0 ? o.level_grid : grid_blocks_cap(std::max<int64_t>(g->n / 4, g->m / 64), BLK, BIG_GRID);
+    a.delta = sssp_width(g, o);
+    a.bucketed = sssp_bucketed_levels(o);
+    a.bucket_edges = o.sssp_bucket_edges < 0 ? 0 : o.sssp_bucket_edges > 0 ? (long long)o.sssp_bucket_edges
+                                                                            : std::max<long long>(1, g->m / 8);
     a.far = ws.hard;
     a.farbm = ws.visited;
-    const int can_pull = a.fa.off[2] != nullptr && a.fa.nviews == 1 && o.engine != MG_ENGINE_GAS;
-    if (can_pull) {
+    const int in_view = a.fa.off[2] != nullptr && a.fa.nviews == 1 && o.engine != MG_ENGINE_GAS;
+    // pull levels relax every vertex from all its in-edges: unbucketed searches only
+    const int can_pull = in_view && !a.bucketed;
+    if (in_view) {
         if ((rc = ensure_dense_tasks(g, a.fa.off[2]))) return rc;
         a.dpull = ws.dense_tasks;
         a.ndpull = ws.dense_ntasks;
@@ -986,7 +1064,11 @@ static int sssp_run(mg_graph *g, int src, int *dist_out, const mg_options &o, mg
     // step runs at least one round; the bucketed narrow loop runs many rounds per step.
     // Twice that bound leaves room; a search that still runs then is reported, not cut.
     const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, 2 * (int64_t)g->n + 2) + 3;
-    if ((rc = drive(g, step, max_steps, KIND_SSSP, (uint64_t)o.directed * 4 + (uint64_t)(a.dpull != nullptr) + ((uint64_t)a.delta << 3))))
+    uint64_t key = mix64((uint64_t)o.directed * 4 + (uint64_t)(a.dpull != nullptr) + 2 * (uint64_t)a.bucketed);
+    key = mix64(key ^ (uint64_t)a.delta);
+    key = mix64(key ^ (uint64_t)a.bucket_edges);
+    key = mix64(key ^ (uint64_t)a.G);
+    if ((rc = drive(g, step, max_steps, KIND_SSSP, key)))
         return rc;
     rc = finish_search(g, stats, ws.vals, INT_MAX, dist_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
     if (rc == MG_OK && o.verbose) rc = print_trace(g, "sssp", "push", "pull");

@@ -255,3 +255,76 @@ def test_ppr_batch_rows_do_not_depend_on_companions(mg, device):
     for j in range(5):
         exp = orc.ppr(g.row_offsets, g.col_idx, int(srcs[j]), iters=3, teleport=0.3)
         assert np.max(np.abs(x[j] - exp)) <= PPR_TOL
+
+
+# ---------------------------------------------------------------- several graphs in one batched search
+def _min_reached(rows, unreached):
+    """Elementwise minimum over rows, ignoring `unreached` entries (multi-source answer)."""
+    big = np.where(rows == unreached, np.iinfo(np.int64).max, rows.astype(np.int64))
+    m = big.min(0)
+    return np.where(m == np.iinfo(np.int64).max, unreached, m).astype(np.int32)
+
+
+def test_union_graph_runs_several_graphs_in_one_batched_search(mg, oracle, device):
+    """Graphs of uneven size (R-MAT, a grid, a sparse random graph with isolated
+    vertices) joined by mg_graph_union advance in one batched search: row j carries the
+    j-th source of every part, and each part's segment of the row equals the
+    single-graph batched search (and the oracle), BFS and SSSP."""
+    hosts = [mg.rmat_graph(13, 16, seed=7, weight_max=64), mg.grid_graph(96, seed=11, weight_max=16),
+             random_graph(mg, 3000, 2000, 4, wmax=9)]
+    dgs = [mg.Graph(h, symmetric=True) for h in hosts]
+    try:
+        u = mg.Graph.union(dgs)
+        assert u.n == sum(h.n for h in hosts) and u.m == sum(h.m for h in hosts)
+        assert list(u.offsets) == [0, hosts[0].n, hosts[0].n + hosts[1].n, u.n]
+        srcs = [mg.pick_sources(hosts[0], 40, seed=1), mg.pick_sources(hosts[1], 77, seed=2),
+                np.array([0, 1, 2, 5, 7], np.int32)]
+        with u:
+            for kind in ("bfs", "sssp"):
+                parts, st = u.parts_batch(kind, srcs, stats=True)
+                te = 0
+                for i, dg in enumerate(dgs):
+                    sep, sst = (dg.bfs_batch if kind == "bfs" else dg.sssp_batch)(srcs[i], stats=True)
+                    bad = np.argwhere(parts[i] != sep)
+                    assert bad.size == 0, (kind, i, bad[:5].tolist())
+                    te += sst.edges_traversed
+                    for j in (0, len(srcs[i]) - 1):
+                        s = int(srcs[i][j])
+                        exp = (oracle.bfs(hosts[i].row_offsets, hosts[i].col_idx, s) if kind == "bfs" else
+                               oracle.sssp(hosts[i].row_offsets, hosts[i].col_idx, hosts[i].values, s))
+                        assert np.array_equal(parts[i][j], exp), (kind, i, j)
+                assert st.edges_traversed == te
+            # a set of several sources in one part is a multi-source search
+            g0 = hosts[0]
+            two = [int(x) for x in mg.pick_sources(g0, 2, seed=9)]
+            row = u.bfs_batch_sets([0, 0], two, 1)[0][: g0.n]
+            single = np.stack([oracle.bfs(g0.row_offsets, g0.col_idx, s) for s in two])
+            assert np.array_equal(row, _min_reached(single, -1))
+            row = u.sssp_batch_sets([0, 0], two, 1)[0][: g0.n]
+            single = np.stack([oracle.sssp(g0.row_offsets, g0.col_idx, g0.values, s) for s in two])
+            assert np.array_equal(row, _min_reached(single, INT_MAX))
+    finally:
+        for d in dgs:
+            d.close()
+
+
+def test_union_graph_and_sets_argument_checks(mg, device):
+    a = mg.Graph(mg.rmat_graph(8, 4, seed=1), symmetric=True)
+    b = mg.Graph(mg.rmat_graph(8, 4, seed=2, symmetric=False))  # directed, unweighted
+    try:
+        with mg.Graph.union([a, b]) as u:
+            assert not (u.flags & mg.MG_GRAPH_SYMMETRIC) and not (u.flags & mg.MG_GRAPH_WEIGHTED)
+            with pytest.raises(mg.MgError):
+                u.sssp_batch_sets([0], [0], 1)          # no weights
+            for rows, srcs, k in (([0], [0], 2), ([2], [0], 2), ([0], [u.n], 1), ([], [], 1), ([0], [0], 300)):
+                with pytest.raises(mg.MgError) as e:
+                    u.bfs_batch_sets(rows, srcs, k)
+                assert e.value.code == mg.MG_ERR_INVALID_ARG
+            # a directed union pushes only; rows still match the separate searches
+            got = u.parts_batch("bfs", [[0, 3], [1]])
+            assert np.array_equal(got[0], a.bfs_batch([0, 3])) and np.array_equal(got[1], b.bfs_batch([1]))
+        with pytest.raises(mg.MgError):
+            mg.Graph.union([])
+    finally:
+        a.close()
+        b.close()

@@ -430,3 +430,30 @@ def test_allocation_failures_are_status_codes(mg, device):
     root = str(Path(__file__).resolve().parents[1])
     r = subprocess.run([sys.executable, "-c", _ALLOC_FAIL_CHILD, root], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0 and "alloc failures ok" in r.stdout, r.stdout[-2000:] + r.stderr[-4000:]
+
+
+@pytest.mark.parametrize("delta,switch", [(0, 0), (1, 0), (2, -1), (4, 1 << 30), (3, 1000), (-1, 0)])
+def test_sssp_bucketed_whole_gpu_levels(mg, oracle, device, delta, switch):
+    """Whole-GPU SSSP levels relax only pending vertices below the bucket bound and carry
+    the others to the next frontier; the bound is lifted once the pending out-edges fall
+    below sssp_bucket_edges (-1: never; huge: at the first empty bucket).  Every width and
+    switch point gives the oracle's distances, on a symmetric R-MAT (range push on wide
+    levels), a directed R-MAT with its in-edge view (directed = 0 expands both views)
+    and a grid."""
+    o = mg.options(sssp_delta=delta, sssp_bucket_edges=switch)
+    rm = mg.rmat_graph(16, 16, seed=13, weight_max=64)
+    with mg.Graph(rm, symmetric=True) as dg:
+        for src in (0, int(mg.pick_sources(rm, 1, seed=4)[0])):
+            got, st = dg.sssp(src, o, stats=True)
+            exp = oracle.sssp(rm.row_offsets, rm.col_idx, rm.values, src)
+            assert np.array_equal(got, exp), (delta, switch, src, np.nonzero(got != exp)[0][:8])
+            assert st.edges_traversed == oracle.teps_edges(rm.row_offsets, exp, INT_MAX)
+    gd = mg.rmat_graph(14, 16, seed=14, symmetric=False, weight_max=64)
+    with mg.Graph(gd, build_csc=True) as dg:
+        for directed in (1, 0):
+            got = dg.sssp(0, mg.options(sssp_delta=delta, sssp_bucket_edges=switch, directed=directed))
+            exp = oracle.sssp(gd.row_offsets, gd.col_idx, gd.values, 0, directed=directed)
+            assert np.array_equal(got, exp), (delta, switch, directed)
+    grid = mg.grid_graph(96, seed=15, weight_max=16)
+    with mg.Graph(grid, symmetric=True) as dg:
+        assert np.array_equal(dg.sssp(0, o), oracle.sssp(grid.row_offsets, grid.col_idx, grid.values, 0))
