@@ -70,6 +70,12 @@ int mg_device_synchronize(void);
 #define MG_GRAPH_SYMMETRIC  0x4u  /* caller asserts the CSR is symmetric: in-edges of v
                                      equal out-edges of v, with w(u,v) == w(v,u) when
                                      weighted (lets pull steps read the CSR) */
+#define MG_GRAPH_RELABEL    0x8u  /* also keep a copy renamed in out-degree order (hubs
+                                     first, rows sorted); mg_bfs and mg_sssp run on it
+                                     (mg_options.layout) and map the source in and the
+                                     result back, so outputs are unchanged.  Packs the
+                                     distances the hubs' neighbours gather into few cache
+                                     lines; costs a second copy of the topology. */
 
 typedef struct mg_graph mg_graph;
 
@@ -140,7 +146,9 @@ typedef struct mg_options {
                                never lifted                                                */
     int32_t level_grid;     /* mg_bfs / mg_sssp: workgroups of the whole-GPU level kernels
                                (1..65536; 0 = auto from n and m)                           */
-    int32_t reserved[2];
+    int32_t layout;         /* graphs created with MG_GRAPH_RELABEL: 0 = mg_bfs / mg_sssp run
+                               on the degree-ordered copy, 1 = on the caller's numbering   */
+    int32_t reserved[1];
 } mg_options;
 
 /* Fills *o with the reference defaults. */
@@ -371,6 +379,13 @@ int mg_edge_weights_hash(int32_t n, const int32_t *row_offsets, const int32_t *c
 int mg_csc_from_csr(int32_t n, int64_t m, const int32_t *row_offsets,
                     const int32_t *col_idx, int32_t *col_offsets, int32_t *row_idx,
                     int32_t *csr_edge_id);
+/* Degree-order relabelling: new_of_old[v] = rank of v by out-degree (highest first,
+ * ties by id); the renamed CSR lists every row's renamed targets ascending, values
+ * travelling with their edges (vals / vals_out may be NULL).  No reference
+ * counterpart: a layout transform for locality (the hubs' entries pack together). */
+int mg_csr_relabel_degree(int32_t n, int64_t m, const int32_t *row_offsets, const int32_t *col_idx,
+                          const int32_t *vals, int32_t *new_of_old, int32_t *row_offsets_out,
+                          int32_t *col_idx_out, int32_t *vals_out);
 /* k seeded uniform draws (with replacement) among vertices of out-degree > 0. */
 int mg_pick_sources(int32_t n, const int32_t *row_offsets, int32_t k, uint64_t seed,
                     int32_t *out);

@@ -32,6 +32,7 @@ MG_PPR_MAX = 64
 MG_GRAPH_WEIGHTED = 0x1
 MG_GRAPH_BUILD_CSC = 0x2
 MG_GRAPH_SYMMETRIC = 0x4
+MG_GRAPH_RELABEL = 0x8
 
 MG_ENGINE_AUTO = 0
 MG_ENGINE_GAS = 1
@@ -53,7 +54,7 @@ EXPORTED = [
     "mg_rmat_num_edges", "mg_gen_rmat_edges", "mg_grid_num_edges", "mg_gen_grid_edges",
     "mg_csr_from_edges", "mg_edge_weights_hash", "mg_csc_from_csr", "mg_pick_sources",
     "mg_graph_union", "mg_bfs_batch_sets", "mg_sssp_batch_sets",
-    "mg_mtx_read_header", "mg_mtx_read", "mg_autotune",
+    "mg_mtx_read_header", "mg_mtx_read", "mg_autotune", "mg_csr_relabel_degree",
     "mg_server_config_default", "mg_server_create", "mg_server_add_graph", "mg_server_submit", "mg_server_wait",
     "mg_server_now_ms", "mg_server_get_stats", "mg_server_destroy", "mg_host_alloc", "mg_host_free",
 ]
@@ -73,7 +74,7 @@ class Options(C.Structure):
         ("pr_active_eps", C.c_float), ("max_queue_sizing", C.c_double),
         ("alpha", C.c_int32), ("beta", C.c_int32), ("batch_alpha", C.c_int32), ("batch_beta", C.c_int32),
         ("sssp_delta", C.c_int32), ("ppr_iters", C.c_int32), ("sssp_bucket_edges", C.c_int32),
-        ("level_grid", C.c_int32), ("reserved", C.c_int32 * 2),
+        ("level_grid", C.c_int32), ("layout", C.c_int32), ("reserved", C.c_int32 * 1),
     ]
 
 
@@ -163,6 +164,8 @@ def lib() -> C.CDLL:
     L.mg_edge_weights_hash.argtypes = [C.c_int32, I32P, I32P, C.c_int64, C.c_uint64, C.c_int32, I32P]
     L.mg_csc_from_csr.argtypes = [C.c_int32, C.c_int64, I32P, I32P, I32P, I32P, I32P]
     L.mg_pick_sources.argtypes = [C.c_int32, I32P, C.c_int32, C.c_uint64, I32P]
+    if getattr(L, "mg_csr_relabel_degree", None) is not None:
+        L.mg_csr_relabel_degree.argtypes = [C.c_int32, C.c_int64, I32P, I32P, I32P, I32P, I32P, I32P, I32P]
     L.mg_mtx_read_header.argtypes = [C.c_char_p, I32P, C.POINTER(C.c_int64), I32P]
     L.mg_mtx_read.argtypes = [C.c_char_p, I32P, I32P, I32P, C.c_int64, C.POINTER(C.c_int64)]
     L.mg_graph_create.argtypes = [I32P, I32P, I32P, C.c_int32, C.c_int64, C.c_uint32, C.POINTER(C.c_void_p)]
@@ -313,6 +316,18 @@ def hash_weights(g: HostCSR, seed: int, wmax: int) -> np.ndarray:
     return w
 
 
+def relabel_degree(g: HostCSR) -> tuple[HostCSR, np.ndarray]:
+    """The graph renamed in out-degree order (highest first, ties by id) with every row's
+    targets ascending, and new_of_old (new id of each vertex)."""
+    perm = np.empty(g.n, np.int32)
+    ro = np.empty(g.n + 1, np.int32)
+    col = np.empty(max(g.m, 1), np.int32)
+    vals = None if g.values is None else np.empty(max(g.m, 1), np.int32)
+    check(lib().mg_csr_relabel_degree(g.n, g.m, _p(g.row_offsets), _p(g.col_idx), _p(g.values), _p(perm), _p(ro),
+                                      _p(col), _p(vals)))
+    return HostCSR(ro, col[: g.m], None if vals is None else vals[: g.m]), perm
+
+
 def pick_sources(g: HostCSR, k: int, seed: int) -> np.ndarray:
     out = np.empty(k, np.int32)
     check(lib().mg_pick_sources(g.n, _p(g.row_offsets), k, seed, _p(out)))
@@ -366,12 +381,12 @@ class Graph:
     """A graph resident in device memory (mg_graph)."""
 
     def __init__(self, host: HostCSR, weighted: bool | None = None, build_csc: bool = False,
-                 symmetric: bool = False):
+                 symmetric: bool = False, relabel: bool = False):
         L = lib()
         self.host = host
         weighted = host.values is not None if weighted is None else weighted
         flags = (MG_GRAPH_WEIGHTED if weighted else 0) | (MG_GRAPH_BUILD_CSC if build_csc else 0) | \
-                (MG_GRAPH_SYMMETRIC if symmetric else 0)
+                (MG_GRAPH_SYMMETRIC if symmetric else 0) | (MG_GRAPH_RELABEL if relabel else 0)
         h = C.c_void_p()
         check(L.mg_graph_create(_p(host.row_offsets), _p(host.col_idx), _p(host.values) if weighted else None,
                                 host.n, host.m, flags, C.byref(h)))

@@ -537,7 +537,8 @@ extern "C" int mg_bfs_pred(mg_graph *g, int32_t src, int32_t *depth_out, int32_t
         return set_error(MG_ERR_UNSUPPORTED, "mg_bfs_pred: needs the in-edge view (MG_GRAPH_BUILD_CSC or MG_GRAPH_SYMMETRIC)");
     mg_options inner = o;
     inner.out_on_device = 1;
-    inner.engine = MG_ENGINE_AUTO;  // the depths must land in the workspace
+    inner.engine = MG_ENGINE_AUTO;  // the depths must land in this graph's workspace
+    inner.layout = 1;               // (not a degree-ordered twin's)
     if ((rc = mg_bfs(g, src, nullptr, &inner, stats))) return rc;  // depths stay in the workspace
     std::lock_guard<std::mutex> lock(g->mu);
     Workspace &ws = g->ws;
@@ -620,6 +621,8 @@ extern "C" int mg_bfs(mg_graph *g, int32_t src, int32_t *depth_out, const mg_opt
     std::lock_guard<std::mutex> lock(g->mu);
     if (src < 0 || src >= g->n) return set_error(MG_ERR_INVALID_ARG, "mg_bfs: source %d outside [0,%d)", src, g->n);
     if (o.engine == MG_ENGINE_GAS) return gas_bfs(g, src, depth_out, o, stats, timer);
+    g = search_graph(g, o, &src);  // the degree-ordered twin, when the graph keeps one
+    if ((rc = ensure_workspace(g))) return rc;
     if ((rc = size_chunk_queue(g, o))) return rc;
     int regrows = 0;
     while ((rc = bfs_run(g, src, depth_out, o, stats, timer)) == MG_ERR_OVERFLOW && regrows < MAX_REGROWS) {

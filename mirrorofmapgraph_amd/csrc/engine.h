@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "mg_internal.h"
 
@@ -141,9 +142,24 @@ struct mg_graph {
     std::mutex mu;
     mg::Workspace ws;
     struct mg_graph_steps *steps = nullptr;  // captured step graphs
+    // MG_GRAPH_RELABEL: `twin` is the graph renamed in out-degree order (owned; used under
+    // this graph's mutex) and h_perm[v] its id for vertex v.  On the twin, perm is the
+    // device copy of that map and unperm receives a result in the caller's numbering.
+    mg_graph *twin = nullptr;
+    std::vector<int32_t> h_perm;
+    int32_t *perm = nullptr;
+    int32_t *unperm = nullptr;
 };
 
 namespace mg {
+
+// The graph a BFS / SSSP search with options o runs on (the degree-ordered twin when
+// there is one and o.layout allows it) and the source's id there.
+inline mg_graph *search_graph(mg_graph *g, const mg_options &o, int *src) {
+    if (!g->twin || o.layout != 0 || o.engine != MG_ENGINE_AUTO) return g;
+    *src = g->h_perm[(size_t)*src];
+    return g->twin;
+}
 
 // Grows the workspace for graph g; returns an MG_* status.
 int ensure_workspace(mg_graph *g);

@@ -206,6 +206,7 @@ int begin_search(mg_graph *g, const mg_options *opt, mg_options *resolved) {
         return set_error(MG_ERR_INVALID_ARG, "directed must be 0 or 1");
     if (resolved->engine != MG_ENGINE_AUTO && resolved->engine != MG_ENGINE_GAS)
         return set_error(MG_ERR_INVALID_ARG, "unknown engine %d", resolved->engine);
+    if (resolved->layout != 0 && resolved->layout != 1) return set_error(MG_ERR_INVALID_ARG, "layout must be 0 or 1");
     if (resolved->level_grid < 0 || resolved->level_grid > 65536)
         return set_error(MG_ERR_INVALID_ARG, "level_grid must be in [0, 65536]");
     if (resolved->directed == 0 && !(g->flags & (MG_GRAPH_BUILD_CSC | MG_GRAPH_SYMMETRIC)))
@@ -314,7 +315,7 @@ int mg_graph_create(const int32_t *row_offsets, const int32_t *col_idx, const in
                          (long long)m);
     if ((flags & MG_GRAPH_WEIGHTED) && m > 0 && !edge_values)
         return set_error(MG_ERR_INVALID_ARG, "mg_graph_create: MG_GRAPH_WEIGHTED without edge_values");
-    if (flags & ~(MG_GRAPH_WEIGHTED | MG_GRAPH_BUILD_CSC | MG_GRAPH_SYMMETRIC))
+    if (flags & ~(MG_GRAPH_WEIGHTED | MG_GRAPH_BUILD_CSC | MG_GRAPH_SYMMETRIC | MG_GRAPH_RELABEL))
         return set_error(MG_ERR_INVALID_ARG, "mg_graph_create: unknown flags 0x%x", flags);
     if (row_offsets[0] != 0 || row_offsets[n] != m)
         return set_error(MG_ERR_INVALID_ARG, "mg_graph_create: row_offsets must start at 0 and end at m");
@@ -367,6 +368,23 @@ int mg_graph_create(const int32_t *row_offsets, const int32_t *col_idx, const in
             (m && hipMemcpy(g->csc_eid, ce.data(), 4 * (size_t)m, hipMemcpyHostToDevice) != hipSuccess))
             return fail(set_error(MG_ERR_DEVICE, "host-to-device copy of the CSC failed"));
         g->topo_bytes += 4 * ((int64_t)n + 1 + 2 * m);
+    }
+    if (flags & MG_GRAPH_RELABEL) {
+        // the degree-ordered twin: renamed on the host, then created like any graph
+        const bool wt = (flags & MG_GRAPH_WEIGHTED) != 0;
+        std::vector<int32_t> perm((size_t)n), ro2((size_t)n + 1), col2((size_t)m), w2(wt ? (size_t)m : 0);
+        if ((rc = mg_csr_relabel_degree(n, m, row_offsets, col_idx, wt ? edge_values : nullptr, perm.data(),
+                                        ro2.data(), col2.data(), wt ? w2.data() : nullptr)))
+            return fail(rc);
+        mg_graph *t = nullptr;
+        if ((rc = mg_graph_create(ro2.data(), col2.data(), wt ? w2.data() : nullptr, n, m, flags & ~MG_GRAPH_RELABEL,
+                                  &t)))
+            return fail(rc);
+        g->twin = t;
+        if ((rc = dalloc(&t->perm, (size_t)n)) || (rc = dalloc(&t->unperm, (size_t)n))) return fail(rc);
+        if (n && hipMemcpy(t->perm, perm.data(), 4 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_error(MG_ERR_DEVICE, "host-to-device copy of the vertex renaming failed"));
+        g->h_perm = std::move(perm);
     }
     if ((rc = ensure_workspace(g))) return fail(rc);
     *out = g;
@@ -457,6 +475,10 @@ int mg_graph_destroy(mg_graph *g) {
         delete g->steps;
         g->steps = nullptr;
     }
+    if (g->twin) mg_graph_destroy(g->twin);
+    g->twin = nullptr;
+    dfree(g->perm);
+    dfree(g->unperm);
     free_workspace(g->ws);
     dfree(g->row_off);
     dfree(g->col);
@@ -491,6 +513,12 @@ int mg_graph_device_bytes(const mg_graph *g, int64_t *bytes) {
         for (const BatchDev *p : g->steps->bdev)
             if (p) b += batch_dev_size(p);
     if (g->steps && g->steps->pdev) b += ppr_dev_size(g->steps->pdev);
+    if (g->perm) b += 8LL * g->n;
+    if (g->twin) {
+        int64_t tb = 0;
+        mg_graph_device_bytes(g->twin, &tb);
+        b += tb;
+    }
     *bytes = b;
     return MG_OK;
 }

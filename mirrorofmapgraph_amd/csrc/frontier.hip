@@ -198,9 +198,20 @@ int print_trace(mg_graph *g, const char *what, const char *dir0, const char *dir
 // Records the closing event, fills *stats from the control block and copies
 // `bytes` of the device result `result` to `out`.  `vals` (nullable) is the int
 // result the TEPS count reads.
+// out[v] = vals[perm[v]]: a result of the degree-ordered twin in the caller's numbering
+__global__ void unpermute_vals(const int *perm, const int *vals, int n, int *out) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) out[v] = vals[perm[v]];
+}
+
 int finish_search(mg_graph *g, mg_stats *stats, const int *vals, int unreached, void *out, const void *result,
                   size_t bytes, bool out_on_device, const CallTimer &timer) {
     Workspace &ws = g->ws;
+    if (g->perm && result == ws.vals) {  // a search on a twin: its result goes back to the caller's ids
+        hipLaunchKernelGGL(unpermute_vals, dim3(grid_blocks_cap(g->n, BLK, 4 * MAX_GRID)), dim3(BLK), 0, g->stream,
+                           g->perm, ws.vals, g->n, g->unperm);
+        MG_HIP(hipGetLastError());
+        result = g->unperm;
+    }
     MG_HIP(hipEventRecord(g->ev1, g->stream));
     MG_HIP(hipEventSynchronize(g->ev1));
     float ms = 0;

@@ -59,6 +59,18 @@ static void parallel_chunks(int64_t n, int nt, F body) {
     for (auto &x : th) x.join();
 }
 
+// Runs body(t) for t in [0, nt) on nt threads.
+template <class F>
+static void parallel_tasks(int nt, F body) {
+    if (nt <= 1) {
+        body(0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) th.emplace_back([=] { body(t); });
+    for (auto &x : th) x.join();
+}
+
 // Stable LSD radix sort of 64-bit keys on their low `bits` bits (11-bit digits),
 // multi-threaded: per-thread histograms, global bucket-major prefix, scatter.
 static void radix_sort_u64(std::vector<uint64_t> &keys, int bits) {
@@ -311,6 +323,100 @@ int mg_csc_from_csr(int32_t n, int64_t m, const int32_t *row_offsets, const int3
             row_idx[slot] = u;
             if (csr_edge_id) csr_edge_id[slot] = (int32_t)e;
         }
+    return MG_OK;
+}
+
+// Degree order: vertex ids sorted by out-degree, highest first, ties by id (a stable
+// counting sort over the degrees, or a stable comparison sort when degrees exceed 4n).
+// Then two counting-sort transpositions: the first enumerates the renamed rows in
+// order and files each edge under its renamed target (so every target's sources come
+// out ascending), the second turns that back into rows whose targets are ascending.
+int mg_csr_relabel_degree(int32_t n, int64_t m, const int32_t *row_offsets, const int32_t *col_idx,
+                          const int32_t *vals, int32_t *new_of_old, int32_t *row_offsets_out,
+                          int32_t *col_idx_out, int32_t *vals_out) {
+    clear_error();
+    if (n < 0 || m < 0 || !row_offsets || !new_of_old || !row_offsets_out || row_offsets[0] != 0 ||
+        row_offsets[n] != m || (m > 0 && (!col_idx || !col_idx_out)) || (vals && m > 0 && !vals_out))
+        return set_error(MG_ERR_INVALID_ARG, "mg_csr_relabel_degree: bad arguments");
+    std::vector<int32_t> order((size_t)n);
+    int64_t maxdeg = 0;
+    for (int32_t v = 0; v < n; ++v) maxdeg = std::max<int64_t>(maxdeg, row_offsets[v + 1] - row_offsets[v]);
+    if (maxdeg <= 4 * (int64_t)n + 16) {
+        std::vector<int64_t> start((size_t)maxdeg + 2, 0);  // by degree, descending
+        for (int32_t v = 0; v < n; ++v) start[(size_t)(maxdeg - (row_offsets[v + 1] - row_offsets[v])) + 1]++;
+        for (int64_t d = 0; d <= maxdeg; ++d) start[(size_t)d + 1] += start[(size_t)d];
+        for (int32_t v = 0; v < n; ++v) order[(size_t)start[(size_t)(maxdeg - (row_offsets[v + 1] - row_offsets[v]))]++] = v;
+    } else {
+        for (int32_t v = 0; v < n; ++v) order[(size_t)v] = v;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+            return row_offsets[x + 1] - row_offsets[x] > row_offsets[y + 1] - row_offsets[y];
+        });
+    }
+    for (int32_t i = 0; i < n; ++i) new_of_old[order[(size_t)i]] = i;
+    // renamed rows i -> files of renamed targets j (sources ascending within a file).
+    // Thread t owns the files of a contiguous range of targets (balanced by edges) and
+    // scans every edge, so each file is filled in row order without synchronisation.
+    std::vector<int32_t> ncol((size_t)m);
+    parallel_chunks(m, host_threads(), [&](int, int64_t lo, int64_t hi) {
+        for (int64_t e = lo; e < hi; ++e) ncol[(size_t)e] = new_of_old[col_idx[e]];
+    });
+    std::vector<int64_t> tstart((size_t)n + 1, 0);
+    for (int64_t e = 0; e < m; ++e) tstart[(size_t)ncol[(size_t)e] + 1]++;
+    for (int32_t j = 0; j < n; ++j) tstart[(size_t)j + 1] += tstart[(size_t)j];
+    std::vector<int32_t> tsrc((size_t)m), tval(vals ? (size_t)m : 0);
+    const int nt = m < (1 << 20) ? 1 : host_threads();
+    // first id of each thread's range: ranges of about m / nt entries along `starts`
+    auto split = [&](const int64_t *starts) {
+        std::vector<int32_t> cut((size_t)nt + 1, n);
+        cut[0] = 0;
+        for (int t = 1; t < nt; ++t)
+            cut[(size_t)t] = (int32_t)(std::upper_bound(starts, starts + n + 1, m * t / nt) - starts - 1);
+        for (int t = 1; t <= nt; ++t) cut[(size_t)t] = std::max(cut[(size_t)t], cut[(size_t)t - 1]);
+        return cut;
+    };
+    {
+        const std::vector<int32_t> cut = split(tstart.data());
+        parallel_tasks(nt, [&](int t) {
+            {
+                const int32_t jlo = cut[(size_t)t], jhi = cut[(size_t)t + 1];
+                std::vector<int64_t> fill(tstart.begin() + jlo, tstart.begin() + jhi);
+                for (int32_t i = 0; i < n; ++i) {
+                    const int32_t v = order[(size_t)i];
+                    for (int64_t e = row_offsets[v]; e < row_offsets[v + 1]; ++e) {
+                        const int32_t j = ncol[(size_t)e];
+                        if (j < jlo || j >= jhi) continue;
+                        const int64_t slot = fill[(size_t)(j - jlo)]++;
+                        tsrc[(size_t)slot] = i;
+                        if (vals) tval[(size_t)slot] = vals[e];
+                    }
+                }
+            }
+        });
+    }
+    // back to rows: row i holds its targets j ascending (thread t owns a range of rows)
+    row_offsets_out[0] = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t v = order[(size_t)i];
+        row_offsets_out[i + 1] = row_offsets_out[i] + (row_offsets[v + 1] - row_offsets[v]);
+    }
+    {
+        std::vector<int64_t> rstart(row_offsets_out, row_offsets_out + n + 1);
+        const std::vector<int32_t> cut = split(rstart.data());
+        parallel_tasks(nt, [&](int t) {
+            {
+                const int32_t ilo = cut[(size_t)t], ihi = cut[(size_t)t + 1];
+                std::vector<int64_t> fill(rstart.begin() + ilo, rstart.begin() + ihi);
+                for (int32_t j = 0; j < n; ++j)
+                    for (int64_t k = tstart[(size_t)j]; k < tstart[(size_t)j + 1]; ++k) {
+                        const int32_t i = tsrc[(size_t)k];
+                        if (i < ilo || i >= ihi) continue;
+                        const int64_t slot = fill[(size_t)(i - ilo)]++;
+                        col_idx_out[slot] = j;
+                        if (vals) vals_out[slot] = tval[(size_t)k];
+                    }
+            }
+        });
+    }
     return MG_OK;
 }
 

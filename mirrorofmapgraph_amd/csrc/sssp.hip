@@ -1090,6 +1090,8 @@ extern "C" int mg_sssp(mg_graph *g, int32_t src, int32_t *dist_out, const mg_opt
     if (src < 0 || src >= g->n)
         return set_error(MG_ERR_INVALID_ARG, "mg_sssp: source %d outside [0,%d)", src, g->n);
     if (o.engine == MG_ENGINE_GAS) return gas_sssp(g, src, dist_out, o, stats, timer);
+    g = search_graph(g, o, &src);  // the degree-ordered twin, when the graph keeps one
+    if ((rc = ensure_workspace(g))) return rc;
     if ((rc = size_chunk_queue(g, o))) return rc;
     int regrows = 0;
     while ((rc = sssp_run(g, src, dist_out, o, stats, timer)) == MG_ERR_OVERFLOW && regrows < MAX_REGROWS) {

@@ -457,3 +457,58 @@ def test_sssp_bucketed_whole_gpu_levels(mg, oracle, device, delta, switch):
     grid = mg.grid_graph(96, seed=15, weight_max=16)
     with mg.Graph(grid, symmetric=True) as dg:
         assert np.array_equal(dg.sssp(0, o), oracle.sssp(grid.row_offsets, grid.col_idx, grid.values, 0))
+
+
+# ---------------------------------------------------------------- degree-ordered twin
+@pytest.mark.parametrize("which", ["rmat14", "grid", "random_dir", "fixture"])
+def test_relabelled_twin_matches_oracle(mg, oracle, device, which):
+    """A graph created with MG_GRAPH_RELABEL runs BFS / SSSP on its degree-ordered copy
+    (layout 0) and maps sources and results back: bit-exact with the oracle and with the
+    caller-numbered searches (layout 1), TEPS counts equal."""
+    sym = which in ("rmat14", "grid")
+    if which == "rmat14":
+        g = mg.rmat_graph(14, 16, seed=42, weight_max=64)
+    elif which == "grid":
+        g = mg.grid_graph(96, seed=3, weight_max=16)
+    elif which == "random_dir":
+        g = random_graph(mg, 4000, 30000, 11, symmetric=False, wmax=20)
+    else:
+        g = load_mtx(mg, "chesapeake.mtx")
+    srcs = [0, int(np.argmax(g.degrees())), g.n - 1, g.n // 2]
+    with mg.Graph(g, symmetric=sym, build_csc=not sym, relabel=True) as dg:
+        for src in srcs:
+            exp_b = oracle.bfs(g.row_offsets, g.col_idx, src)
+            exp_s = oracle.sssp(g.row_offsets, g.col_idx, g.values, src)
+            for lay in (0, 1):
+                o = mg.options(layout=lay)
+                b, sb = dg.bfs(src, o, stats=True)
+                s, ss = dg.sssp(src, o, stats=True)
+                assert np.array_equal(b, exp_b), (which, src, lay)
+                assert np.array_equal(s, exp_s), (which, src, lay)
+                assert sb.edges_traversed == oracle.teps_edges(g.row_offsets, exp_b)
+                assert ss.edges_traversed == oracle.teps_edges(g.row_offsets, exp_s, INT_MAX)
+        # the calls that keep the caller's numbering are unaffected by the twin
+        assert np.array_equal(dg.cc(), oracle.cc(g.row_offsets, g.col_idx))
+        if sym:
+            d, p = dg.bfs_pred(srcs[1])
+            assert np.array_equal(d, oracle.bfs(g.row_offsets, g.col_idx, srcs[1]))
+
+
+def test_relabelled_twin_device_output_and_bytes(mg, oracle, device):
+    """Results of a twin search written to a device buffer (out_on_device) are in the
+    caller's numbering too; the twin's topology shows in the graph's device bytes."""
+    import ctypes as C
+
+    g = mg.rmat_graph(12, 16, seed=9, weight_max=64)
+    with mg.Graph(g, symmetric=True) as plain, mg.Graph(g, symmetric=True, relabel=True) as dg:
+        assert dg.device_bytes() >= plain.device_bytes() + 4 * (g.n + 1 + 2 * g.m)
+        hip = C.CDLL("libamdhip64.so")
+        p = C.c_void_p()
+        assert hip.hipMalloc(C.byref(p), C.c_size_t(4 * g.n)) == 0
+        try:
+            out = np.empty(g.n, np.int32)
+            mg.check(mg.lib().mg_sssp(dg.handle, 5, p, C.byref(mg.options(out_on_device=1)), None))
+            assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), p, C.c_size_t(4 * g.n), 2) == 0
+        finally:
+            hip.hipFree(p)
+        assert np.array_equal(out, oracle.sssp(g.row_offsets, g.col_idx, g.values, 5))

@@ -37,6 +37,7 @@ def test_exports_every_declared_symbol(mg):
 def test_struct_layouts(mg):
     # mg_options / mg_stats sizes as laid out by the C compiler
     assert C.sizeof(mg.Options) == 4 * 10 + 8 + 4 * 2 + 4 * 8
+    assert mg.Options.layout.offset == mg.Options.level_grid.offset + 4
     assert C.sizeof(mg.Stats) == 8 * 3 + 4 * 2 + 8 * 2 + 4 * 8
     o = mg.options()
     assert o.iter_num == 2**31 - 1 and o.threshold == 10000 and o.directed == 1
@@ -141,6 +142,34 @@ def test_csc_builder(mg):
     assert np.array_equal(ce, order)
     assert np.array_equal(ri, rows[order])
     assert np.array_equal(g.col_idx[ce], np.repeat(np.arange(g.n), np.diff(co)))
+
+
+@pytest.mark.parametrize("kind", ["rmat_sym", "rmat_dir", "multi"])
+def test_degree_relabel(mg, kind):
+    """mg_csr_relabel_degree: new ids rank vertices by out-degree (highest first, ties by
+    id), every edge (u, v, w) appears as (new[u], new[v], w), rows are sorted; checked
+    against a numpy restatement (stable argsort of -degree, lexicographic edge sort)."""
+    if kind == "multi":  # duplicate edges and self-loops survive the renaming
+        rng = np.random.default_rng(3)
+        n = 300
+        s, d = rng.integers(0, n, 4000).astype(np.int32), rng.integers(0, n, 4000).astype(np.int32)
+        g = mg.csr_from_edges(n, s, d, 0, vals=rng.integers(1, 99, 4000).astype(np.int32))
+    else:
+        g = mg.rmat_graph(12, seed=5, symmetric=kind == "rmat_sym", weight_max=64)
+    h, perm = mg.relabel_degree(g)
+    deg = g.degrees().astype(np.int64)
+    order = np.argsort(-deg, kind="stable")
+    new = np.empty(g.n, np.int64)
+    new[order] = np.arange(g.n)
+    assert np.array_equal(perm, new)
+    assert np.all(np.diff(h.degrees()) <= 0)  # degrees non-increasing in new ids
+    assert np.array_equal(h.degrees(), deg[order])
+    src = np.repeat(np.arange(g.n), deg)
+    key = (new[src] << 32) | new[g.col_idx]
+    o = np.argsort(key, kind="stable")
+    assert np.array_equal(h.col_idx, (key[o] & 0xFFFFFFFF).astype(np.int32))
+    assert np.array_equal(h.values, g.values[o])
+    assert np.array_equal(np.repeat(np.arange(g.n), h.degrees()), (key[o] >> 32))
 
 
 def test_mtx_reader(mg, tmp_path):
