@@ -71,11 +71,13 @@ int mg_device_synchronize(void);
                                      equal out-edges of v, with w(u,v) == w(v,u) when
                                      weighted (lets pull steps read the CSR) */
 #define MG_GRAPH_RELABEL    0x8u  /* also keep a copy renamed in out-degree order (hubs
-                                     first, rows sorted); mg_bfs and mg_sssp run on it
-                                     (mg_options.layout) and map the source in and the
-                                     result back, so outputs are unchanged.  Packs the
-                                     distances the hubs' neighbours gather into few cache
-                                     lines; costs a second copy of the topology. */
+                                     first, rows sorted); mg_bfs, mg_sssp and mg_pagerank
+                                     run on it (mg_options.layout) and map the source in
+                                     and the result back: BFS / SSSP outputs are unchanged,
+                                     PageRank sums its in-edges in the copy's order (same
+                                     tolerance).  Packs the values the hubs' neighbours
+                                     gather into few cache lines; costs a second copy of
+                                     the topology. */
 
 typedef struct mg_graph mg_graph;
 
@@ -146,8 +148,9 @@ typedef struct mg_options {
                                never lifted                                                */
     int32_t level_grid;     /* mg_bfs / mg_sssp: workgroups of the whole-GPU level kernels
                                (1..65536; 0 = auto from n and m)                           */
-    int32_t layout;         /* graphs created with MG_GRAPH_RELABEL: 0 = mg_bfs / mg_sssp run
-                               on the degree-ordered copy, 1 = on the caller's numbering   */
+    int32_t layout;         /* graphs created with MG_GRAPH_RELABEL: 0 = mg_bfs / mg_sssp /
+                               mg_pagerank run on the degree-ordered copy, 1 = on the
+                               caller's numbering                                          */
     int32_t reserved[1];
 } mg_options;
 

@@ -655,7 +655,7 @@ int finish_batch(mg_graph *g, BatchDev &b, mg_stats *stats, void *out, size_t by
     MG_HIP(hipEventSynchronize(g->ev1));
     float ms = 0;
     MG_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
-    MG_HIP(hipMemcpy(b.h_ctl, b.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    MG_HIP(copy_on(g->stream, b.h_ctl, b.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
     if (b.h_ctl->changed & (1 << 30))
         return set_error(MG_ERR_OVERFLOW, "batched task queue overflow (capacity %lld)", (long long)b.task_cap);
     if (stats) {

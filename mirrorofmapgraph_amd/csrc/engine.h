@@ -82,6 +82,7 @@ struct Ctl {
     long long thr;            // bucketed SSSP: distances below thr are in the current bucket
     long long far_hi;         // bucketed SSSP: highest bucket holding a waiting distance
     int pair_end;             // batched SSSP: pair slots below this are valid this iteration
+    long long fmin;           // SSSP: a lower bound on the distances of the current frontier
 };
 
 // Pooled per-search device state, grown on demand and reused across searches.
@@ -153,11 +154,21 @@ struct mg_graph {
 
 namespace mg {
 
-// The graph a BFS / SSSP search with options o runs on (the degree-ordered twin when
-// there is one and o.layout allows it) and the source's id there.
+// A copy ordered on stream st that returns when it is done.  Library code never uses
+// the blocking null-stream hipMemcpy: that waits on every blocking stream of the device
+// and fails while another host thread captures a step graph.
+inline hipError_t copy_on(hipStream_t st, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    if (!bytes) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+    return e == hipSuccess ? hipStreamSynchronize(st) : e;
+}
+
+// The graph a BFS / SSSP / PageRank search with options o runs on (the degree-ordered
+// twin when there is one and o.layout allows it) and the source's id there.  Results
+// of a search on a twin are mapped back to the caller's numbering by finish_search.
 inline mg_graph *search_graph(mg_graph *g, const mg_options &o, int *src) {
     if (!g->twin || o.layout != 0 || o.engine != MG_ENGINE_AUTO) return g;
-    *src = g->h_perm[(size_t)*src];
+    if (src) *src = g->h_perm[(size_t)*src];
     return g->twin;
 }
 

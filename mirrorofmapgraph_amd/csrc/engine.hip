@@ -338,12 +338,12 @@ int mg_graph_create(const int32_t *row_offsets, const int32_t *col_idx, const in
     };
     if ((rc = dalloc(&g->row_off, (size_t)n + 1)) || (rc = dalloc(&g->col, (size_t)m))) return fail(rc);
     g->topo_bytes = 4 * ((int64_t)n + 1 + m);
-    if (hipMemcpy(g->row_off, row_offsets, 4 * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
-        (m && hipMemcpy(g->col, col_idx, 4 * (size_t)m, hipMemcpyHostToDevice) != hipSuccess))
+    if (copy_on(g->stream, g->row_off, row_offsets, 4 * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        copy_on(g->stream, g->col, col_idx, 4 * (size_t)m, hipMemcpyHostToDevice) != hipSuccess)
         return fail(set_error(MG_ERR_DEVICE, "host-to-device copy of the CSR failed"));
     if (flags & MG_GRAPH_WEIGHTED) {
         if ((rc = dalloc(&g->w, (size_t)m))) return fail(rc);
-        if (m && hipMemcpy(g->w, edge_values, 4 * (size_t)m, hipMemcpyHostToDevice) != hipSuccess)
+        if (copy_on(g->stream, g->w, edge_values, 4 * (size_t)m, hipMemcpyHostToDevice) != hipSuccess)
             return fail(set_error(MG_ERR_DEVICE, "host-to-device copy of the edge values failed"));
         int32_t mn = INT32_MAX, mx = INT32_MIN;
         double sum = 0;
@@ -363,9 +363,9 @@ int mg_graph_create(const int32_t *row_offsets, const int32_t *col_idx, const in
         if ((rc = dalloc(&g->col_off, (size_t)n + 1)) || (rc = dalloc(&g->row_idx, (size_t)m)) ||
             (rc = dalloc(&g->csc_eid, (size_t)m)))
             return fail(rc);
-        if (hipMemcpy(g->col_off, co.data(), 4 * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
-            (m && hipMemcpy(g->row_idx, ri.data(), 4 * (size_t)m, hipMemcpyHostToDevice) != hipSuccess) ||
-            (m && hipMemcpy(g->csc_eid, ce.data(), 4 * (size_t)m, hipMemcpyHostToDevice) != hipSuccess))
+        if (copy_on(g->stream, g->col_off, co.data(), 4 * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+            copy_on(g->stream, g->row_idx, ri.data(), 4 * (size_t)m, hipMemcpyHostToDevice) != hipSuccess ||
+            copy_on(g->stream, g->csc_eid, ce.data(), 4 * (size_t)m, hipMemcpyHostToDevice) != hipSuccess)
             return fail(set_error(MG_ERR_DEVICE, "host-to-device copy of the CSC failed"));
         g->topo_bytes += 4 * ((int64_t)n + 1 + 2 * m);
     }
@@ -382,7 +382,7 @@ int mg_graph_create(const int32_t *row_offsets, const int32_t *col_idx, const in
             return fail(rc);
         g->twin = t;
         if ((rc = dalloc(&t->perm, (size_t)n)) || (rc = dalloc(&t->unperm, (size_t)n))) return fail(rc);
-        if (n && hipMemcpy(t->perm, perm.data(), 4 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
+        if (copy_on(t->stream, t->perm, perm.data(), 4 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
             return fail(set_error(MG_ERR_DEVICE, "host-to-device copy of the vertex renaming failed"));
         g->h_perm = std::move(perm);
     }
@@ -528,9 +528,9 @@ int mg_graph_get_csc(const mg_graph *g, int32_t *col_offsets, int32_t *row_idx, 
     if (!g) return set_error(MG_ERR_INVALID_ARG, "null graph");
     if (!g->col_off) return set_error(MG_ERR_UNSUPPORTED, "graph has no CSC view (MG_GRAPH_BUILD_CSC)");
     MG_HIP(hipSetDevice(g->device));
-    if (col_offsets) MG_HIP(hipMemcpy(col_offsets, g->col_off, 4 * ((size_t)g->n + 1), hipMemcpyDeviceToHost));
-    if (row_idx && g->m) MG_HIP(hipMemcpy(row_idx, g->row_idx, 4 * (size_t)g->m, hipMemcpyDeviceToHost));
-    if (csr_edge_id && g->m) MG_HIP(hipMemcpy(csr_edge_id, g->csc_eid, 4 * (size_t)g->m, hipMemcpyDeviceToHost));
+    if (col_offsets) MG_HIP(copy_on(g->stream, col_offsets, g->col_off, 4 * ((size_t)g->n + 1), hipMemcpyDeviceToHost));
+    if (row_idx) MG_HIP(copy_on(g->stream, row_idx, g->row_idx, 4 * (size_t)g->m, hipMemcpyDeviceToHost));
+    if (csr_edge_id) MG_HIP(copy_on(g->stream, csr_edge_id, g->csc_eid, 4 * (size_t)g->m, hipMemcpyDeviceToHost));
     return MG_OK;
 }
 

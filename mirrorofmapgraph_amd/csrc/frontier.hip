@@ -81,7 +81,7 @@ static int build_pull_tasks(mg_graph *g, const int *d_in_off, int longer, ChunkR
     if (*view == d_in_off) return MG_OK;
     const int n = g->n;
     std::vector<int32_t> off((size_t)n + 1);
-    MG_HIP(hipMemcpy(off.data(), d_in_off, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
+    MG_HIP(copy_on(g->stream, off.data(), d_in_off, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
     std::vector<ChunkRec> t;
     for (int v = 0; v < n; ++v) {
         if (off[v + 1] - off[v] <= longer) continue;
@@ -93,7 +93,7 @@ static int build_pull_tasks(mg_graph *g, const int *d_in_off, int longer, ChunkR
     *view = nullptr;
     int rc = alloc_chunks(tasks, t.size());
     if (rc) return rc;
-    if (!t.empty()) MG_HIP(hipMemcpy(*tasks, t.data(), sizeof(ChunkRec) * t.size(), hipMemcpyHostToDevice));
+    if (!t.empty()) MG_HIP(copy_on(g->stream, *tasks, t.data(), sizeof(ChunkRec) * t.size(), hipMemcpyHostToDevice));
     *ntasks = (int)t.size();
     *view = d_in_off;
     return MG_OK;
@@ -109,7 +109,7 @@ int ensure_dense_tasks(mg_graph *g, const int *d_in_off) {
     if (ws.dense_view == d_in_off) return MG_OK;
     const int n = g->n;
     std::vector<int32_t> off((size_t)n + 1);
-    MG_HIP(hipMemcpy(off.data(), d_in_off, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
+    MG_HIP(copy_on(g->stream, off.data(), d_in_off, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
     std::vector<ChunkRec> t;
     // pieces of the long lists
     for (int v = 0; v < n; ++v)
@@ -139,7 +139,7 @@ int ensure_dense_tasks(mg_graph *g, const int *d_in_off) {
     ws.dense_view = nullptr;
     int rc = alloc_chunks(&ws.dense_tasks, t.size());
     if (rc) return rc;
-    if (!t.empty()) MG_HIP(hipMemcpy(ws.dense_tasks, t.data(), sizeof(ChunkRec) * t.size(), hipMemcpyHostToDevice));
+    if (!t.empty()) MG_HIP(copy_on(g->stream, ws.dense_tasks, t.data(), sizeof(ChunkRec) * t.size(), hipMemcpyHostToDevice));
     ws.dense_ntasks = (int)t.size();
     ws.dense_view = d_in_off;
     return MG_OK;
@@ -180,11 +180,11 @@ int launch_reached_stats(mg_graph *g, const int *vals, int unreached, mg_stats *
 
 int print_trace(mg_graph *g, const char *what, const char *dir0, const char *dir1) {
     Workspace &ws = g->ws;
-    MG_HIP(hipMemcpy(ws.h_ctl, ws.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    MG_HIP(copy_on(g->stream, ws.h_ctl, ws.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
     const int levels = ws.h_ctl->level;
     const int nl = std::min(levels, TRACE_CAP);
     std::vector<long long> t(2 * (size_t)std::max(nl, 1));
-    if (nl) MG_HIP(hipMemcpy(t.data(), ws.trace, 16 * (size_t)nl, hipMemcpyDeviceToHost));
+    if (nl) MG_HIP(copy_on(g->stream, t.data(), ws.trace, 16 * (size_t)nl, hipMemcpyDeviceToHost));
     for (int L = 0; L < nl; ++L) {
         const long long nf = t[2 * L] & ((1LL << 40) - 1);
         const int dir = (int)(t[2 * L] >> 40);
@@ -206,9 +206,10 @@ __global__ void unpermute_vals(const int *perm, const int *vals, int n, int *out
 int finish_search(mg_graph *g, mg_stats *stats, const int *vals, int unreached, void *out, const void *result,
                   size_t bytes, bool out_on_device, const CallTimer &timer) {
     Workspace &ws = g->ws;
-    if (g->perm && result == ws.vals) {  // a search on a twin: its result goes back to the caller's ids
+    if (g->perm && result && bytes == 4 * (size_t)g->n) {
+        // a search on a twin: its per-vertex result (int32 or float) goes back to the caller's ids
         hipLaunchKernelGGL(unpermute_vals, dim3(grid_blocks_cap(g->n, BLK, 4 * MAX_GRID)), dim3(BLK), 0, g->stream,
-                           g->perm, ws.vals, g->n, g->unperm);
+                           g->perm, (const int *)result, g->n, g->unperm);
         MG_HIP(hipGetLastError());
         result = g->unperm;
     }
@@ -216,7 +217,7 @@ int finish_search(mg_graph *g, mg_stats *stats, const int *vals, int unreached, 
     MG_HIP(hipEventSynchronize(g->ev1));
     float ms = 0;
     MG_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
-    MG_HIP(hipMemcpy(ws.h_ctl, ws.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    MG_HIP(copy_on(g->stream, ws.h_ctl, ws.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
     if (ws.h_ctl->changed & (1 << 30))
         return set_error(MG_ERR_OVERFLOW, "edge-chunk queue overflow (capacity %lld)", (long long)ws.chunk_cap);
     if (stats) {

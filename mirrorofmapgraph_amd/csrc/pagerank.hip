@@ -291,7 +291,7 @@ static int pr_buckets(mg_graph *g, const int *d_in_off) {
     if (ws.pr_view == d_in_off) return MG_OK;
     const int n = g->n;
     std::vector<int32_t> off((size_t)n + 1);
-    MG_HIP(hipMemcpy(off.data(), d_in_off, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
+    MG_HIP(copy_on(g->stream, off.data(), d_in_off, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
     std::vector<int32_t> mid, bigv;
     std::vector<ChunkRec> ch;
     for (int v = 0; v < n; ++v) {
@@ -321,9 +321,9 @@ static int pr_buckets(mg_graph *g, const int *d_in_off) {
         (rc = alloc_i32(&ws.pr_bigv, bigv.size())) || (rc = alloc_f32(&ws.pr_part, ch.size())) ||
         (rc = alloc_i32(&ws.pr_pact, ch.size())))
         return rc;
-    if (!mid.empty()) MG_HIP(hipMemcpy(ws.pr_mid, mid.data(), 4 * mid.size(), hipMemcpyHostToDevice));
-    if (!ch.empty()) MG_HIP(hipMemcpy(ws.pr_chunks, ch.data(), sizeof(ChunkRec) * ch.size(), hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(ws.pr_bigv, bigv.data(), 4 * bigv.size(), hipMemcpyHostToDevice));
+    if (!mid.empty()) MG_HIP(copy_on(g->stream, ws.pr_mid, mid.data(), 4 * mid.size(), hipMemcpyHostToDevice));
+    if (!ch.empty()) MG_HIP(copy_on(g->stream, ws.pr_chunks, ch.data(), sizeof(ChunkRec) * ch.size(), hipMemcpyHostToDevice));
+    MG_HIP(copy_on(g->stream, ws.pr_bigv, bigv.data(), 4 * bigv.size(), hipMemcpyHostToDevice));
     ws.pr_nmid = (int)mid.size();
     ws.pr_nchunks = (int)ch.size();
     ws.pr_nbig = (int)bigv.size() - 1;
@@ -354,7 +354,8 @@ extern "C" int mg_pagerank(mg_graph *g, float *rank_out, const mg_options *opt, 
                                                  "MG_PR_GAS, teleport 0.15, eps 0.01)");
         return gas_pagerank(g, rank_out, o, stats, timer);
     }
-    if ((rc = ensure_pr_workspace(g))) return rc;
+    g = search_graph(g, o, nullptr);  // the degree-ordered twin, when the graph keeps one
+    if ((rc = ensure_workspace(g)) || (rc = ensure_pr_workspace(g))) return rc;
     const int *in_off = sym ? g->row_off : g->col_off;
     const int *in_adj = sym ? g->col : g->row_idx;
     if ((rc = pr_buckets(g, in_off))) return rc;

@@ -43,6 +43,7 @@ struct SsspArgs {
     // bucket_edges the bound is lifted (UNBUCKETED) and the tail runs as Bellman-Ford
     int bucketed;
     long long bucket_edges;
+    int wmin;  // smallest edge weight (>= 0)
 };
 
 constexpr long long UNBUCKETED = LLONG_MAX / 4;  // a bucket bound no distance reaches
@@ -154,17 +155,23 @@ struct SsspPush {
     }
 };
 
-// Pull: the best dist[u] + w over all in-edge slots [beg, end), PULL_U loads in
-// flight (neighbour ids, then their distances and weights).  No frontier test: a
-// vertex u outside the frontier has relaxed its out-edges with its present distance
-// already (its last change put it in a frontier), so its candidate dist[u] + w cannot
-// beat dist[v] and the minimum is unchanged; unreached u give INT_MAX + w.  That
-// trades a bitmap gather per edge for a distance gather per non-frontier edge, a gain
-// since pull levels run only when the frontier holds most edges.
+// Pull: the best dist[u] + w over the in-edge slots [beg, end) of vertex v, PULL_U
+// loads in flight (neighbour ids and weights, then the distances they need).  No
+// frontier test: a vertex u outside the frontier has relaxed its out-edges with its
+// present distance already (its last change put it in a frontier), so its candidate
+// dist[u] + w cannot beat dist[v]; unreached u give INT_MAX + w.
+//
+// Bound: every frontier vertex has dist >= fmin (the level's lower bound), so an edge
+// of weight w can only improve v when fmin + w < dist[v]; the distance gather of any
+// other edge is skipped (its lane reads dist[hot] instead, one cached line shared by
+// the wave, and contributes nothing).  On R-MAT from a hub most distances are small
+// after the first wide level, so the dense levels gather for a few percent of their
+// edges: the pull streams neighbour ids and weights and gathers little.
 constexpr int PULL_U = 8;
 template <bool EID>
-__device__ __forceinline__ long long pull_best(const SsspArgs &a, int beg, int end, int step) {
-    long long best = INT_MAX;
+__device__ __forceinline__ long long pull_best(const SsspArgs &a, int beg, int end, int step, long long fmin,
+                                               long long dv, int hot) {
+    long long best = dv;  // candidates at or above the current distance change nothing
     const int *adj = a.fa.adj[2];
     const int *eid = a.fa.eid[2];
     for (int e = beg; e < end; e += PULL_U * step) {
@@ -187,12 +194,15 @@ __device__ __forceinline__ long long pull_best(const SsspArgs &a, int beg, int e
 #pragma unroll
             for (int k = 0; k < PULL_U; ++k) wt[k] = a.w[sl[k]];
         }
+        bool need[PULL_U];
         int du[PULL_U];
 #pragma unroll
-        for (int k = 0; k < PULL_U; ++k) du[k] = a.dist[u[k]];
+        for (int k = 0; k < PULL_U; ++k) need[k] = ok[k] && fmin + wt[k] < best;
+#pragma unroll
+        for (int k = 0; k < PULL_U; ++k) du[k] = a.dist[need[k] ? u[k] : hot];
 #pragma unroll
         for (int k = 0; k < PULL_U; ++k) {
-            const long long cand = ok[k] ? (long long)du[k] + wt[k] : (long long)INT_MAX;
+            const long long cand = need[k] ? (long long)du[k] + wt[k] : LLONG_MAX;
             best = cand < best ? cand : best;
         }
     }
@@ -200,7 +210,8 @@ __device__ __forceinline__ long long pull_best(const SsspArgs &a, int beg, int e
 }
 
 // Dense pull level: every vertex takes min(dist[u] + w) over all its in-edges (no
-// frontier test, see pull_best), in one launch over the static dense tasks:
+// frontier test, see pull_best; the same fmin bound skips the distance gathers of
+// edges that cannot improve), in one launch over the static dense tasks:
 //   * a piece of a long list: a wave minimum, one atomicMin into dist[v];
 //   * a range of up to DENSE_V vertices of whole bitmap words with short lists: the
 //     wave walks the concatenation of their in-lists in windows of 64 slots, one per
@@ -209,9 +220,11 @@ __device__ __forceinline__ long long pull_best(const SsspArgs &a, int beg, int e
 //     distances the only gathers, DENSE_WIN windows in flight together.  Owners are
 //     non-decreasing along a window, so a segmented minimum scan across the wave
 //     reduces each owner's run and its last lane folds it into the owner's LDS
-//     minimum; the range's own vertices are then updated with plain stores (no other
-//     task writes them) and their frontier words set with one atomicOr each (long-list
-//     vertices of the same words are marked by their pieces).
+//     minimum (which starts at the owner's distance and bounds its gathers); the
+//     range's own vertices are then updated with plain stores (no other task writes
+//     them) and their frontier words set with one atomicOr each (long-list vertices
+//     of the same words are marked by their pieces).
+// mn takes the smallest improved distance (the next level's fmin).
 constexpr int DENSE_WIN = 4;
 constexpr int DENSE_PER = DENSE_V / WAVE;  // vertices per lane in a range task
 struct DenseLds {
@@ -220,8 +233,8 @@ struct DenseLds {
     int min[BLK / WAVE][DENSE_V];
 };
 template <bool EID>
-__device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, int vb, DenseLds &lds, long long &nf,
-                                            long long &mf, long long &ex) {
+__device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, int vb, long long fmin, DenseLds &lds,
+                                            long long &nf, long long &mf, long long &ex, long long &mn) {
     const FrontierArgs &fa = a.fa;
     uint32_t *nx = fa.bm[(L + 1) % 3];
     const int *adj = fa.adj[2];
@@ -231,15 +244,17 @@ __device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, in
     int *pre = lds.pre[wv], *begs = lds.beg[wv], *mins = lds.min[wv];
     const uint64_t upto = lane == WAVE - 1 ? ~0ull : ((2ull << lane) - 1);  // lanes 0..lane
     // vertex va + 64 j + lane is list 64 j + lane
-    int beg[DENSE_PER], len[DENSE_PER];
+    int beg[DENSE_PER], len[DENSE_PER], cur[DENSE_PER];
 #pragma unroll
     for (int j = 0; j < DENSE_PER; ++j) {
         const int v = va + j * WAVE + lane;
         const bool in = v < vb;
         const int vv = in ? v : va;
         const int o0 = off[vv], o1 = off[vv + 1];
+        cur[j] = a.dist[vv];
         beg[j] = o0;
-        len[j] = in && o1 - o0 <= DENSE_BIG ? o1 - o0 : 0;
+        // a vertex at or below fmin + (smallest weight) cannot improve: its list is skipped
+        len[j] = in && o1 - o0 <= DENSE_BIG && fmin + a.wmin < cur[j] ? o1 - o0 : 0;
     }
     int total = 0;
 #pragma unroll
@@ -252,14 +267,14 @@ __device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, in
         }
         pre[j * WAVE + lane + 1] = total + incl;
         begs[j * WAVE + lane] = beg[j];
-        mins[j * WAVE + lane] = INT_MAX;
+        mins[j * WAVE + lane] = cur[j];
         total += __shfl(incl, WAVE - 1);
     }
     if (lane == 0) pre[0] = 0;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    ex += total;
+    if (lane == 0) ex += total;  // the wave's edges, counted once
     // an in-bounds slot for the lanes past the end: off[va] (the windows run only when
     // some list of the range is non-empty, and it starts at or after off[va])
     const int safe = begs[0];
@@ -282,11 +297,12 @@ __device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, in
             for (int k = 0; k < DENSE_WIN; ++k)
                 if (q[k] <= p[k]) own[k] += st;
         }
-        int first[DENSE_WIN], nextp[DENSE_WIN];
+        int first[DENSE_WIN], nextp[DENSE_WIN], bound[DENSE_WIN];
 #pragma unroll
         for (int k = 0; k < DENSE_WIN; ++k) {
             first[k] = pre[own[k]];
             nextp[k] = pre[own[k] + 1];
+            bound[k] = mins[own[k]];
             slot[k] = ok[k] ? begs[own[k]] + (p[k] - first[k]) : safe;
         }
         int u[DENSE_WIN], wt[DENSE_WIN];
@@ -302,40 +318,40 @@ __device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, in
 #pragma unroll
             for (int k = 0; k < DENSE_WIN; ++k) wt[k] = a.w[slot[k]];
         }
+        bool need[DENSE_WIN];
         int du[DENSE_WIN];
 #pragma unroll
-        for (int k = 0; k < DENSE_WIN; ++k) du[k] = a.dist[u[k]];
+        for (int k = 0; k < DENSE_WIN; ++k) need[k] = ok[k] && fmin + wt[k] < (long long)bound[k];
+#pragma unroll
+        for (int k = 0; k < DENSE_WIN; ++k) du[k] = a.dist[need[k] ? u[k] : va];
 #pragma unroll
         for (int k = 0; k < DENSE_WIN; ++k) {
             const long long d = (long long)du[k] + wt[k];
-            int best = ok[k] && d < INT_MAX ? (int)d : INT_MAX;
+            int best = need[k] && d < INT_MAX ? (int)d : INT_MAX;
             // runs: one per owner; the lanes past the end form one more run
             const bool head = lane == 0 || (ok[k] ? p[k] == first[k] : p[k] == total);
             const uint64_t heads = wave_ballot(head);
             const int start = 63 - __builtin_clzll(heads & upto);
             best = seg_min_scan(best, lane, start);
             const bool tail = lane == WAVE - 1 || p[k] + 1 == nextp[k];
-            if (ok[k] && tail && best < INT_MAX) atomicMin(&mins[own[k]], best);
+            if (ok[k] && tail && best < bound[k]) atomicMin(&mins[own[k]], best);
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // write-back: the range's distances, then the degrees of the improved vertices,
+    // write-back: the improved distances, then the degrees of the improved vertices,
     // each stage's loads in flight together (clamped to va where not needed)
-    int cur[DENSE_PER];
     bool found[DENSE_PER];
-#pragma unroll
-    for (int j = 0; j < DENSE_PER; ++j) {
-        const int v = va + j * WAVE + lane;
-        cur[j] = a.dist[v < vb ? v : va];
-    }
 #pragma unroll
     for (int j = 0; j < DENSE_PER; ++j) {
         const int v = va + j * WAVE + lane;
         const int best = mins[j * WAVE + lane];
         found[j] = len[j] > 0 && best < cur[j];
-        if (found[j]) a.dist[v] = best;
+        if (found[j]) {
+            a.dist[v] = best;
+            mn = best < mn ? best : mn;
+        }
     }
     int o0[DENSE_PER], o1[DENSE_PER], p0[DENSE_PER], p1[DENSE_PER];
 #pragma unroll
@@ -366,24 +382,27 @@ __device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, in
 
 template <bool EID>
 __device__ __forceinline__ void dense_pull(const SsspArgs &a, int L, DenseLds &lds, long long &nf, long long &mf,
-                                           long long &ex) {
+                                           long long &ex, long long &mn) {
     const int lane = lane_id();
     const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
+    const long long fmin = a.fa.ctl->fmin;
     for (int t = wave; t < a.ndpull; t += nwaves) {
         const ChunkRec r = a.dpull[t];
         if (r.v < 0) {
-            dense_range<EID>(a, L, r.beg, r.end, lds, nf, mf, ex);
+            dense_range<EID>(a, L, r.beg, r.end, fmin, lds, nf, mf, ex, mn);
             continue;
         }
         ex += (r.end - r.beg + WAVE - 1 - lane) / WAVE;
-        long long best = pull_best<EID>(a, r.beg + lane, r.end, WAVE);
+        const long long dv = a.dist[r.v];
+        long long best = pull_best<EID>(a, r.beg + lane, r.end, WAVE, fmin, dv, r.v);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             const long long y = __shfl_xor(best, o);
             best = y < best ? y : best;
         }
-        if (lane == 0 && best < a.dist[r.v]) {
+        if (lane == 0 && best < dv && best < a.dist[r.v]) {
             const int old = atomicMin(&a.dist[r.v], (int)best);
+            mn = best < mn ? best : mn;
             if (best < old && set_bit_first(a.fa.bm[(L + 1) % 3], r.v)) {
                 ++nf;
                 mf += view_degree(a.fa, r.v);
@@ -540,6 +559,7 @@ __device__ void sssp_close(const SsspArgs &a, int L, long long nf, long long mf,
     Ctl *c = a.fa.ctl;
     int dir = c->dir;
     trace_level(a.fa, L, dir, nf, mf);
+    c->fmin = mn;  // every vertex of the next frontier improved (or waits) at mn or above
     if (a.bucketed && c->thr < UNBUCKETED) {
         if (mf < a.bucket_edges) c->thr = UNBUCKETED;
         else if (mn >= c->thr) c->thr = (mn / a.delta + 1) * a.delta;
@@ -574,6 +594,7 @@ __global__ void sssp_begin(SsspArgs a, int src, int iter_limit, int can_pull, lo
     a.fa.q[0][0] = src;
     ctl_reset(c, a.fa.n, iter_limit, 1);
     c->thr = a.delta;  // the first bucket: [0, delta)
+    c->fmin = 0;
     c->mf_cur = view_degree(a.fa, src);
     // pull only pays on the densest levels: every in-edge is read and the frontier
     // ones gather dist[u]; push gathers only along frontier out-edges
@@ -989,9 +1010,9 @@ __global__ __launch_bounds__(BLK) void sssp_level_b(SsspArgs a) {
         ex = pol.ex;
         mn = pol.mn;
     } else if (fa.eid[2]) {
-        dense_pull<true>(a, L, lds, nf, mf, ex);
+        dense_pull<true>(a, L, lds, nf, mf, ex, mn);
     } else {
-        dense_pull<false>(a, L, lds, nf, mf, ex);
+        dense_pull<false>(a, L, lds, nf, mf, ex, mn);
     }
     write_part<BLK>(fa, nf, mf, ex, mn);
     if (blockIdx.x == 0 && threadIdx.x == 0) c->pending = 1;
@@ -1037,6 +1058,7 @@ static int sssp_run(mg_graph *g, int src, int *dist_out, const mg_options &o, mg
                                                                             : std::max<long long>(1, g->m / 8);
     a.far = ws.hard;
     a.farbm = ws.visited;
+    a.wmin = g->min_weight;
     const int in_view = a.fa.off[2] != nullptr && a.fa.nviews == 1 && o.engine != MG_ENGINE_GAS;
     // pull levels relax every vertex from all its in-edges: unbucketed searches only
     const int can_pull = in_view && !a.bucketed;

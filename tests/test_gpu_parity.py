@@ -139,6 +139,23 @@ def test_sssp_bucket_widths(mg, oracle, device, delta):
         assert st.edges_traversed == oracle.teps_edges(g.row_offsets, exp, INT_MAX)
 
 
+@pytest.mark.parametrize("wmax", [1, 2, 3])
+@pytest.mark.parametrize("relabel", [False, True])
+def test_sssp_zero_and_tiny_weights(mg, oracle, device, wmax, relabel):
+    """Weights drawn from [0, wmax) (zero-weight edges included) on R-MAT scale 15, where
+    the dense pull levels run: the distance bound of the pull (fmin + w < dist) and the
+    skip of vertices at or below fmin + (smallest weight) must keep every improvement."""
+    g = mg.rmat_graph(15, 16, seed=21, weight_max=wmax)
+    g.values = (g.values - 1).astype(np.int32) if wmax > 1 else np.zeros(g.m, np.int32)
+    # keep the weights symmetric: hashed weights already are, the shift keeps that
+    for src in (0, int(mg.pick_sources(g, 1, 5)[0])):
+        with mg.Graph(g, symmetric=True, relabel=relabel) as dg:
+            got, st = dg.sssp(src, stats=True)
+        exp = oracle.sssp(g.row_offsets, g.col_idx, g.values, src)
+        assert np.array_equal(got, exp), (wmax, src, np.nonzero(got != exp)[0][:10])
+        assert st.edges_traversed == oracle.teps_edges(g.row_offsets, exp, INT_MAX)
+
+
 # ---------------------------------------------------------------- limits and errors
 def test_iter_num_limits_levels(mg, oracle, device, s16):
     with mg.Graph(s16, symmetric=True) as dg:
@@ -512,3 +529,67 @@ def test_relabelled_twin_device_output_and_bytes(mg, oracle, device):
         finally:
             hip.hipFree(p)
         assert np.array_equal(out, oracle.sssp(g.row_offsets, g.col_idx, g.values, 5))
+
+
+@pytest.mark.parametrize("mode", ["all", "gas"])
+def test_pagerank_on_relabelled_twin(mg, oracle, device, mode):
+    """PageRank on the degree-ordered copy: ranks mapped back to the caller's numbering,
+    within the FP32 tolerance of the oracle (in-edges are summed in the copy's order)."""
+    g = mg.rmat_graph(14, 16, seed=42, symmetric=False)
+    with mg.Graph(g, build_csc=True, relabel=True) as dg:
+        if mode == "all":
+            got, st = dg.pagerank(mg.options(pr_mode=mg.MG_PR_ALL, iter_num=50), stats=True)
+            assert st.iterations == 50
+            exp, _, _ = oracle.pr_jacobi_f32(g.row_offsets, g.col_idx, iters=50)
+        else:
+            got, st = dg.pagerank(stats=True)
+            exp, it = oracle.pr_gas(g.row_offsets, g.col_idx)
+            assert abs(st.iterations - it) <= 1
+        same = dg.pagerank(mg.options(pr_mode=mg.MG_PR_ALL, iter_num=50, layout=1)) if mode == "all" else None
+    assert _pr_err(got, exp, g.n) <= PR_TOL
+    if same is not None:
+        assert _pr_err(same, exp, g.n) <= PR_TOL
+
+
+# ---------------------------------------------------------------- host threads
+def test_concurrent_host_threads(mg, oracle, device):
+    """Searches from several host threads at once: each thread on its own graph, and all
+    of them on one shared graph (calls on a handle are serialised by its mutex; each
+    graph has its own stream).  Every result equals the oracle's."""
+    import threading
+
+    a = mg.rmat_graph(13, 16, seed=31, weight_max=64)
+    b = mg.grid_graph(64, seed=32, weight_max=16)
+    shared = mg.rmat_graph(12, 16, seed=33, weight_max=64)
+    exp = {}
+    for name, g in (("a", a), ("b", b), ("shared", shared)):
+        for s in (0, 7, 100):
+            exp[name, "bfs", s] = oracle.bfs(g.row_offsets, g.col_idx, s)
+            exp[name, "sssp", s] = oracle.sssp(g.row_offsets, g.col_idx, g.values, s)
+    dgs = {"a": mg.Graph(a, symmetric=True), "b": mg.Graph(b, symmetric=True, relabel=True),
+           "shared": mg.Graph(shared, symmetric=True)}
+    errors = []
+
+    def worker(own):
+        try:
+            for rep in range(6):
+                for name in (own, "shared"):
+                    for s in (0, 7, 100):
+                        dg = dgs[name]
+                        if not np.array_equal(dg.bfs(s), exp[name, "bfs", s]):
+                            errors.append((own, name, "bfs", s, rep))
+                        if not np.array_equal(dg.sssp(s), exp[name, "sssp", s]):
+                            errors.append((own, name, "sssp", s, rep))
+        except Exception as e:  # reported below, in the main thread
+            errors.append((own, repr(e)))
+
+    try:
+        th = [threading.Thread(target=worker, args=(o,)) for o in ("a", "b", "a", "b")]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+    finally:
+        for d in dgs.values():
+            d.close()
+    assert not errors, errors[:5]
