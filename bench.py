@@ -193,12 +193,12 @@ def grid_part(mg, oracle, seed: int, reps: int, log) -> dict:
     return out
 
 
-def pagerank_part(mg, oracle, seed: int, reps: int, log) -> dict:
+def pagerank_part(mg, oracle, seed: int, reps: int, log, relabel: bool = True) -> dict:
     """PageRank on the directed scale-22 R-MAT (dangling vertices kept), all-vertex Jacobi, damping 0.85,
     FP32, 50 iterations (the L1 stop at 1e-6 never triggers in FP32 at this size).  Ranks are compared with
     the FP32 Jacobi oracle (normalised max-abs <= 1e-6) and their error against the FP64 oracle is stated."""
     host = mg.rmat_graph(22, 16, seed=seed, symmetric=False)
-    with mg.Graph(host, build_csc=True) as dg:
+    with mg.Graph(host, build_csc=True, relabel=relabel) as dg:
         o = mg.options(pr_mode=mg.MG_PR_ALL, iter_num=50, pr_tol=1e-6)
         t, st = median_ms(lambda: dg.pagerank(o, stats=True, copy=False), reps)
         got = dg.pagerank(o)
@@ -372,7 +372,9 @@ def serving_part(mg, seed: int, seconds: float, rate: float, log) -> dict:
            "steady_p50_ms": float(np.percentile(lat[steady], 50)) if steady.any() else None,
            "steady_p99_ms": float(np.percentile(lat[steady], 99)) if steady.any() else None,
            "aggregate_gteps": te / (t_end - t_start) / 1e6, "completed_qps": nq / ((t_end - t_start) / 1e3),
-           "device_bytes": st["device_bytes"], "graphs": names, "checked_queries": len(checked)}
+           "device_bytes": st["device_bytes"], "graphs": names, "checked_queries": len(checked),
+           "thin_engines": st["thin_engines"], "dispatcher_busy_frac": st["dispatch_busy_ms"] / (t_end - t_start),
+           "dispatcher_passes": st["dispatch_passes"], "rounds": st["rounds"]}
     per = {}
     for k, kn in [(mg.MG_Q_BFS, "bfs"), (mg.MG_Q_SSSP, "sssp"), (mg.MG_Q_PPR, "ppr")]:
         for g, gn in enumerate(names):
@@ -432,6 +434,8 @@ def main() -> int:
                          "batched vs plain) and serve (Poisson query stream over six resident graphs); '' for none")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="length of the serving arrival window")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="mean query arrival rate (queries/s)")
+    ap.add_argument("--no-relabel", action="store_true",
+                    help="keep the R-MAT graphs in the caller's numbering only (no degree-ordered twin)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="for counter runs: one warm-up and one measured SSSP, CC and BFS search, then a "
@@ -452,7 +456,11 @@ def main() -> int:
     t0 = time.perf_counter()
     g = mg.rmat_graph(a.scale, 16, seed=a.seed, weight_max=64)
     log(f"graph s{a.scale}: n={g.n} m={g.m} built in {time.perf_counter() - t0:.1f}s")
-    dg = mg.Graph(g, symmetric=True)
+    t0 = time.perf_counter()
+    # resident with its degree-ordered twin: BFS / SSSP run there (results in the caller's ids)
+    dg = mg.Graph(g, symmetric=True, relabel=not a.no_relabel)
+    create_s = time.perf_counter() - t0
+    log(f"device graph created in {create_s:.1f}s (relabel {not a.no_relabel})")
     src = 0
 
     if a.pmc_pass:
@@ -476,8 +484,15 @@ def main() -> int:
         dg.close()
         return 0
 
+    # the autotuner's choices for this graph class (mirrorofmapgraph_amd/tuned.json; defaults when absent)
+    from mirrorofmapgraph_amd.tuning import tuned_options
+
+    o_sssp, tuned_sssp = tuned_options(g, "sssp")
+    o_bfs, tuned_bfs = tuned_options(g, "bfs")
+    log(f"tuned options: sssp {tuned_sssp}, bfs {tuned_bfs}")
+
     def step():
-        _, s1 = dg.sssp(src, stats=True, copy=False)
+        _, s1 = dg.sssp(src, o_sssp, stats=True, copy=False)
         _, s2 = dg.cc(stats=True, copy=False)
         return s1, s2
 
@@ -500,10 +515,10 @@ def main() -> int:
 
     # BFS from the same source, timed beside the step
     for _ in range(2):
-        dg.bfs(src, stats=True, copy=False)
+        dg.bfs(src, o_bfs, stats=True, copy=False)
     bfs_ms = []
     for _ in range(a.steps):
-        _, sb = dg.bfs(src, stats=True, copy=False)
+        _, sb = dg.bfs(src, o_bfs, stats=True, copy=False)
         bfs_ms.append(sb.device_ms)
 
     def part(te, ms, nbytes):
@@ -544,9 +559,9 @@ def main() -> int:
             if d_ref is None:
                 d_ref = oracle.sssp(g.row_offsets, g.col_idx, g.values, src)
                 cc_ref = oracle.cc_union_find(g.row_offsets, g.col_idx)
-            require_equal("rmat22 SSSP from 0", dg.sssp(src), d_ref)
+            require_equal("rmat22 SSSP from 0", dg.sssp(src, o_sssp), d_ref)
             require_equal("rmat22 CC", dg.cc(), cc_ref)
-            require_equal("rmat22 BFS from 0", dg.bfs(src), oracle.bfs(g.row_offsets, g.col_idx, src))
+            require_equal("rmat22 BFS from 0", dg.bfs(src, o_bfs), oracle.bfs(g.row_offsets, g.col_idx, src))
             del d_ref, cc_ref
             log("main step checked against the oracle")
 
@@ -560,7 +575,7 @@ def main() -> int:
         if "grid" in parts_wanted:
             parts["grid2048"] = grid_part(mg, oracle, a.seed, a.reps, log)
         if "pagerank" in parts_wanted:
-            parts["pagerank_s22d"] = pagerank_part(mg, oracle, a.seed, a.reps, log)
+            parts["pagerank_s22d"] = pagerank_part(mg, oracle, a.seed, a.reps, log, relabel=not a.no_relabel)
         if "batch" in parts_wanted:
             parts["batched"] = batched_part(mg, oracle, a.seed, a.reps, log)
         if "serve" in parts_wanted:
@@ -575,7 +590,9 @@ def main() -> int:
         "dtype": "int32", "data": "synthetic",
         "config": {"workload": "rmat22_sssp_cc", "scale": a.scale, "edge_factor": 16, "rmat": [0.57, 0.19, 0.19, 0.05],
                    "symmetrized": True, "n": int(dg.n), "m": int(dg.m), "weights": [1, 64], "source": src,
-                   "seed": a.seed, "parallelism": "single GPU"},
+                   "seed": a.seed, "parallelism": "single GPU",
+                   "layout": "caller ids" if a.no_relabel else "degree-ordered twin (MG_GRAPH_RELABEL)",
+                   "graph_create_s": create_s, "tuned_options": {"sssp": tuned_sssp, "bfs": tuned_bfs}},
         "checked_against_oracle": not a.no_check,
         "roofline": roofline, "cpu_baseline": cpu, "parts": parts,
     }

@@ -303,7 +303,12 @@ typedef struct mg_server_config {
     float   ppr_teleport;     /* personalized PageRank teleport t (default 0.15)         */
     int32_t steps_per_round;  /* levels per engine round between admissions (default 8)  */
     int32_t sssp_delta;       /* SSSP bucket width, as mg_options.sssp_delta (0 = auto)  */
-    int32_t reserved[7];
+    int32_t thin_levels;      /* BFS / SSSP on a graph whose probe BFS (from a sampled
+                                 vertex, at mg_server_add_graph) takes at least this many
+                                 levels run as thin searches: one workgroup runs a whole
+                                 query inside one launch (0 = 256; -1 = never)          */
+    int32_t thin_lanes;       /* concurrent thin queries per (graph, kind) (0 = 32)      */
+    int32_t reserved[5];
 } mg_server_config;
 
 typedef struct mg_query_info {
@@ -318,7 +323,11 @@ typedef struct mg_query_info {
 typedef struct mg_server_stats {
     int64_t submitted, completed, rounds, edges_traversed, device_bytes;
     double  latency_ms_sum;
-    int32_t reserved[8];
+    int32_t thin_engines;     /* (graph, kind) engines serving thin searches              */
+    int32_t pad;
+    double  dispatch_busy_ms; /* dispatcher time spent advancing engines (host side)     */
+    int64_t dispatch_passes;  /* dispatcher passes over the engines                      */
+    int32_t reserved[2];
 } mg_server_stats;
 
 void mg_server_config_default(mg_server_config *c);

@@ -94,7 +94,7 @@ _lib = None
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("max_slots", C.c_int32), ("ppr_iters", C.c_int32),
                 ("ppr_teleport", C.c_float), ("steps_per_round", C.c_int32), ("sssp_delta", C.c_int32),
-                ("reserved", C.c_int32 * 7)]
+                ("thin_levels", C.c_int32), ("thin_lanes", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 MG_TUNE_BFS, MG_TUNE_SSSP, MG_TUNE_BFS_BATCH, MG_TUNE_SSSP_BATCH = 0, 1, 2, 3
@@ -124,7 +124,8 @@ class QueryInfo(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rounds", C.c_int64),
                 ("edges_traversed", C.c_int64), ("device_bytes", C.c_int64), ("latency_ms_sum", C.c_double),
-                ("reserved", C.c_int32 * 8)]
+                ("thin_engines", C.c_int32), ("pad", C.c_int32), ("dispatch_busy_ms", C.c_double),
+                ("dispatch_passes", C.c_int64), ("reserved", C.c_int32 * 2)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -20,6 +20,7 @@
 // not depend on what else runs: BFS depths and SSSP distances are unique, and a
 // PageRank slot's arithmetic is fixed by the graph.
 #include "batch.h"
+#include "thin.h"
 
 #include <climits>
 #include <condition_variable>
@@ -46,6 +47,15 @@ struct Query {
 
 struct Server;
 
+// A launch of thin searches in flight on its own stream.
+struct ThinSlot {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    bool busy = false;
+    std::vector<Query *> qs;
+};
+constexpr int THIN_STREAMS = 2;  // thin launches in flight per engine
+
 struct Engine {
     Server *srv = nullptr;
     mg_graph *g = nullptr;
@@ -63,7 +73,12 @@ struct Engine {
     std::vector<Query *> retiring;          // rows copied in the round in flight
     int running = 0;
     double idle_since = 0;
-    int64_t bytes() const { return allocated ? (kind == MG_Q_PPR ? p.bytes : b.bytes) : 0; }
+    // thin engines (BFS / SSSP on a high-diameter graph): one workgroup per query
+    bool thin = false;
+    ThinDev td;
+    std::vector<ThinSlot> tslots;
+    std::vector<int> free_lanes;
+    int64_t bytes() const { return allocated ? (kind == MG_Q_PPR ? p.bytes : thin ? td.bytes : b.bytes) : 0; }
 };
 
 struct Server {
@@ -96,14 +111,40 @@ static void engine_free(Server &s, Engine &e) {
     e.chunk = nullptr;
     s.bytes_used -= e.bytes();
     if (e.kind == MG_Q_PPR) ppr_dev_free(e.p);
+    else if (e.thin) thin_dev_free(e.td);
     else batch_dev_free(e.b);
+    e.free_lanes.clear();
     e.allocated = false;
     e.S = 0;
 }
 
 // Allocates the engine's device state within the budget: idle engines are freed
 // first if needed, and the slot count is halved until it fits.
+static int thin_lanes(const Server &s) { return s.cfg.thin_lanes > 0 ? s.cfg.thin_lanes : 32; }
+
 static int engine_alloc(Server &s, Engine &e) {
+    if (e.thin) {
+        const bool sssp = e.kind == MG_Q_SSSP;
+        int L = thin_lanes(s);
+        if (s.bytes_used + thin_dev_bytes(e.g, sssp, L) > s.cfg.hbm_budget) {
+            for (auto &o : s.engines)
+                if (o && o.get() != &e && o->allocated && !o->inflight && o->running == 0 && o->waiting.empty())
+                    engine_free(s, *o);
+        }
+        while (L > 1 && s.bytes_used + thin_dev_bytes(e.g, sssp, L) > s.cfg.hbm_budget) L /= 2;
+        if (s.bytes_used + thin_dev_bytes(e.g, sssp, L) > s.cfg.hbm_budget) return MG_ERR_OOM;  // wait for memory
+        mg_options o;
+        mg_options_default(&o);
+        o.sssp_delta = s.cfg.sssp_delta;
+        e.delta = sssp_delta(e.g, o);
+        int rc = thin_dev_alloc(e.td, e.g, sssp, L, e.delta);
+        if (rc) return rc;
+        e.allocated = true;
+        s.bytes_used += e.bytes();
+        e.free_lanes.clear();
+        for (int l = L - 1; l >= 0; --l) e.free_lanes.push_back(l);
+        return MG_OK;
+    }
     const int want = e.kind == MG_Q_PPR ? PPR_SLOTS : std::min(MG_BATCH_MAX, std::max(1, s.cfg.max_slots));
     auto need = [&](int S) {
         return e.kind == MG_Q_PPR ? ppr_dev_bytes(e.g) : batch_dev_bytes(e.g, e.kind == MG_Q_SSSP, S);
@@ -164,8 +205,79 @@ static void finish_query(Server &s, Query *q, double t) {
     s.stats.latency_ms_sum += q->t_done - q->t_submit;
 }
 
+// One round of a thin engine: finished launches hand their queries back (rows and
+// counts are in place when a launch's event completes), then waiting queries are
+// launched, one workgroup each, on every idle launch stream.  Called with s.mu held.
+static int thin_round(Server &s, Engine &e, bool *queued) {
+    *queued = false;
+    bool any_done = false;
+    for (ThinSlot &ts : e.tslots) {
+        if (!ts.busy) continue;
+        const hipError_t q = hipEventQuery(ts.ev);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) return set_error(MG_ERR_DEVICE, "thin search event: %s", hipGetErrorString(q));
+        ts.busy = false;
+        const double t = s.now();
+        for (Query *r : ts.qs) {
+            r->te = e.td.h_stats[2 * r->slot];
+            r->nr = e.td.h_stats[2 * r->slot + 1];
+            e.free_lanes.push_back(r->slot);
+            r->slot = -1;
+            finish_query(s, r, t);
+            e.running--;
+        }
+        ts.qs.clear();
+        any_done = true;
+    }
+    if (any_done) s.cv_done.notify_all();
+    e.inflight = false;
+    for (const ThinSlot &ts : e.tslots) e.inflight |= ts.busy;
+    if (e.waiting.empty()) {
+        if (e.running == 0 && e.idle_since == 0) e.idle_since = s.now();
+        return MG_OK;
+    }
+    e.idle_since = 0;
+    if (!e.allocated) {
+        const int rc = engine_alloc(s, e);
+        if (rc == MG_ERR_OOM && !e.allocated) {
+            clear_error();
+            return MG_OK;  // queries wait for memory
+        }
+        if (rc) return rc;
+    }
+    const size_t n = (size_t)e.g->n;
+    for (ThinSlot &ts : e.tslots) {
+        if (ts.busy || e.waiting.empty() || e.free_lanes.empty()) continue;
+        ThinBatch jb{};
+        while (jb.n < THIN_MAX_JOBS && !e.waiting.empty() && !e.free_lanes.empty()) {
+            Query *r = e.waiting.front();
+            e.waiting.pop_front();
+            r->slot = e.free_lanes.back();
+            e.free_lanes.pop_back();
+            r->t_admit = s.now();
+            jb.lane[jb.n] = r->slot;
+            jb.src[jb.n++] = r->src;
+            ts.qs.push_back(r);
+            e.running++;
+        }
+        int rc = thin_launch(e.td.a, jb, e.kind == MG_Q_SSSP, ts.st);
+        if (rc) return rc;
+        for (Query *r : ts.qs)
+            if (r->out)
+                MG_HIP(hipMemcpyAsync(r->out, e.td.a.rows + (size_t)r->slot * n, 4 * n,
+                                      r->out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ts.st));
+        MG_HIP(hipEventRecord(ts.ev, ts.st));
+        ts.busy = true;
+        e.inflight = true;
+        *queued = true;
+        s.stats.rounds++;
+    }
+    return MG_OK;
+}
+
 // One round of an engine; returns true if it queued work.  Called with s.mu held.
 static int engine_round(Server &s, Engine &e, bool *queued) {
+    if (e.thin) return thin_round(s, e, queued);
     *queued = false;
     std::vector<int> finished;
     if (e.inflight) {
@@ -311,6 +423,7 @@ static void dispatcher(Server *sp) {
     std::unique_lock<std::mutex> lk(s.mu);
     while (!s.stop) {
         bool any = false, pending = false;
+        const double t_pass = s.now();
         for (auto &ep : s.engines) {
             if (!ep) continue;
             bool q = false;
@@ -329,12 +442,21 @@ static void dispatcher(Server *sp) {
                         finish_query(s, r, s.now());
                         r = nullptr;
                     }
+                for (ThinSlot &ts : ep->tslots) {
+                    for (Query *r : ts.qs) {
+                        r->status = rc;
+                        finish_query(s, r, s.now());
+                    }
+                    ts.qs.clear();
+                }
                 ep->running = 0;
                 s.cv_done.notify_all();
             }
             any |= q;
             pending |= ep->inflight || ep->running > 0 || !ep->waiting.empty();
         }
+        s.stats.dispatch_busy_ms += s.now() - t_pass;
+        s.stats.dispatch_passes++;
         if (!any) {
             // nothing queued this pass: wait briefly for events or new queries
             s.cv_work.wait_for(lk, std::chrono::microseconds(pending ? 20 : 2000));
@@ -386,6 +508,29 @@ int mg_server_add_graph(mg_server *s, mg_graph *g, int32_t *graph_id) {
     clear_error();
     if (!s || !g || !graph_id) return set_error(MG_ERR_INVALID_ARG, "null argument");
     if (g->device != s->device) return set_error(MG_ERR_INVALID_ARG, "graph lives on another device");
+    // High-diameter graphs serve BFS / SSSP as thin searches: decided by the levels of
+    // a probe BFS from the first vertex with out-edges (before the server lock: the
+    // probe is an ordinary search on the graph)
+    bool thin = false;
+    const int thin_levels = s->cfg.thin_levels == 0 ? 256 : s->cfg.thin_levels;
+    if (thin_levels > 0 && g->n > 0) {
+        int probe = -1;
+        std::vector<int32_t> ro(2);
+        for (int v = 0; v < g->n && probe < 0; v += std::max(1, g->n / 64)) {
+            if (copy_on(g->stream, ro.data(), g->row_off + v, 8, hipMemcpyDeviceToHost) != hipSuccess)
+                return set_error(MG_ERR_DEVICE, "probe: offsets copy failed");
+            if (ro[1] > ro[0]) probe = v;
+        }
+        if (probe >= 0) {
+            mg_options po;
+            mg_options_default(&po);
+            po.out_on_device = 1;
+            mg_stats st;
+            const int rc = mg_bfs(g, probe, nullptr, &po, &st);
+            if (rc) return rc;
+            thin = st.iterations >= thin_levels;
+        }
+    }
     std::lock_guard<std::mutex> lk(s->mu);
     const int id = (int)s->graphs.size();
     s->graphs.push_back(g);
@@ -394,9 +539,17 @@ int mg_server_add_graph(mg_server *s, mg_graph *g, int32_t *graph_id) {
         e->srv = s;
         e->g = g;
         e->kind = k;
+        e->thin = thin && k != MG_Q_PPR;
         if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&e->ev, hipEventDisableTiming) != hipSuccess)
             return set_error(MG_ERR_DEVICE, "stream/event creation failed");
+        if (e->thin) {
+            e->tslots.resize(THIN_STREAMS);
+            for (ThinSlot &ts : e->tslots)
+                if (hipStreamCreateWithFlags(&ts.st, hipStreamNonBlocking) != hipSuccess ||
+                    hipEventCreateWithFlags(&ts.ev, hipEventDisableTiming) != hipSuccess)
+                    return set_error(MG_ERR_DEVICE, "stream/event creation failed");
+        }
         s->engines.push_back(std::move(e));
     }
     *graph_id = id;
@@ -477,6 +630,9 @@ int mg_server_get_stats(mg_server *s, mg_server_stats *st) {
     std::lock_guard<std::mutex> lk(s->mu);
     *st = s->stats;
     st->device_bytes = s->bytes_used;
+    st->thin_engines = 0;
+    for (const auto &e : s->engines)
+        if (e && e->thin) st->thin_engines++;
     return MG_OK;
 }
 
@@ -493,9 +649,15 @@ int mg_server_destroy(mg_server *s) {
     for (auto &e : s->engines) {
         if (!e) continue;
         if (e->st) (void)hipStreamSynchronize(e->st);
+        for (ThinSlot &ts : e->tslots)
+            if (ts.st) (void)hipStreamSynchronize(ts.st);
         engine_free(*s, *e);
         if (e->ev) (void)hipEventDestroy(e->ev);
         if (e->st) (void)hipStreamDestroy(e->st);
+        for (ThinSlot &ts : e->tslots) {
+            if (ts.ev) (void)hipEventDestroy(ts.ev);
+            if (ts.st) (void)hipStreamDestroy(ts.st);
+        }
     }
     delete s;
     return MG_OK;

@@ -79,6 +79,61 @@ def test_serving_mixed_equals_alone(mg, device, graphs):
     assert joined > 0
 
 
+@pytest.mark.parametrize("delta", [0, 1, 9, -1])
+def test_serving_thin_searches_equal_alone(mg, device, graphs, delta):
+    """thin_levels=1 serves every BFS / SSSP query as a thin search (one workgroup runs a
+    whole query inside one launch, with delta-stepping buckets of width `delta` for
+    SSSP, -1 unbucketed): results equal the plain searches bit for bit, with the same
+    TEPS counts, and queries join while others of the same engine run."""
+    hosts, dgs = graphs
+    qs = [q for q in _queries(mg, hosts, 160, seed=5 + delta) if q[1] != mg.MG_Q_PPR]
+    with mg.Server(thin_levels=1, thin_lanes=8, sssp_delta=delta) as srv:
+        ids = [srv.add_graph(d) for d in dgs]
+        assert srv.stats()["thin_engines"] == 2 * len(dgs)
+        outs, qids = [], []
+        for gi, kind, src in qs:
+            outs.append(mg.PinnedArray(hosts[gi].n))
+            qids.append(srv.submit(ids[gi], kind, src, outs[-1]))
+        infos = [srv.wait(q, 60000) for q in qids]
+    for (gi, kind, src), out, info in zip(qs, outs, infos):
+        exp = _alone(mg, dgs[gi], kind, src)
+        assert _same(mg, kind, out.array, exp), (gi, kind, src, delta)
+        reached = exp >= 0 if kind == mg.MG_Q_BFS else exp != 2**31 - 1
+        assert info["vertices_reached"] == int(reached.sum())
+        assert info["edges_traversed"] == int(np.diff(hosts[gi].row_offsets)[reached].sum())
+    joined = sum(1 for a, (qa, ia) in enumerate(zip(qs, infos)) for qb, ib in zip(qs[a + 1:], infos[a + 1:])
+                 if qa[:2] == qb[:2] and ia["t_admit_ms"] < ib["t_admit_ms"] < ia["t_done_ms"])
+    assert joined > 0
+
+
+def test_serving_high_diameter_graph_runs_thin(mg, oracle, device):
+    """A 300 x 300 grid (about 600 BFS levels) is detected by the probe search and its
+    BFS / SSSP engines run thin searches by default; the R-MAT graph beside it does
+    not.  Results equal the oracle's."""
+    grid = mg.grid_graph(300, seed=41, weight_max=16)
+    rm = mg.rmat_graph(12, 16, seed=42, weight_max=64)
+    dgs = [mg.Graph(grid, symmetric=True), mg.Graph(rm, symmetric=True)]
+    try:
+        with mg.Server() as srv:
+            ids = [srv.add_graph(d) for d in dgs]
+            assert srv.stats()["thin_engines"] == 2
+            srcs = [int(s) for s in mg.pick_sources(grid, 12, seed=3)]
+            outs, qids = [], []
+            for i, s in enumerate(srcs):
+                kind = mg.MG_Q_BFS if i % 2 == 0 else mg.MG_Q_SSSP
+                outs.append(mg.PinnedArray(grid.n))
+                qids.append(srv.submit(ids[0], kind, s, outs[-1]))
+            for q in qids:
+                srv.wait(q, 60000)
+        for i, s in enumerate(srcs):
+            exp = oracle.bfs(grid.row_offsets, grid.col_idx, s) if i % 2 == 0 else \
+                oracle.sssp(grid.row_offsets, grid.col_idx, grid.values, s)
+            assert np.array_equal(outs[i].array, exp), (i, s)
+    finally:
+        for d in dgs:
+            d.close()
+
+
 def test_serving_timed_stream(mg, device, graphs):
     hosts, dgs = graphs
     qs = _queries(mg, hosts, 200, seed=2)
