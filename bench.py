@@ -177,18 +177,23 @@ def rmat16_part(mg, oracle, seed: int, reps: int, log) -> dict:
 def grid_part(mg, oracle, seed: int, reps: int, log) -> dict:
     """BFS and SSSP from vertex 0 (a corner) of the 2048x2048 4-neighbour grid with 5% of the edges deleted
     (seeded) and weights 1..16: about 4,100 levels, the high-diameter case.  Results checked against the oracle."""
+    from mirrorofmapgraph_amd.tuning import tuned_options
+
     host = mg.grid_graph(2048, seed=seed, weight_max=16)
     out = {"n": host.n, "m": host.m}
     with mg.Graph(host, symmetric=True) as dg:
         for kind, fn, nb in [("bfs", dg.bfs, bytes_bfs), ("sssp", dg.sssp, bytes_sssp)]:
-            t, st = median_ms(lambda: fn(0, stats=True, copy=False), reps)
+            o, tuned = tuned_options(host, kind)
+            t, st = median_ms(lambda: fn(0, o, stats=True, copy=False), reps)
             b = nb(st.edges_traversed, st.vertices_reached)
             out[kind] = {"device_ms": t, "levels": st.iterations, "gteps": st.edges_traversed / t / 1e6,
                          "edges_examined": st.edges_examined, "us_per_level": 1e3 * t / max(st.iterations, 1),
-                         "hbm_frac": b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                         "hbm_frac": b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, "tuned_options": tuned}
             log(f"grid2048 {kind}: {t:.2f} ms, {st.iterations} levels")
-        require_equal("grid2048 BFS from 0", dg.bfs(0), oracle.bfs(host.row_offsets, host.col_idx, 0))
-        require_equal("grid2048 SSSP from 0", dg.sssp(0), oracle.sssp(host.row_offsets, host.col_idx, host.values, 0))
+        require_equal("grid2048 BFS from 0", dg.bfs(0, tuned_options(host, "bfs")[0]),
+                      oracle.bfs(host.row_offsets, host.col_idx, 0))
+        require_equal("grid2048 SSSP from 0", dg.sssp(0, tuned_options(host, "sssp")[0]),
+                      oracle.sssp(host.row_offsets, host.col_idx, host.values, 0))
     out["checked"] = True
     return out
 
@@ -227,6 +232,8 @@ def batched_part(mg, oracle, seed: int, reps: int, log) -> dict:
              ("grid2048", mg.grid_graph(2048, seed=seed, weight_max=16))]
     sources = {name: mg.pick_sources(host, 256, seed=seed) for name, host in hosts}
     out["both_graphs_one_search"] = multi_graph_part(mg, hosts, sources, reps, log)
+    from mirrorofmapgraph_amd.tuning import tuned_options
+
     for name, host in hosts:
         srcs = sources[name]
         with mg.Graph(host, symmetric=True) as dg:
@@ -235,13 +242,16 @@ def batched_part(mg, oracle, seed: int, reps: int, log) -> dict:
                     ("bfs", dg.bfs_batch, dg.bfs, lambda s: oracle.bfs(host.row_offsets, host.col_idx, s)),
                     ("sssp", dg.sssp_batch, dg.sssp,
                      lambda s: oracle.sssp(host.row_offsets, host.col_idx, host.values, s))]:
-                tb, st = median_ms(lambda: batch(srcs, stats=True, copy=False), reps)
+                # each form with the autotuner's options for this graph class and search kind
+                ob, tuned_b = tuned_options(host, kind + "_batch")
+                op, tuned_p = tuned_options(host, kind)
+                tb, st = median_ms(lambda: batch(srcs, ob, stats=True, copy=False), reps)
                 te = st.edges_traversed
-                rows = batch(srcs)
-                plain(int(srcs[0]), copy=False)
+                rows = batch(srcs, ob)
+                plain(int(srcs[0]), op, copy=False)
                 tp = 0.0
                 for j, v in enumerate(srcs):
-                    row, pst = plain(int(v), stats=True)
+                    row, pst = plain(int(v), op, stats=True)
                     tp += pst.device_ms
                     require_equal(f"{name} {kind} batched row {j} vs its plain search", rows[j], row)
                 for j in (0, 255):
@@ -249,7 +259,8 @@ def batched_part(mg, oracle, seed: int, reps: int, log) -> dict:
                 del rows
                 res[kind] = {"batched_ms": tb, "batched_gteps": te / tb / 1e6, "plain_ms": tp,
                              "plain_gteps": te / tp / 1e6, "speedup": tp / tb, "levels": st.iterations,
-                             "plain_sources_timed": 256, "checked": True}
+                             "plain_sources_timed": 256, "checked": True,
+                             "tuned_options": {"batched": tuned_b, "plain": tuned_p}}
                 log(f"batched {name} {kind}: {tb:.2f} ms vs plain {tp:.1f} ms ({tp / tb:.1f}x)")
         out[name] = res
     return out
