@@ -71,6 +71,16 @@ static Csr build(int n, const std::vector<int32_t> &s, const std::vector<int32_t
     g.ri.assign(m > 0 ? m : 1, 0);
     g.eid.assign(m > 0 ? m : 1, 0);
     CHECK(mg_csc_from_csr(n, m, g.ro.data(), g.col.data(), g.co.data(), g.ri.data(), g.eid.data()) == MG_OK);
+    // the degree-order renaming: a permutation, rows of the renamed graph non-increasing in degree
+    std::vector<int32_t> perm(n > 0 ? n : 1), ro2(n + 1), col2(m > 0 ? m : 1), val2(m > 0 ? m : 1);
+    CHECK(mg_csr_relabel_degree(n, m, g.ro.data(), g.col.data(), v ? g.val.data() : nullptr, perm.data(), ro2.data(),
+                                col2.data(), v ? val2.data() : nullptr) == MG_OK);
+    std::vector<char> seen(n > 0 ? n : 1, 0);
+    for (int i = 0; i < n; ++i) {
+        CHECK(perm[i] >= 0 && perm[i] < n && !seen[perm[i]]);
+        if (perm[i] >= 0 && perm[i] < n) seen[perm[i]] = 1;
+    }
+    for (int i = 0; i + 1 < n; ++i) CHECK(ro2[i + 1] - ro2[i] >= ro2[i + 2] - ro2[i + 1]);
     return g;
 }
 
@@ -213,6 +223,7 @@ int main(int argc, char **argv) {
     CHECK(mg_edge_weights_hash(2, ro, col, 2, 1, 0, out) != MG_OK);
     CHECK(mg_pick_sources(2, ro, -1, 1, out) != MG_OK);
     CHECK(mg_csc_from_csr(2, 2, nullptr, col, out, out, out) != MG_OK);
+    CHECK(mg_csr_relabel_degree(2, 2, nullptr, col, nullptr, out, out, out, nullptr) != MG_OK);
     CHECK(mg_mtx_read_header(nullptr, nullptr, nullptr, nullptr) != MG_OK);
     CHECK(mg_last_error(msg, sizeof msg) > 0);
     CHECK(mg_last_error(nullptr, 0) > 0);
