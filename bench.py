@@ -236,8 +236,9 @@ def batched_part(mg, oracle, seed: int, reps: int, log) -> dict:
 
     for name, host in hosts:
         srcs = sources[name]
-        with mg.Graph(host, symmetric=True) as dg:
-            res = {"n": host.n, "m": host.m, "sources": 256}
+        # the R-MAT graph keeps its degree-ordered twin (plain and batched searches both run there)
+        with mg.Graph(host, symmetric=True, relabel=name.startswith("rmat")) as dg:
+            res = {"n": host.n, "m": host.m, "sources": 256, "relabel": name.startswith("rmat")}
             for kind, batch, plain, orc in [
                     ("bfs", dg.bfs_batch, dg.bfs, lambda s: oracle.bfs(host.row_offsets, host.col_idx, s)),
                     ("sssp", dg.sssp_batch, dg.sssp,
@@ -321,7 +322,7 @@ def varied_sources_part(mg, oracle, dg, g, seed: int, log) -> dict:
     return out
 
 
-def serving_part(mg, seed: int, seconds: float, rate: float, log) -> dict:
+def serving_part(mg, seed: int, seconds: float, rate: float, log, mix=(0.6, 0.3, 0.1), graphs=None) -> dict:
     """Poisson query stream (seeded) over six resident graphs -- R-MAT scales 16, 18, 20, 22 (symmetrized,
     weights 1..64) and 1024^2 / 2048^2 grids (weights 1..16) -- with 60% BFS, 30% SSSP and 10% personalized
     PageRank (FP32, 20 iterations) from sources drawn among non-isolated vertices.  Latency = result copied
@@ -332,12 +333,15 @@ def serving_part(mg, seed: int, seconds: float, rate: float, log) -> dict:
     hosts = [mg.rmat_graph(s, 16, seed=seed + s, weight_max=64) for s in (16, 18, 20, 22)]
     hosts += [mg.grid_graph(w, seed=seed + w, weight_max=16) for w in (1024, 2048)]
     names = ["rmat16", "rmat18", "rmat20", "rmat22", "grid1024", "grid2048"]
+    if graphs:  # a subset of the six, for isolating engines
+        keep = [names.index(x) for x in graphs]
+        hosts, names = [hosts[i] for i in keep], [names[i] for i in keep]
     dgs = [mg.Graph(h, symmetric=True) for h in hosts]
     rng = np.random.default_rng(seed)
     nq = int(rng.poisson(rate * seconds))
     gaps = rng.exponential(1.0 / rate, nq)
     gsel = rng.integers(0, len(hosts), nq)
-    kinds = rng.choice([mg.MG_Q_BFS, mg.MG_Q_SSSP, mg.MG_Q_PPR], p=[0.6, 0.3, 0.1], size=nq)
+    kinds = rng.choice([mg.MG_Q_BFS, mg.MG_Q_SSSP, mg.MG_Q_PPR], p=list(mix), size=nq)
     pools = [mg.pick_sources(h, 4096, seed=seed + i) for i, h in enumerate(hosts)]
     srcs = [int(pools[g][i % 4096]) for i, g in enumerate(gsel)]
     bufs = [[mg.PinnedArray(h.n) for _ in range(4)] for h in hosts]  # result buffers, reused round-robin
@@ -378,7 +382,7 @@ def serving_part(mg, seed: int, seconds: float, rate: float, log) -> dict:
     te = sum(i["edges_traversed"] for i in infos)
     # steady state: queries arriving after the first second and before the arrival window ends
     steady = (arrive > 1000.0) & (arrive <= seconds * 1e3)
-    res = {"queries": nq, "rate_qps": rate, "arrival_window_s": seconds, "p50_ms": float(np.percentile(lat, 50)),
+    res = {"queries": nq, "rate_qps": rate, "arrival_window_s": seconds, "mix_bfs_sssp_ppr": list(mix), "p50_ms": float(np.percentile(lat, 50)),
            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
            "steady_p50_ms": float(np.percentile(lat[steady], 50)) if steady.any() else None,
            "steady_p99_ms": float(np.percentile(lat[steady], 99)) if steady.any() else None,
@@ -386,13 +390,16 @@ def serving_part(mg, seed: int, seconds: float, rate: float, log) -> dict:
            "device_bytes": st["device_bytes"], "graphs": names, "checked_queries": len(checked),
            "thin_engines": st["thin_engines"], "dispatcher_busy_frac": st["dispatch_busy_ms"] / (t_end - t_start),
            "dispatcher_passes": st["dispatch_passes"], "rounds": st["rounds"]}
+    # service time: admitted into a slot / lane until done (the rest of the latency is waiting)
+    svc = np.array([i["t_done_ms"] - i["t_admit_ms"] for i in infos])
     per = {}
     for k, kn in [(mg.MG_Q_BFS, "bfs"), (mg.MG_Q_SSSP, "sssp"), (mg.MG_Q_PPR, "ppr")]:
         for g, gn in enumerate(names):
             sel = (kinds == k) & (gsel == g)
             if sel.any():
                 per[f"{kn}_{gn}"] = {"n": int(sel.sum()), "p50_ms": float(np.percentile(lat[sel], 50)),
-                                     "p99_ms": float(np.percentile(lat[sel], 99))}
+                                     "p99_ms": float(np.percentile(lat[sel], 99)),
+                                     "service_p50_ms": float(np.percentile(svc[sel], 50))}
     res["per_kind_graph"] = per
     log(f"serving: {nq} queries, p50 {res['p50_ms']:.2f} ms, p99 {res['p99_ms']:.2f} ms, "
         f"{res['aggregate_gteps']:.1f} GTEPS, {res['completed_qps']:.0f} q/s")
@@ -445,6 +452,8 @@ def main() -> int:
                          "batched vs plain) and serve (Poisson query stream over six resident graphs); '' for none")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="length of the serving arrival window")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="mean query arrival rate (queries/s)")
+    ap.add_argument("--serve-mix", default="0.6,0.3,0.1", help="BFS,SSSP,PPR shares of the query stream")
+    ap.add_argument("--serve-graphs", default="", help="comma-separated subset of the six serving graphs")
     ap.add_argument("--no-relabel", action="store_true",
                     help="keep the R-MAT graphs in the caller's numbering only (no degree-ordered twin)")
     ap.add_argument("--verbose", action="store_true")
@@ -590,7 +599,9 @@ def main() -> int:
         if "batch" in parts_wanted:
             parts["batched"] = batched_part(mg, oracle, a.seed, a.reps, log)
         if "serve" in parts_wanted:
-            parts["serving"] = serving_part(mg, a.seed, a.serve_seconds, a.serve_rate, log)
+            mix = tuple(float(x) for x in a.serve_mix.split(","))
+            parts["serving"] = serving_part(mg, a.seed, a.serve_seconds, a.serve_rate, log, mix,
+                                            [x for x in a.serve_graphs.split(",") if x] or None)
     except Mismatch as e:
         print(f"bench.py: PARITY FAILURE: {e}", file=sys.stderr, flush=True)
         return 1

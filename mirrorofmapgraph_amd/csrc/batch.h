@@ -140,6 +140,7 @@ struct BatchDev {
     uint4 *V = nullptr, *F = nullptr, *N = nullptr, *FAR = nullptr;
     int *D = nullptr;                            // SSSP distances, n x KS
     int *rows = nullptr;                         // S x n results
+    int *rows_out = nullptr;                     // on a degree-ordered twin: rows in the caller's ids
     int2 *pairs = nullptr;                       // SSSP: (vertex, slot) pairs of sparse vertices
     int64_t pair_cap = 0;
     int nwords = 0;

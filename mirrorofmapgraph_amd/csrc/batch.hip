@@ -507,6 +507,7 @@ int64_t batch_dev_bytes(const mg_graph *g, bool sssp, int slots) {
     const int KS = (slots + 15) / 16 * 16;
     int64_t b = (int64_t)sizeof(Ctl) + sizeof(SlotCtl) + 3LL * MAX_GRID * sizeof(LevelPart) + 32LL * MAX_GRID;
     b += 2 * 4 * nwords + 3 * (32 * n + 32) + 4LL * slots * n + 4;
+    if (g->perm) b += 4LL * slots * n + 4;  // rows_out
     if (sssp) {
         b += (32 * n + 32) + 4LL * SB_RING * nwords + 4LL * KS * n + 64 + 8LL * 4 * n;
         b += (int64_t)sizeof(ChunkRec) * (g->m / SB_TASK + n + 64);
@@ -533,6 +534,7 @@ void batch_dev_free(BatchDev &b) {
     free_dev(b.D);
     free_dev(b.pairs);
     free_dev(b.rows);
+    free_dev(b.rows_out);
     if (b.h_ctl) (void)hipHostFree(b.h_ctl);
     if (b.h_sc) (void)hipHostFree(b.h_sc);
     b = BatchDev{};
@@ -563,6 +565,7 @@ int batch_dev_alloc(BatchDev &b, mg_graph *g, bool sssp, int slots) {
     A((void **)&b.F, 32 * n + 32);
     A((void **)&b.N, 32 * n + 32);
     A((void **)&b.rows, 4 * (size_t)slots * n + 4);
+    if (g->perm) A((void **)&b.rows_out, 4 * (size_t)slots * n + 4);
     if (sssp) {
         A((void **)&b.FAR, 32 * n + 32);
         A((void **)&b.ring, 4 * (size_t)SB_RING * b.nwords);
@@ -649,8 +652,26 @@ int check_batch_sources(const char *fn, const mg_graph *g, const int32_t *srcs, 
     return MG_OK;
 }
 
+// out[s * n + v] = rows[s * n + perm[v]]: batched rows of a degree-ordered twin in the
+// caller's numbering (grid: x over vertices, y over rows)
+__global__ void unpermute_rows(const int *perm, const int *rows, int n, int *out) {
+    const int64_t base = (int64_t)blockIdx.y * n;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        out[base + v] = rows[base + perm[v]];
+}
+
 int finish_batch(mg_graph *g, BatchDev &b, mg_stats *stats, void *out, size_t bytes, bool out_on_device,
                  const CallTimer &timer) {
+    const int *result = b.rows;
+    if (g->perm) {  // a search on a twin: every row back to the caller's ids
+        const int k = (int)(bytes / (4 * (size_t)std::max(g->n, 1)));
+        if (k > 0 && g->n > 0) {
+            hipLaunchKernelGGL(unpermute_rows, dim3(grid_blocks_cap(g->n, BLK, 64), k), dim3(BLK), 0, g->stream,
+                               (const int *)g->perm, (const int *)b.rows, g->n, b.rows_out);
+            MG_HIP(hipGetLastError());
+        }
+        result = b.rows_out;
+    }
     MG_HIP(hipEventRecord(g->ev1, g->stream));
     MG_HIP(hipEventSynchronize(g->ev1));
     float ms = 0;
@@ -667,7 +688,7 @@ int finish_batch(mg_graph *g, BatchDev &b, mg_stats *stats, void *out, size_t by
         stats->vertices_reached = b.h_ctl->reached;
         stats->device_ms = ms;
     }
-    int rc = deliver(g, out, b.rows, bytes, out_on_device);
+    int rc = deliver(g, out, result, bytes, out_on_device);
     if (rc) return rc;
     if (stats) stats->host_ms = timer.ms();
     return MG_OK;
@@ -739,9 +760,11 @@ extern "C" int mg_bfs_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_t
     if ((rc = check_batch_sources("mg_bfs_batch", g, srcs, k))) return rc;
     if (o.directed == 0 && !(g->flags & MG_GRAPH_SYMMETRIC))
         return set_error(MG_ERR_UNSUPPORTED, "mg_bfs_batch: directed=0 needs a symmetric graph");
-    int32_t rows[MG_BATCH_MAX];
+    int32_t rows[MG_BATCH_MAX], tsrc[MG_BATCH_MAX];
     for (int s = 0; s < k; ++s) rows[s] = s;
-    return bfs_batch_run(g, rows, srcs, k, k, depth_out, o, stats, timer);
+    mg_graph *sg = batch_graph(g, o, srcs, k, tsrc);  // the degree-ordered twin, when the graph keeps one
+    if (sg != g && (rc = ensure_workspace(sg))) return rc;
+    return bfs_batch_run(sg, rows, sg != g ? tsrc : srcs, k, k, depth_out, o, stats, timer);
 }
 
 extern "C" int mg_bfs_batch_sets(mg_graph *g, const int32_t *row_of, const int32_t *srcs, int32_t npairs, int32_t k,
@@ -754,5 +777,8 @@ extern "C" int mg_bfs_batch_sets(mg_graph *g, const int32_t *row_of, const int32
     if ((rc = check_batch_sets("mg_bfs_batch_sets", g, row_of, srcs, npairs, k))) return rc;
     if (o.directed == 0 && !(g->flags & MG_GRAPH_SYMMETRIC))
         return set_error(MG_ERR_UNSUPPORTED, "mg_bfs_batch_sets: directed=0 needs a symmetric graph");
-    return bfs_batch_run(g, row_of, srcs, npairs, k, depth_out, o, stats, timer);
+    std::vector<int32_t> tsrc((size_t)npairs);
+    mg_graph *sg = batch_graph(g, o, srcs, npairs, tsrc.data());
+    if (sg != g && (rc = ensure_workspace(sg))) return rc;
+    return bfs_batch_run(sg, row_of, sg != g ? tsrc.data() : srcs, npairs, k, depth_out, o, stats, timer);
 }

@@ -676,9 +676,11 @@ extern "C" int mg_sssp_batch(mg_graph *g, const int32_t *srcs, int32_t k, int32_
     std::lock_guard<std::mutex> lock(g->mu);
     if ((rc = check_batch_sources("mg_sssp_batch", g, srcs, k))) return rc;
     if ((rc = sssp_batch_checks("mg_sssp_batch", g, o))) return rc;
-    int32_t rows[MG_BATCH_MAX];
+    int32_t rows[MG_BATCH_MAX], tsrc[MG_BATCH_MAX];
     for (int s = 0; s < k; ++s) rows[s] = s;
-    return sssp_batch_run(g, rows, srcs, k, k, dist_out, o, stats, timer);
+    mg_graph *sg = batch_graph(g, o, srcs, k, tsrc);  // the degree-ordered twin, when the graph keeps one
+    if (sg != g && (rc = ensure_workspace(sg))) return rc;
+    return sssp_batch_run(sg, rows, sg != g ? tsrc : srcs, k, k, dist_out, o, stats, timer);
 }
 
 extern "C" int mg_sssp_batch_sets(mg_graph *g, const int32_t *row_of, const int32_t *srcs, int32_t npairs, int32_t k,
@@ -690,5 +692,8 @@ extern "C" int mg_sssp_batch_sets(mg_graph *g, const int32_t *row_of, const int3
     std::lock_guard<std::mutex> lock(g->mu);
     if ((rc = check_batch_sets("mg_sssp_batch_sets", g, row_of, srcs, npairs, k))) return rc;
     if ((rc = sssp_batch_checks("mg_sssp_batch_sets", g, o))) return rc;
-    return sssp_batch_run(g, row_of, srcs, npairs, k, dist_out, o, stats, timer);
+    std::vector<int32_t> tsrc((size_t)npairs);
+    mg_graph *sg = batch_graph(g, o, srcs, npairs, tsrc.data());
+    if (sg != g && (rc = ensure_workspace(sg))) return rc;
+    return sssp_batch_run(sg, row_of, sg != g ? tsrc.data() : srcs, npairs, k, dist_out, o, stats, timer);
 }

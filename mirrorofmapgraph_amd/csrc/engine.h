@@ -172,6 +172,15 @@ inline mg_graph *search_graph(mg_graph *g, const mg_options &o, int *src) {
     return g->twin;
 }
 
+// As search_graph for a batched call: the graph it runs on, with the sources renamed
+// into out[0, k) when that is the twin.
+inline mg_graph *batch_graph(mg_graph *g, const mg_options &o, const int32_t *srcs, int k, int32_t *out) {
+    mg_graph *t = search_graph(g, o, nullptr);
+    if (t != g)
+        for (int i = 0; i < k; ++i) out[i] = g->h_perm[(size_t)srcs[i]];
+    return t;
+}
+
 // Grows the workspace for graph g; returns an MG_* status.
 int ensure_workspace(mg_graph *g);
 // Fault injection for tests: with MG_FAIL_ALLOC_AT=k in the environment, the k-th

@@ -6,7 +6,9 @@
 // (dangling) mass dropped, teleport t applied as written there (t + (1 - t) * sum):
 //   x_0 = e_s,   x_{i+1}[u] = (1 - t) * sum_{v -> u} x_i[v] / outdeg(v)  (+ t when u = s).
 // Ranks are vertex-major, X[v * 64 + slot] (256 bytes per vertex), so one wave
-// gathers one in-edge for all slots (lane = slot; idle slots load nothing).
+// gathers one in-edge for all slots (lane = slot).  Slots are handed out lowest
+// first, so while at most 32 are in use only the first 128-byte line of a row is read
+// and written.
 // Vertices are covered by static tasks: runs of consecutive vertices with at most
 // PPR_T in-edges in total (one wave walks them in order), and PPR_T-edge pieces
 // of longer in-lists whose partial sums are added in piece order.  Every slot's
@@ -50,8 +52,11 @@ __global__ __launch_bounds__(BLK) void ppr_contrib(PprArgs a) {
 // Sum over in-edges [beg, end) of the lane's column, in edge order: full groups of
 // PPR_U rows with every load in flight (the group test is wave-uniform, so the
 // loads need no per-load branch), then the remainder one row at a time.
+// `col` is the column the lane reads: its own slot, or, when no slot of the upper half
+// of the row is in use (slots are handed out lowest first), the lower half's column
+// of the same cache line, so a row costs one 128-byte line instead of two.
 constexpr int PPR_U = 8;
-__device__ __forceinline__ float gather_sum(const PprArgs &a, int beg, int end, int lane, bool on) {
+__device__ __forceinline__ float gather_sum(const PprArgs &a, int beg, int end, int col, bool on) {
     float acc = 0.f;
     int e = beg;
     for (; e + PPR_U <= end; e += PPR_U) {
@@ -60,12 +65,18 @@ __device__ __forceinline__ float gather_sum(const PprArgs &a, int beg, int end, 
 #pragma unroll
         for (int j = 0; j < PPR_U; ++j) v[j] = a.in_adj[e + j];
 #pragma unroll
-        for (int j = 0; j < PPR_U; ++j) c[j] = a.C[(int64_t)v[j] * PPR_SLOTS + lane];
+        for (int j = 0; j < PPR_U; ++j) c[j] = a.C[(int64_t)v[j] * PPR_SLOTS + col];
 #pragma unroll
         for (int j = 0; j < PPR_U; ++j) acc += c[j];
     }
-    for (; e < end; ++e) acc += a.C[(int64_t)a.in_adj[e] * PPR_SLOTS + lane];
+    for (; e < end; ++e) acc += a.C[(int64_t)a.in_adj[e] * PPR_SLOTS + col];
     return on ? acc : 0.f;
+}
+
+// The column lane `lane` reads: its own, unless the row's upper half is idle.
+__device__ __forceinline__ int ppr_col(int lane, bool on) {
+    const bool hi = __any(on && lane >= PPR_SLOTS / 2);
+    return hi ? lane : (lane & (PPR_SLOTS / 2 - 1));
 }
 
 __device__ __forceinline__ float ppr_apply(float s, float t, bool is_src) {
@@ -79,16 +90,17 @@ __global__ __launch_bounds__(BLK) void ppr_gather(PprArgs a) {
     const int src = a.src[lane];
     const bool on = src >= 0;
     if (!__any(on)) return;
+    const int col = ppr_col(lane, on);
     for (int t = wave; t < a.ntasks; t += nwaves) {
         const ChunkRec r = a.tasks[t];
         if (r.pad > 0) {
             for (int u = r.v; u < r.v + r.pad; ++u) {
-                const float s = gather_sum(a, a.in_off[u], a.in_off[u + 1], lane, on);
-                a.Xn[(int64_t)u * PPR_SLOTS + lane] = on ? ppr_apply(s, a.t, u == src) : 0.f;
+                const float s = gather_sum(a, a.in_off[u], a.in_off[u + 1], col, on);
+                if (on) a.Xn[(int64_t)u * PPR_SLOTS + lane] = ppr_apply(s, a.t, u == src);  // idle columns unused
             }
         } else {
-            const float s = gather_sum(a, r.beg, r.end, lane, on);
-            a.part[(int64_t)(-r.pad - 1) * PPR_SLOTS + lane] = s;
+            const float s = gather_sum(a, r.beg, r.end, col, on);
+            if (on) a.part[(int64_t)(-r.pad - 1) * PPR_SLOTS + lane] = s;
         }
     }
 }
@@ -102,7 +114,7 @@ __global__ __launch_bounds__(BLK) void ppr_combine(PprArgs a) {
         const int u = a.big[3 * b], p0 = a.big[3 * b + 1], np = a.big[3 * b + 2];
         float s = 0.f;
         for (int p = p0; p < p0 + np; ++p) s += on ? a.part[(int64_t)p * PPR_SLOTS + lane] : 0.f;
-        a.Xn[(int64_t)u * PPR_SLOTS + lane] = on ? ppr_apply(s, a.t, u == src) : 0.f;
+        if (on) a.Xn[(int64_t)u * PPR_SLOTS + lane] = ppr_apply(s, a.t, u == src);
     }
 }
 

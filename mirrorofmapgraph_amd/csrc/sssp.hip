@@ -167,7 +167,7 @@ struct SsspPush {
 // the wave, and contributes nothing).  On R-MAT from a hub most distances are small
 // after the first wide level, so the dense levels gather for a few percent of their
 // edges: the pull streams neighbour ids and weights and gathers little.
-constexpr int PULL_U = 8;
+constexpr int PULL_U = 4;
 template <bool EID>
 __device__ __forceinline__ long long pull_best(const SsspArgs &a, int beg, int end, int step, long long fmin,
                                                long long dv, int hot) {
@@ -225,7 +225,7 @@ __device__ __forceinline__ long long pull_best(const SsspArgs &a, int beg, int e
 //     them) and their frontier words set with one atomicOr each (long-list vertices
 //     of the same words are marked by their pieces).
 // mn takes the smallest improved distance (the next level's fmin).
-constexpr int DENSE_WIN = 4;
+constexpr int DENSE_WIN = 2;  // windows in flight (4 and 8: more registers, fewer waves, slower)
 constexpr int DENSE_PER = DENSE_V / WAVE;  // vertices per lane in a range task
 struct DenseLds {
     int pre[BLK / WAVE][DENSE_V + 1];

@@ -328,3 +328,35 @@ def test_union_graph_and_sets_argument_checks(mg, device):
     finally:
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("kind", ["bfs", "sssp"])
+def test_batch_on_relabelled_twin(mg, oracle, device, kind):
+    """A graph created with MG_GRAPH_RELABEL runs batched searches on its degree-ordered
+    copy (sources renamed in, every row mapped back): rows equal the oracle's and the
+    caller-numbered batch (layout 1), stats equal; the same through the per-row source
+    sets entry point."""
+    g = mg.rmat_graph(13, 16, seed=17, weight_max=64)
+    srcs = mg.pick_sources(g, 96, seed=5)
+    with mg.Graph(g, symmetric=True, relabel=True) as dg:
+        fn = dg.bfs_batch if kind == "bfs" else dg.sssp_batch
+        got, st = fn(srcs, stats=True)
+        same, st1 = fn(srcs, mg.options(layout=1), stats=True)
+        sets = dg.bfs_batch_sets if kind == "bfs" else dg.sssp_batch_sets
+        rows_of = np.arange(96, dtype=np.int32) % 48
+        two = sets(rows_of, srcs, 48)
+    assert np.array_equal(got, same)
+    assert st.edges_traversed == st1.edges_traversed and st.vertices_reached == st1.vertices_reached
+    for j in (0, 5, 95):
+        s = int(srcs[j])
+        exp = oracle.bfs(g.row_offsets, g.col_idx, s) if kind == "bfs" else \
+            oracle.sssp(g.row_offsets, g.col_idx, g.values, s)
+        assert np.array_equal(got[j], exp), j
+    # rows of the sets call: row r is the elementwise minimum over its two sources' rows
+    for r in (0, 47):
+        a, b = got[r], got[r + 48]
+        if kind == "bfs":
+            exp = np.where((a >= 0) & (b >= 0), np.minimum(a, b), np.maximum(a, b))
+        else:
+            exp = np.minimum(a, b)
+        assert np.array_equal(two[r], exp), r
