@@ -34,16 +34,18 @@ from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
-# Serving runs one stream per (graph, query kind) engine (18 for six graphs); with HIP's
-# default of 4 hardware queues per process, short rounds queue behind long ones.  Read at
-# runtime initialisation, so it is set before the library loads.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# Serving runs one stream per (graph, query kind) engine (18 for six graphs) plus three
+# launch streams per thin engine (12 for the two grids); streams that share a hardware
+# queue run in order, so short rounds would queue behind long launches.  Read at runtime
+# initialisation, so it is set before the library loads.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "32")
 
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 INT_MAX = 2**31 - 1
 PR_TOL = 1e-6  # PageRank: max-abs in normalised units p = r / n
+SERVE_CONFIG = {}  # mg_server_config fields the serving part sets (defaults otherwise)
 METRIC = "GTEPS & %HBM roofline, R-MAT s22 BFS/SSSP; batched vs plain; p99 query ms"
 
 
@@ -322,7 +324,8 @@ def varied_sources_part(mg, oracle, dg, g, seed: int, log) -> dict:
     return out
 
 
-def serving_part(mg, seed: int, seconds: float, rate: float, log, mix=(0.6, 0.3, 0.1), graphs=None) -> dict:
+def serving_part(mg, seed: int, seconds: float, rate: float, log, mix=(0.6, 0.3, 0.1), graphs=None,
+                 cfg=None) -> dict:
     """Poisson query stream (seeded) over six resident graphs -- R-MAT scales 16, 18, 20, 22 (symmetrized,
     weights 1..64) and 1024^2 / 2048^2 grids (weights 1..16) -- with 60% BFS, 30% SSSP and 10% personalized
     PageRank (FP32, 20 iterations) from sources drawn among non-isolated vertices.  Latency = result copied
@@ -348,7 +351,8 @@ def serving_part(mg, seed: int, seconds: float, rate: float, log, mix=(0.6, 0.3,
     checked = {i: mg.PinnedArray(hosts[gsel[i]].n) for i in range(0, nq, 25)}  # kept for the alone check
     qids = [0] * nq
     arrive = np.cumsum(gaps) * 1e3  # ms after the stream starts
-    with mg.Server() as srv:
+    cfg = dict(SERVE_CONFIG, **(cfg or {}))
+    with mg.Server(**cfg) as srv:
         ids = [srv.add_graph(d) for d in dgs]
         t_start = srv.now_ms()
 
@@ -389,7 +393,7 @@ def serving_part(mg, seed: int, seconds: float, rate: float, log, mix=(0.6, 0.3,
            "aggregate_gteps": te / (t_end - t_start) / 1e6, "completed_qps": nq / ((t_end - t_start) / 1e3),
            "device_bytes": st["device_bytes"], "graphs": names, "checked_queries": len(checked),
            "thin_engines": st["thin_engines"], "dispatcher_busy_frac": st["dispatch_busy_ms"] / (t_end - t_start),
-           "dispatcher_passes": st["dispatch_passes"], "rounds": st["rounds"]}
+           "dispatcher_passes": st["dispatch_passes"], "rounds": st["rounds"], "server_config": cfg}
     # service time: admitted into a slot / lane until done (the rest of the latency is waiting)
     svc = np.array([i["t_done_ms"] - i["t_admit_ms"] for i in infos])
     per = {}
@@ -454,6 +458,7 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="mean query arrival rate (queries/s)")
     ap.add_argument("--serve-mix", default="0.6,0.3,0.1", help="BFS,SSSP,PPR shares of the query stream")
     ap.add_argument("--serve-graphs", default="", help="comma-separated subset of the six serving graphs")
+    ap.add_argument("--serve-config", default="", help="mg_server_config fields, e.g. max_slots=64,thin_lanes=32")
     ap.add_argument("--no-relabel", action="store_true",
                     help="keep the R-MAT graphs in the caller's numbering only (no degree-ordered twin)")
     ap.add_argument("--verbose", action="store_true")
@@ -600,8 +605,9 @@ def main() -> int:
             parts["batched"] = batched_part(mg, oracle, a.seed, a.reps, log)
         if "serve" in parts_wanted:
             mix = tuple(float(x) for x in a.serve_mix.split(","))
+            cfg = {k: int(v) for k, v in (kv.split("=") for kv in a.serve_config.split(",") if kv)}
             parts["serving"] = serving_part(mg, a.seed, a.serve_seconds, a.serve_rate, log, mix,
-                                            [x for x in a.serve_graphs.split(",") if x] or None)
+                                            [x for x in a.serve_graphs.split(",") if x] or None, cfg)
     except Mismatch as e:
         print(f"bench.py: PARITY FAILURE: {e}", file=sys.stderr, flush=True)
         return 1

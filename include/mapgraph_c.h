@@ -308,7 +308,8 @@ typedef struct mg_server_config {
                                  levels run as thin searches: one workgroup runs a whole
                                  query inside one launch (0 = 256; -1 = never)          */
     int32_t thin_lanes;       /* concurrent thin queries per (graph, kind) (0 = 32)      */
-    int32_t reserved[5];
+    int32_t thin_streams;     /* thin launches in flight per (graph, kind), 1..8 (0 = 2) */
+    int32_t reserved[4];
 } mg_server_config;
 
 typedef struct mg_query_info {

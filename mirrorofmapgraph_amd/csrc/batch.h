@@ -183,7 +183,9 @@ int check_batch_sets(const char *fn, const mg_graph *g, const int32_t *slot_of, 
 int bfs_batch_step(BatchDev &b, mg_graph *g, hipStream_t st);
 int sssp_batch_reset(BatchDev &b, mg_graph *g, hipStream_t st, int iter_limit, long long delta);
 // fresh: the context was just reset (distance columns already clear).
-int sssp_batch_seed(BatchDev &b, mg_graph *g, hipStream_t st, const SeedList &sl, long long delta, bool fresh);
+// defer: while the search runs, new slots start at the next bucket (serving; see sb_seed)
+int sssp_batch_seed(BatchDev &b, mg_graph *g, hipStream_t st, const SeedList &sl, long long delta, bool fresh,
+                    bool defer);
 int sssp_batch_step(BatchDev &b, mg_graph *g, hipStream_t st, long long delta);
 // SSSP: the distance columns of the listed slots into their rows.
 int sssp_batch_extract(BatchDev &b, mg_graph *g, hipStream_t st, const SlotList &sl);

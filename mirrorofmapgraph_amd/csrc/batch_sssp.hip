@@ -93,18 +93,32 @@ __global__ void sb_prepare(SBatchArgs a, M8 newmask, SlotList sl) {
 }
 
 // One thread per (slot, source) pair; see batch_seed for the level bookkeeping.
-__global__ void sb_seed(SBatchArgs a, SeedList sl) {
+// `defer` (serving): while the search runs, a new slot starts at the next bucket
+// instead of the current one -- its source pair waits in FAR (and the ring) and is
+// activated when the clock reaches that bucket.  Seeded into the current bucket, a
+// slot whose whole search fits in one bucket width keeps that bucket busy, and under
+// a steady stream of admissions the clock never advances: older slots waiting in
+// later buckets would starve.  Deferred, every slot waits at most for the current
+// bucket to drain.
+__global__ void sb_seed(SBatchArgs a, SeedList sl, int defer) {
     Ctl *c = a.ctl;
     const int i = threadIdx.x;
     const int pending = c->pending;
     const int Lf = pending ? c->level + 1 : c->level;
-    const long long base = c->thr - a.delta;  // the current bucket's lower bound
+    const bool later = defer && !c->done;
+    const long long T = c->thr;               // the current bucket's upper bound, the next one's base
+    const long long base = later ? T : T - a.delta;  // the slot's bucket base
     if (i < sl.n) {
         const int s = sl.slot[i], v = sl.src[i];
         const uint32_t sb = 1u << (s & 31);
         a.D[(int64_t)v * a.KS + s] = 0;
-        atomicOr(reinterpret_cast<uint32_t *>(a.F + 2 * v) + (s >> 5), sb);
-        atomicOr(&a.act[Lf & 1][v >> 5], 1u << (v & 31));
+        if (later) {
+            atomicOr(reinterpret_cast<uint32_t *>(a.FAR + 2 * v) + (s >> 5), sb);
+            atomicOr(&a.ring[(int64_t)((T / a.delta) % SB_RING) * a.nwords + (v >> 5)], 1u << (v & 31));
+        } else {
+            atomicOr(reinterpret_cast<uint32_t *>(a.F + 2 * v) + (s >> 5), sb);
+            atomicOr(&a.act[Lf & 1][v >> 5], 1u << (v & 31));
+        }
         atomicOr(&a.sc->assigned[s >> 5], sb);
         atomicOr(&a.sc->live[s >> 5], sb);
         a.sc->off[s] = base;
@@ -114,7 +128,11 @@ __global__ void sb_seed(SBatchArgs a, SeedList sl) {
     }
     __syncthreads();
     if (i == 0 && sl.n > 0) {
-        if (pending) {
+        if (later) {
+            // the clock must reach the slots' bucket: the close advances while a waiting
+            // bucket lies above the current one
+            c->far_hi = max(c->far_hi, T / a.delta);
+        } else if (pending) {
             atomicAdd((unsigned long long *)&a.part[0].nf, (unsigned long long)sl.n);
         } else {
             c->done = c->iter_limit <= 0;
@@ -559,7 +577,8 @@ int sssp_batch_reset(BatchDev &b, mg_graph *g, hipStream_t st, int iter_limit, l
     return MG_OK;
 }
 
-int sssp_batch_seed(BatchDev &b, mg_graph *g, hipStream_t st, const SeedList &sl, long long delta, bool fresh) {
+int sssp_batch_seed(BatchDev &b, mg_graph *g, hipStream_t st, const SeedList &sl, long long delta, bool fresh,
+                    bool defer) {
     if (sl.n <= 0) return MG_OK;
     SBatchArgs a = sssp_args(b, g, delta);
     if (!fresh) {
@@ -572,7 +591,7 @@ int sssp_batch_seed(BatchDev &b, mg_graph *g, hipStream_t st, const SeedList &sl
         }
         hipLaunchKernelGGL(sb_prepare, dim3(grid_blocks(g->n, BLK)), dim3(BLK), 0, st, a, host_mask8(m), ls);
     }
-    hipLaunchKernelGGL(sb_seed, dim3(1), dim3(MG_BATCH_MAX), 0, st, a, sl);
+    hipLaunchKernelGGL(sb_seed, dim3(1), dim3(MG_BATCH_MAX), 0, st, a, sl, defer ? 1 : 0);
     MG_HIP(hipGetLastError());
     return MG_OK;
 }
@@ -643,7 +662,7 @@ static int sssp_batch_run(mg_graph *g, const int32_t *slot_of, const int32_t *sr
                 sl.slot[sl.n] = slot_of[i];
                 sl.src[sl.n++] = srcs[i];
             }
-            if ((rc = sssp_batch_seed(*b, g, g->stream, sl, delta, true))) return rc;
+            if ((rc = sssp_batch_seed(*b, g, g->stream, sl, delta, true, false))) return rc;
         }
         auto step = [&]() -> int { return sssp_batch_step(*b, g, g->stream, delta); };
         if ((rc = drive(g, step, (int64_t)o.iter_num + 3, KIND_BATCH_SSSP, mix64((uint64_t)delta), b->ctl, b->h_ctl)))

@@ -235,10 +235,12 @@ constexpr int TRACE_CAP = 16384;  // levels recorded in the per-level trace (ver
 constexpr int PULL_TASK = 512;   // static pull tasks: in-edges per wave (8 per lane in flight)
 // Dense SSSP pull tasks: in-lists longer than DENSE_BIG are cut into DENSE_T-slot
 // pieces; the other vertices are grouped into ranges of whole bitmap words (at most
-// DENSE_V vertices) holding about DENSE_T in-edges.
+// DENSE_V vertices) holding about DENSE_T in-edges.  A range holds at most DENSE_T
+// in-edges, or a single word's (32 vertices of up to DENSE_BIG each): DENSE_RANGE_MAX.
 constexpr int DENSE_BIG = 512;
 constexpr int DENSE_T = 2048;
 constexpr int DENSE_V = 256;
+constexpr int DENSE_RANGE_MAX = DENSE_T > 32 * DENSE_BIG ? DENSE_T : 32 * DENSE_BIG;
 enum { KIND_BFS = 0, KIND_SSSP = 1, KIND_PR = 2, KIND_GAS = 3, KIND_BATCH = 4, KIND_BATCH_SSSP = 5, NKINDS = 8 };
 
 struct StepGraph {

@@ -132,6 +132,7 @@ int ensure_dense_tasks(mg_graph *g, const int *d_in_off) {
             edges = 0;
         }
         edges += we;
+        if (edges > DENSE_RANGE_MAX) return set_error(MG_ERR_OVERFLOW, "dense range of %lld in-edges", (long long)edges);
     }
     if (edges) t.push_back(ChunkRec{-1, va, n, 0});
     free_dev(ws.dense_tasks);

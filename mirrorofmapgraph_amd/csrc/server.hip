@@ -54,7 +54,6 @@ struct ThinSlot {
     bool busy = false;
     std::vector<Query *> qs;
 };
-constexpr int THIN_STREAMS = 2;  // thin launches in flight per engine
 
 struct Engine {
     Server *srv = nullptr;
@@ -121,6 +120,8 @@ static void engine_free(Server &s, Engine &e) {
 // Allocates the engine's device state within the budget: idle engines are freed
 // first if needed, and the slot count is halved until it fits.
 static int thin_lanes(const Server &s) { return s.cfg.thin_lanes > 0 ? s.cfg.thin_lanes : 32; }
+// Thin launches in flight per engine.
+static int thin_streams(const Server &s) { return s.cfg.thin_streams > 0 ? s.cfg.thin_streams : 2; }
 
 static int engine_alloc(Server &s, Engine &e) {
     if (e.thin) {
@@ -383,7 +384,7 @@ static int engine_round(Server &s, Engine &e, bool *queued) {
     if (e.kind == MG_Q_BFS) {
         if ((rc = bfs_batch_seed(e.b, e.g, e.st, seeds))) return rc;
     } else if (e.kind == MG_Q_SSSP) {
-        if ((rc = sssp_batch_seed(e.b, e.g, e.st, seeds, e.delta, false))) return rc;
+        if ((rc = sssp_batch_seed(e.b, e.g, e.st, seeds, e.delta, false, true))) return rc;
     } else if (seeds.n || fl.n) {
         if ((rc = ppr_seed(e.p, e.st, seeds, fl))) return rc;
     }
@@ -491,7 +492,7 @@ int mg_server_create(const mg_server_config *cfg, mg_server **out) {
     if (cfg) s->cfg = *cfg;
     else mg_server_config_default(&s->cfg);
     if (s->cfg.hbm_budget <= 0 || s->cfg.max_slots < 1 || s->cfg.max_slots > MG_BATCH_MAX || s->cfg.ppr_iters < 1 ||
-        s->cfg.steps_per_round < 1 || s->cfg.steps_per_round > 64) {
+        s->cfg.steps_per_round < 1 || s->cfg.steps_per_round > 64 || s->cfg.thin_streams < 0 || s->cfg.thin_streams > 8) {
         delete s;
         return set_error(MG_ERR_INVALID_ARG, "invalid server configuration");
     }
@@ -544,7 +545,7 @@ int mg_server_add_graph(mg_server *s, mg_graph *g, int32_t *graph_id) {
             hipEventCreateWithFlags(&e->ev, hipEventDisableTiming) != hipSuccess)
             return set_error(MG_ERR_DEVICE, "stream/event creation failed");
         if (e->thin) {
-            e->tslots.resize(THIN_STREAMS);
+            e->tslots.resize(thin_streams(*s));
             for (ThinSlot &ts : e->tslots)
                 if (hipStreamCreateWithFlags(&ts.st, hipStreamNonBlocking) != hipSuccess ||
                     hipEventCreateWithFlags(&ts.ev, hipEventDisableTiming) != hipSuccess)

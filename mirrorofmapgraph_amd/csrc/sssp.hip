@@ -215,23 +215,93 @@ __device__ __forceinline__ long long pull_best(const SsspArgs &a, int beg, int e
 //   * a piece of a long list: a wave minimum, one atomicMin into dist[v];
 //   * a range of up to DENSE_V vertices of whole bitmap words with short lists: the
 //     wave walks the concatenation of their in-lists in windows of 64 slots, one per
-//     lane; a lane finds its slot's owner by binary search in the lists' prefix sums
-//     (LDS), so neighbour ids and weights are coalesced loads and the neighbour
-//     distances the only gathers, DENSE_WIN windows in flight together.  Owners are
-//     non-decreasing along a window, so a segmented minimum scan across the wave
-//     reduces each owner's run and its last lane folds it into the owner's LDS
-//     minimum (which starts at the owner's distance and bounds its gathers); the
-//     range's own vertices are then updated with plain stores (no other task writes
-//     them) and their frontier words set with one atomicOr each (long-list vertices
-//     of the same words are marked by their pieces).
+//     lane, with the range index below (so neighbour ids and weights are coalesced
+//     loads and the neighbour distances the only gathers), DENSE_WIN windows in flight
+//     together.  Owners are non-decreasing along a window, so a segmented minimum scan
+//     across the wave reduces each owner's run and its last lane folds it into the
+//     owner's LDS minimum (which starts at the owner's distance and bounds its
+//     gathers); the range's own vertices are then updated with plain stores (no other
+//     task writes them) and their frontier words set with one atomicOr each
+//     (long-list vertices of the same words are marked by their pieces).
 // mn takes the smallest improved distance (the next level's fmin).
 constexpr int DENSE_WIN = 2;  // windows in flight (4 and 8: more registers, fewer waves, slower)
 constexpr int DENSE_PER = DENSE_V / WAVE;  // vertices per lane in a range task
+
+// Range index: a range task's lists concatenated, for the wave that walks them.  Bit p
+// of `heads` is set where a non-empty list starts at position p of the concatenation
+// (and at the end of the last list when that is not a multiple of 64), so the owner of
+// position p is the number of set bits at or below p, less one: one 64-bit word and a
+// lane bit count per window of 64 positions, the windows walked in order with a running
+// count of the owners before them.  A window's head bits are also the run starts of
+// its segmented scan.  Per owner (its rank among the range's non-empty lists):
+// `base`, the list's first edge slot less its position, and a 32-bit value.
+// A range holds at most DENSE_RANGE_MAX slots (checked where the tasks are built); the
+// heads rows also cover the words read past the end by the windows in flight.
+constexpr int RANGE_WORDS = DENSE_RANGE_MAX / WAVE + DENSE_WIN;
 struct DenseLds {
-    int pre[BLK / WAVE][DENSE_V + 1];
-    int beg[BLK / WAVE][DENSE_V];
-    int min[BLK / WAVE][DENSE_V];
+    uint64_t heads[BLK / WAVE][RANGE_WORDS];
+    int base[BLK / WAVE][DENSE_V];
+    int val[BLK / WAVE][DENSE_V];
 };
+
+// Builds the wave's range index over its lists (lane l's list j is list j * WAVE + l:
+// first edge slot beg[j], length len[j], value val[j]); rank[j] receives the list's
+// owner index (meaningful where len[j] > 0).  Returns the total length.
+__device__ __forceinline__ int range_index(DenseLds &lds, int wv, int lane, const int (&beg)[DENSE_PER],
+                                           const int (&len)[DENSE_PER], const int (&val)[DENSE_PER],
+                                           int (&rank)[DENSE_PER]) {
+    uint64_t *hd = lds.heads[wv];
+    int *bs = lds.base[wv], *vs = lds.val[wv];
+    int total = 0, lists = 0, pos[DENSE_PER];
+#pragma unroll
+    for (int j = 0; j < DENSE_PER; ++j) {
+        int incl = len[j];
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const int y = __shfl_up(incl, d);
+            if (lane >= d) incl += y;
+        }
+        pos[j] = total + incl - len[j];
+        total += __shfl(incl, WAVE - 1);
+        const uint64_t ne = wave_ballot(len[j] > 0);
+        rank[j] = lists + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ne >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ne, 0u));
+        lists += __popcll(ne);
+        if (len[j] > 0) {
+            bs[rank[j]] = beg[j] - pos[j];
+            vs[rank[j]] = val[j];
+        }
+    }
+    const int words = (total >> 6) + DENSE_WIN;  // <= RANGE_WORDS for total <= DENSE_RANGE_MAX
+    for (int i = lane; i < words; i += WAVE) hd[i] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int j = 0; j < DENSE_PER; ++j)
+        if (len[j] > 0) atomicOr((unsigned long long *)&hd[pos[j] >> 6], 1ull << (pos[j] & 63));
+    if (lane == 0 && (total & 63)) atomicOr((unsigned long long *)&hd[total >> 6], 1ull << (total & 63));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return total;
+}
+
+// The head bits of the window starting at position q (a multiple of 64), as a
+// wave-uniform value.
+__device__ __forceinline__ uint64_t range_word(const uint64_t *hd, int q) {
+    const uint64_t w = hd[q >> 6];
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)w);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(w >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Owner index of this lane's position in a window with head bits w, `before` owners
+// having started before the window.
+__device__ __forceinline__ int range_owner(uint64_t w, int before, int lane) {
+    const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(w >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)w, 0u));
+    return before + below + (int)((w >> lane) & 1) - 1;
+}
+
 template <bool EID>
 __device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, int vb, long long fmin, DenseLds &lds,
                                             long long &nf, long long &mf, long long &ex, long long &mn) {
@@ -241,10 +311,12 @@ __device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, in
     const int *eid = fa.eid[2];
     const int *off = fa.off[2];
     const int lane = lane_id(), wv = (int)threadIdx.x / WAVE;
-    int *pre = lds.pre[wv], *begs = lds.beg[wv], *mins = lds.min[wv];
+    const uint64_t *hd = lds.heads[wv];
+    const int *bs = lds.base[wv];
+    int *mins = lds.val[wv];
     const uint64_t upto = lane == WAVE - 1 ? ~0ull : ((2ull << lane) - 1);  // lanes 0..lane
     // vertex va + 64 j + lane is list 64 j + lane
-    int beg[DENSE_PER], len[DENSE_PER], cur[DENSE_PER];
+    int beg[DENSE_PER], len[DENSE_PER], cur[DENSE_PER], rank[DENSE_PER];
 #pragma unroll
     for (int j = 0; j < DENSE_PER; ++j) {
         const int v = va + j * WAVE + lane;
@@ -256,54 +328,25 @@ __device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, in
         // a vertex at or below fmin + (smallest weight) cannot improve: its list is skipped
         len[j] = in && o1 - o0 <= DENSE_BIG && fmin + a.wmin < cur[j] ? o1 - o0 : 0;
     }
-    int total = 0;
-#pragma unroll
-    for (int j = 0; j < DENSE_PER; ++j) {
-        int incl = len[j];
-#pragma unroll
-        for (int d = 1; d < WAVE; d <<= 1) {
-            const int y = __shfl_up(incl, d);
-            if (lane >= d) incl += y;
-        }
-        pre[j * WAVE + lane + 1] = total + incl;
-        begs[j * WAVE + lane] = beg[j];
-        mins[j * WAVE + lane] = cur[j];
-        total += __shfl(incl, WAVE - 1);
-    }
-    if (lane == 0) pre[0] = 0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int total = range_index(lds, wv, lane, beg, len, cur, rank);
     if (lane == 0) ex += total;  // the wave's edges, counted once
     // an in-bounds slot for the lanes past the end: off[va] (the windows run only when
     // some list of the range is non-empty, and it starts at or after off[va])
-    const int safe = begs[0];
+    const int safe = __builtin_amdgcn_readfirstlane(beg[0]);
+    int before = 0;
     for (int p0 = 0; p0 < total; p0 += DENSE_WIN * WAVE) {
-        int p[DENSE_WIN], own[DENSE_WIN], slot[DENSE_WIN];
+        int p[DENSE_WIN], own[DENSE_WIN], slot[DENSE_WIN], bound[DENSE_WIN];
+        uint64_t w[DENSE_WIN];
         bool ok[DENSE_WIN];
 #pragma unroll
         for (int k = 0; k < DENSE_WIN; ++k) {
+            w[k] = range_word(hd, p0 + k * WAVE);
             p[k] = p0 + k * WAVE + lane;
             ok[k] = p[k] < total;
-            own[k] = 0;
-        }
-        // owner = last list whose prefix is <= p (binary search, all windows in step)
-#pragma unroll
-        for (int st = DENSE_V / 2; st > 0; st >>= 1) {
-            int q[DENSE_WIN];
-#pragma unroll
-            for (int k = 0; k < DENSE_WIN; ++k) q[k] = pre[own[k] + st];
-#pragma unroll
-            for (int k = 0; k < DENSE_WIN; ++k)
-                if (q[k] <= p[k]) own[k] += st;
-        }
-        int first[DENSE_WIN], nextp[DENSE_WIN], bound[DENSE_WIN];
-#pragma unroll
-        for (int k = 0; k < DENSE_WIN; ++k) {
-            first[k] = pre[own[k]];
-            nextp[k] = pre[own[k] + 1];
+            own[k] = ok[k] ? range_owner(w[k], before, lane) : 0;
+            before += __popcll(w[k]);
+            slot[k] = ok[k] ? bs[own[k]] + p[k] : safe;
             bound[k] = mins[own[k]];
-            slot[k] = ok[k] ? begs[own[k]] + (p[k] - first[k]) : safe;
         }
         int u[DENSE_WIN], wt[DENSE_WIN];
 #pragma unroll
@@ -328,12 +371,11 @@ __device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, in
         for (int k = 0; k < DENSE_WIN; ++k) {
             const long long d = (long long)du[k] + wt[k];
             int best = need[k] && d < INT_MAX ? (int)d : INT_MAX;
-            // runs: one per owner; the lanes past the end form one more run
-            const bool head = lane == 0 || (ok[k] ? p[k] == first[k] : p[k] == total);
-            const uint64_t heads = wave_ballot(head);
-            const int start = 63 - __builtin_clzll(heads & upto);
+            // runs: one per owner (lane 0 starts one); the lanes past the end follow the
+            // end bit and form one more run
+            const int start = 63 - __builtin_clzll((w[k] | 1ull) & upto);
             best = seg_min_scan(best, lane, start);
-            const bool tail = lane == WAVE - 1 || p[k] + 1 == nextp[k];
+            const bool tail = lane == WAVE - 1 || ((w[k] >> lane) & 2ull);
             if (ok[k] && tail && best < bound[k]) atomicMin(&mins[own[k]], best);
         }
     }
@@ -346,7 +388,7 @@ __device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, in
 #pragma unroll
     for (int j = 0; j < DENSE_PER; ++j) {
         const int v = va + j * WAVE + lane;
-        const int best = mins[j * WAVE + lane];
+        const int best = len[j] > 0 ? mins[rank[j]] : INT_MAX;
         found[j] = len[j] > 0 && best < cur[j];
         if (found[j]) {
             a.dist[v] = best;
@@ -413,7 +455,7 @@ __device__ __forceinline__ void dense_pull(const SsspArgs &a, int L, DenseLds &l
 
 // Wide push levels of a symmetric graph (frontier out-edges above m / 16) reuse the
 // dense tasks, restricted to frontier vertices: a range task concatenates the lists of
-// its frontier vertices (the same LDS prefix sums and owner search as the dense pull,
+// its frontier vertices (the same range index as the dense pull,
 // so neighbour ids and weights are coalesced loads), a piece of a frontier hub's list
 // is walked by its wave; every edge relaxes its target with atomicMin and marks it.
 // One launch balances hubs and low-degree vertices, instead of lanes per vertex.
@@ -429,7 +471,8 @@ __device__ __forceinline__ void range_push(const SsspArgs &a, int L, DenseLds &l
     const int *adj = fa.adj[0];
     const int *off = fa.off[0];
     const int lane = lane_id(), wv = (int)threadIdx.x / WAVE;
-    int *pre = lds.pre[wv], *begs = lds.beg[wv], *dsrc = lds.min[wv];
+    const uint64_t *hd = lds.heads[wv];
+    const int *bs = lds.base[wv], *dsrc = lds.val[wv];
     const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
     const long long thr = a.bucketed ? fa.ctl->thr : UNBUCKETED;
     // a frontier vertex at or above the bucket bound waits in the next frontier (each
@@ -473,7 +516,7 @@ __device__ __forceinline__ void range_push(const SsspArgs &a, int L, DenseLds &l
             continue;
         }
         const int va = r.beg, vb = r.end;
-        int beg[DENSE_PER], len[DENSE_PER], dvs[DENSE_PER];
+        int beg[DENSE_PER], len[DENSE_PER], dvs[DENSE_PER], rank[DENSE_PER];
 #pragma unroll
         for (int j = 0; j < DENSE_PER; ++j) {
             const int v = va + j * WAVE + lane;
@@ -489,47 +532,20 @@ __device__ __forceinline__ void range_push(const SsspArgs &a, int L, DenseLds &l
                 defer(va + j * WAVE + lane, dvs[j]);
                 len[j] = 0;
             }
-        int total = 0;
-#pragma unroll
-        for (int j = 0; j < DENSE_PER; ++j) {
-            int incl = len[j];
-#pragma unroll
-            for (int d = 1; d < WAVE; d <<= 1) {
-                const int y = __shfl_up(incl, d);
-                if (lane >= d) incl += y;
-            }
-            pre[j * WAVE + lane + 1] = total + incl;
-            begs[j * WAVE + lane] = beg[j];
-            dsrc[j * WAVE + lane] = len[j] ? dvs[j] : 0;
-            total += __shfl(incl, WAVE - 1);
-        }
-        if (lane == 0) pre[0] = 0;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int safe = begs[0];  // in bounds when total > 0
+        const int total = range_index(lds, wv, lane, beg, len, dvs, rank);
+        const int safe = __builtin_amdgcn_readfirstlane(beg[0]);  // off[va]: in bounds when total > 0
+        int before = 0;
         for (int p0 = 0; p0 < total; p0 += DENSE_WIN * WAVE) {
-            int p[DENSE_WIN], own[DENSE_WIN];
-#pragma unroll
-            for (int k = 0; k < DENSE_WIN; ++k) {
-                p[k] = p0 + k * WAVE + lane;
-                own[k] = 0;
-            }
-#pragma unroll
-            for (int st = DENSE_V / 2; st > 0; st >>= 1) {
-                int q[DENSE_WIN];
-#pragma unroll
-                for (int k = 0; k < DENSE_WIN; ++k) q[k] = pre[own[k] + st];
-#pragma unroll
-                for (int k = 0; k < DENSE_WIN; ++k)
-                    if (q[k] <= p[k]) own[k] += st;
-            }
-            int sl[DENSE_WIN], u[DENSE_WIN], wt[DENSE_WIN], du[DENSE_WIN];
+            int p[DENSE_WIN], own[DENSE_WIN], sl[DENSE_WIN], u[DENSE_WIN], wt[DENSE_WIN], du[DENSE_WIN];
             bool ok[DENSE_WIN];
 #pragma unroll
             for (int k = 0; k < DENSE_WIN; ++k) {
+                const uint64_t w = range_word(hd, p0 + k * WAVE);
+                p[k] = p0 + k * WAVE + lane;
                 ok[k] = p[k] < total;
-                sl[k] = ok[k] ? begs[own[k]] + (p[k] - pre[own[k]]) : safe;
+                own[k] = ok[k] ? range_owner(w, before, lane) : 0;
+                before += __popcll(w);
+                sl[k] = ok[k] ? bs[own[k]] + p[k] : safe;
             }
 #pragma unroll
             for (int k = 0; k < DENSE_WIN; ++k) u[k] = adj[sl[k]];
