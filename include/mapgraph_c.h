@@ -309,7 +309,10 @@ typedef struct mg_server_config {
                                  query inside one launch (0 = 256; -1 = never)          */
     int32_t thin_lanes;       /* concurrent thin queries per (graph, kind) (0 = 32)      */
     int32_t thin_streams;     /* thin launches in flight per (graph, kind), 1..8 (0 = 2) */
-    int32_t reserved[4];
+    int32_t thin_slice_us;    /* a thin search pauses at a level boundary after this many
+                                 microseconds in a launch and continues in a later one
+                                 (0 = 2000; -1 = runs to the end in one launch)          */
+    int32_t reserved[3];
 } mg_server_config;
 
 typedef struct mg_query_info {

@@ -203,8 +203,10 @@ constexpr int PPR_SLOTS = 64;  // slots per context: one per lane
 struct PprDev {
     int n = 0, ntasks = 0, nbig = 0, npart = 0;
     int cur = 0;                          // X[cur] holds the current ranks
+    int width = 8;                        // row width W: 8, 16, 32 or 64 columns
+    uint64_t live = 0;                    // slots in use
     int64_t bytes = 0;
-    float *X[2] = {nullptr, nullptr};     // ranks, n x PPR_SLOTS
+    float *X[2] = {nullptr, nullptr};     // ranks, n x W (room for n x PPR_SLOTS)
     float *C = nullptr;                   // contributions
     float *part = nullptr;                // partial sums of long in-lists
     float *rows = nullptr;                // PPR_SLOTS x n results

@@ -130,6 +130,7 @@ struct Workspace {
 struct mg_graph {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool stream_borrowed = false;  // a twin runs on its parent's stream
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int32_t n = 0;
     int64_t m = 0;

@@ -381,6 +381,13 @@ int mg_graph_create(const int32_t *row_offsets, const int32_t *col_idx, const in
                                   &t)))
             return fail(rc);
         g->twin = t;
+        // the twin is searched only under this graph's lock: it runs on this graph's
+        // stream (each stream holds a hardware queue while it exists)
+        if (hipStreamSynchronize(t->stream) != hipSuccess)
+            return fail(set_error(MG_ERR_DEVICE, "twin creation did not complete"));
+        (void)hipStreamDestroy(t->stream);
+        t->stream = g->stream;
+        t->stream_borrowed = true;
         if ((rc = dalloc(&t->perm, (size_t)n)) || (rc = dalloc(&t->unperm, (size_t)n))) return fail(rc);
         if (copy_on(t->stream, t->perm, perm.data(), 4 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
             return fail(set_error(MG_ERR_DEVICE, "host-to-device copy of the vertex renaming failed"));
@@ -488,7 +495,7 @@ int mg_graph_destroy(mg_graph *g) {
     dfree(g->csc_eid);
     if (g->ev0) (void)hipEventDestroy(g->ev0);
     if (g->ev1) (void)hipEventDestroy(g->ev1);
-    if (g->stream) (void)hipStreamDestroy(g->stream);
+    if (g->stream && !g->stream_borrowed) (void)hipStreamDestroy(g->stream);
     delete g;
     return MG_OK;
 }

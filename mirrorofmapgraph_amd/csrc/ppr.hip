@@ -5,15 +5,16 @@
 // conventions: contribution r[v] / outdeg(v) gathered over in-edges, rank-sink
 // (dangling) mass dropped, teleport t applied as written there (t + (1 - t) * sum):
 //   x_0 = e_s,   x_{i+1}[u] = (1 - t) * sum_{v -> u} x_i[v] / outdeg(v)  (+ t when u = s).
-// Ranks are vertex-major, X[v * 64 + slot] (256 bytes per vertex), so one wave
-// gathers one in-edge for all slots (lane = slot).  Slots are handed out lowest
-// first, so while at most 32 are in use only the first 128-byte line of a row is read
-// and written.
+// Ranks are vertex-major, X[v * W + slot], W = 8, 16, 32 or 64 floats per vertex: the
+// smallest width above every slot in use (slots are handed out lowest first; the rows
+// are repacked when it changes), so a wave gathers one in-edge's row for 64 / W edges
+// at a time and a row is one 32-byte sector while at most 8 slots are in use.
 // Vertices are covered by static tasks: runs of consecutive vertices with at most
-// PPR_T in-edges in total (one wave walks them in order), and PPR_T-edge pieces
-// of longer in-lists whose partial sums are added in piece order.  Every slot's
-// arithmetic is thus fixed by the graph alone: a query gives the same bits alone
-// or beside any other queries.
+// PPR_T in-edges in total (one wave walks them), and PPR_T-edge pieces of longer
+// in-lists whose partial sums are added in piece order.  Every list is summed in one
+// order fixed by the list alone, whatever the width (see gather_sum), so a slot's
+// arithmetic is fixed by the graph: a query gives the same bits alone or beside any
+// other queries.
 #include "batch.h"
 
 #include <vector>
@@ -26,10 +27,10 @@ constexpr int PPR_RUN = 64;    // vertices per run task
 struct PprArgs {
     const int *in_off, *in_adj;  // in-edges (CSR of a symmetric graph, or the CSC)
     const int *out_off;          // out-degrees
-    const float *X;              // current ranks, n x PPR_SLOTS
+    const float *X;              // current ranks, n x W
     float *Xn;                   // next ranks
     float *C;                    // contributions X / outdeg
-    float *part;                 // piece partial sums, npart x PPR_SLOTS
+    float *part;                 // piece partial sums, npart x W
     const ChunkRec *tasks;       // runs: {v0, -, -, count}; pieces: {v, beg, end, -(part + 1)}
     const int *big;              // per long list: vertex, first part, parts (3 ints)
     const int *src;              // per slot: source or -1
@@ -37,46 +38,102 @@ struct PprArgs {
     float t;                     // teleport
 };
 
+// Row width: ranks, contributions and partial sums are stored n x W, W the smallest
+// of 8, 16, 32, 64 above every slot in use (slots are handed out lowest first), so a
+// gathered row is W floats: 32 bytes (one sector) while at most 8 slots are in use,
+// and the contribution array of a 4M-vertex graph is 128 MB instead of 1 GB.
+__host__ __device__ constexpr int ppr_groups(int W) { return WAVE / W; }
+
+// C[v][c] = X[v][c] / outdeg(v) (0 for dangling v), over all n x W entries.
+template <int W>
 __global__ __launch_bounds__(BLK) void ppr_contrib(PprArgs a) {
-    const int lane = lane_id();
-    const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
-    const bool on = a.src[lane] >= 0;
-    if (!__any(on)) return;
-    for (int v = wave; v < a.n; v += nwaves) {
+    const int64_t total = (int64_t)a.n * W;
+    for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < total; i += (int64_t)gridDim.x * BLK) {
+        const int v = (int)(i / W);
         const int d = a.out_off[v + 1] - a.out_off[v];
-        if (on) a.C[(int64_t)v * PPR_SLOTS + lane] = d > 0 ? a.X[(int64_t)v * PPR_SLOTS + lane] / (float)d : 0.f;
+        a.C[i] = d > 0 ? a.X[i] / (float)d : 0.f;
     }
 }
 
-// Sum of C over in-edges [beg, end) in order, four loads in flight.
-// Sum over in-edges [beg, end) of the lane's column, in edge order: full groups of
-// PPR_U rows with every load in flight (the group test is wave-uniform, so the
-// loads need no per-load branch), then the remainder one row at a time.
-// `col` is the column the lane reads: its own slot, or, when no slot of the upper half
-// of the row is in use (slots are handed out lowest first), the lower half's column
-// of the same cache line, so a row costs one 128-byte line instead of two.
-constexpr int PPR_U = 8;
-__device__ __forceinline__ float gather_sum(const PprArgs &a, int beg, int end, int col, bool on) {
-    float acc = 0.f;
-    int e = beg;
-    for (; e + PPR_U <= end; e += PPR_U) {
-        int v[PPR_U];
-        float c[PPR_U];
+// Sum of the lane's column over in-edges [beg, end), in one order for every width:
+// the edge at offset k of the list goes to partial sum k mod 8 (each partial sum adds
+// its edges in list order), and the eight partial sums are combined as
+// ((s0 + s1) + (s2 + s3)) + ((s4 + s5) + (s6 + s7)).  A wave holds G = 64 / W edge
+// groups of W lanes (lane = g * W + column): group g owns partial sums g + G j
+// (j < 8 / G), so a wave instruction gathers G rows; the combination steps over
+// partial sums on different groups are lane butterflies (xor W, 2W, 4W), the others
+// adds inside the lane, and addition is commutative, so every width and every lane of
+// a pair computes the same bits.  Eight gathers per lane are in flight per step
+// (G rounds of eight edges); the last step loads from clamped slots and adds only
+// the edges in range.
+template <int W>
+__device__ __forceinline__ float gather_sum(const PprArgs &a, int beg, int end, int g, int col) {
+    constexpr int G = ppr_groups(W), J = 8 / G;
+    float acc[J];
 #pragma unroll
-        for (int j = 0; j < PPR_U; ++j) v[j] = a.in_adj[e + j];
+    for (int j = 0; j < J; ++j) acc[j] = 0.f;
+    for (int e0 = beg; e0 < end; e0 += 8 * G) {
+        // round r of the step covers edges e0 + 8 r .. e0 + 8 r + 7; this lane's edge
+        // of partial sum g + G j in round r is e0 + 8 r + g + G j
+        int u[G][J];
+        bool ok[G][J];
 #pragma unroll
-        for (int j = 0; j < PPR_U; ++j) c[j] = a.C[(int64_t)v[j] * PPR_SLOTS + col];
+        for (int r = 0; r < G; ++r)
 #pragma unroll
-        for (int j = 0; j < PPR_U; ++j) acc += c[j];
+            for (int j = 0; j < J; ++j) {
+                const int e = e0 + 8 * r + g + G * j;
+                ok[r][j] = e < end;
+                u[r][j] = a.in_adj[ok[r][j] ? e : beg];
+            }
+        float c[G][J];
+#pragma unroll
+        for (int r = 0; r < G; ++r)
+#pragma unroll
+            for (int j = 0; j < J; ++j) c[r][j] = a.C[(int64_t)u[r][j] * W + col];
+#pragma unroll
+        for (int r = 0; r < G; ++r)
+#pragma unroll
+            for (int j = 0; j < J; ++j)
+                if (ok[r][j]) acc[j] += c[r][j];
     }
-    for (; e < end; ++e) acc += a.C[(int64_t)a.in_adj[e] * PPR_SLOTS + col];
-    return on ? acc : 0.f;
+    // combine: level b pairs partial sums whose index differs in bit b
+    float s[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) s[j] = acc[j];
+    int nj = J;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        if ((1 << b) < G) {  // partner on group g ^ (1 << b)
+#pragma unroll
+            for (int j = 0; j < J; ++j) s[j] += __shfl_xor(s[j], W << b);
+        } else {  // partner in this lane: j ^ (1 << (b - log2 G))
+            const int h = nj / 2;
+#pragma unroll
+            for (int j = 0; j < J / 2; ++j)
+                if (j < h) s[j] = s[2 * j] + s[2 * j + 1];
+            nj = h;
+        }
+    }
+    return s[0];
 }
 
-// The column lane `lane` reads: its own, unless the row's upper half is idle.
-__device__ __forceinline__ int ppr_col(int lane, bool on) {
-    const bool hi = __any(on && lane >= PPR_SLOTS / 2);
-    return hi ? lane : (lane & (PPR_SLOTS / 2 - 1));
+// The same sum for a list of at most 8 edges, by one group alone: edge k is partial
+// sum k (the others are 0), combined in the same tree inside the lane.  Slots past
+// the end load from beg, which is in range only for a non-empty list (an empty list
+// at the end of the array has beg = m): an empty list loads nothing and sums to 0,
+// as the tree of eight zero partial sums does.
+template <int W>
+__device__ __forceinline__ float short_sum(const PprArgs &a, int beg, int end, int col) {
+    if (end <= beg) return 0.f;
+    int u[8];
+    float c[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) u[k] = a.in_adj[beg + k < end ? beg + k : beg];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[k] = a.C[(int64_t)u[k] * W + col];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[k] = beg + k < end ? 0.f + c[k] : 0.f;
+    return ((c[0] + c[1]) + (c[2] + c[3])) + ((c[4] + c[5]) + (c[6] + c[7]));
 }
 
 __device__ __forceinline__ float ppr_apply(float s, float t, bool is_src) {
@@ -84,45 +141,78 @@ __device__ __forceinline__ float ppr_apply(float s, float t, bool is_src) {
     return is_src ? t + v : v;
 }
 
+template <int W>
 __global__ __launch_bounds__(BLK) void ppr_gather(PprArgs a) {
-    const int lane = lane_id();
+    constexpr int G = ppr_groups(W);
+    const int lane = lane_id(), g = lane / W, col = lane % W;
     const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
-    const int src = a.src[lane];
+    const int src = a.src[col];
     const bool on = src >= 0;
     if (!__any(on)) return;
-    const int col = ppr_col(lane, on);
     for (int t = wave; t < a.ntasks; t += nwaves) {
         const ChunkRec r = a.tasks[t];
         if (r.pad > 0) {
-            for (int u = r.v; u < r.v + r.pad; ++u) {
-                const float s = gather_sum(a, a.in_off[u], a.in_off[u + 1], col, on);
-                if (on) a.Xn[(int64_t)u * PPR_SLOTS + lane] = ppr_apply(s, a.t, u == src);  // idle columns unused
+            // G vertices of the run at a time: lists of at most 8 edges are summed by
+            // their group alone, longer ones by the whole wave, one after another
+            // (idle columns unused)
+            for (int u0 = r.v; u0 < r.v + r.pad; u0 += G) {
+                const int u = u0 + g;
+                const bool has = u < r.v + r.pad;
+                const int b = has ? a.in_off[u] : 0, e = has ? a.in_off[u + 1] : 0;
+                const bool shortl = has && e - b <= 8;
+                if (__any(shortl) && shortl) {
+                    const float s = short_sum<W>(a, b, e, col);
+                    if (on) a.Xn[(int64_t)u * W + col] = ppr_apply(s, a.t, u == src);
+                }
+                uint64_t longs = wave_ballot(has && !shortl && col == 0);  // bit g * W per long list
+                while (longs) {
+                    const int gl = (__ffsll((unsigned long long)longs) - 1) / W;
+                    longs &= longs - 1;
+                    const int ul = u0 + gl;
+                    const float s = gather_sum<W>(a, a.in_off[ul], a.in_off[ul + 1], g, col);
+                    if (on && g == 0) a.Xn[(int64_t)ul * W + col] = ppr_apply(s, a.t, ul == src);
+                }
             }
         } else {
-            const float s = gather_sum(a, r.beg, r.end, col, on);
-            if (on) a.part[(int64_t)(-r.pad - 1) * PPR_SLOTS + lane] = s;
+            const float s = gather_sum<W>(a, r.beg, r.end, g, col);
+            if (on && g == 0) a.part[(int64_t)(-r.pad - 1) * W + col] = s;
         }
     }
 }
 
+// Long in-lists: the pieces' partial sums added in piece order (G lists per wave).
+template <int W>
 __global__ __launch_bounds__(BLK) void ppr_combine(PprArgs a) {
-    const int lane = lane_id();
+    constexpr int G = ppr_groups(W);
+    const int lane = lane_id(), g = lane / W, col = lane % W;
     const int wave = (blockIdx.x * BLK + (int)threadIdx.x) / WAVE, nwaves = gridDim.x * BLK / WAVE;
-    const int src = a.src[lane];
+    const int src = a.src[col];
     const bool on = src >= 0;
-    for (int b = wave; b < a.nbig; b += nwaves) {
+    for (int b0 = wave * G; b0 < a.nbig; b0 += nwaves * G) {
+        const int b = b0 + g;
+        if (b >= a.nbig) continue;
         const int u = a.big[3 * b], p0 = a.big[3 * b + 1], np = a.big[3 * b + 2];
         float s = 0.f;
-        for (int p = p0; p < p0 + np; ++p) s += on ? a.part[(int64_t)p * PPR_SLOTS + lane] : 0.f;
-        if (on) a.Xn[(int64_t)u * PPR_SLOTS + lane] = ppr_apply(s, a.t, u == src);
+        for (int p = p0; p < p0 + np; ++p) s += on ? a.part[(int64_t)p * W + col] : 0.f;
+        if (on) a.Xn[(int64_t)u * W + col] = ppr_apply(s, a.t, u == src);
+    }
+}
+
+// Rows of width w1 -> rows of width w2 (columns beyond w1 start at 0).
+__global__ void ppr_repack(const float *X, int w1, float *Y, int w2, int n) {
+    const int64_t total = (int64_t)n * w2;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = i / w2;
+        const int c = (int)(i % w2);
+        Y[i] = c < w1 ? X[v * w1 + c] : 0.f;
     }
 }
 
 // Column reset of the listed slots: x_0 = e_src.
-__global__ void ppr_seed_cols(float *X, int n, SeedList sl) {
+__global__ void ppr_seed_cols(float *X, int n, int W, SeedList sl) {
     const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, nt = (int64_t)gridDim.x * blockDim.x;
     for (int64_t u = tid; u < n; u += nt)
-        for (int j = 0; j < sl.n; ++j) X[u * PPR_SLOTS + sl.slot[j]] = u == sl.src[j] ? 1.f : 0.f;
+        for (int j = 0; j < sl.n; ++j) X[u * W + sl.slot[j]] = u == sl.src[j] ? 1.f : 0.f;
 }
 
 __global__ void ppr_set_src(int *src, SeedList sl, SlotList off) {
@@ -132,15 +222,15 @@ __global__ void ppr_set_src(int *src, SeedList sl, SlotList off) {
     if (i < sl.n) src[sl.slot[i]] = sl.src[i];
 }
 
-// Rank columns of the listed slots -> their rows (64 x 64 LDS tile).
-__global__ __launch_bounds__(BLK) void ppr_extract(const float *X, float *rows, int n, SlotList sl) {
+// Rank columns of the listed slots -> their rows (64 vertices x W columns through LDS).
+__global__ __launch_bounds__(BLK) void ppr_extract(const float *X, int W, float *rows, int n, SlotList sl) {
     __shared__ float tile[64][65];
     const int tv = (n + 63) / 64;
     for (int t = blockIdx.x; t < tv; t += gridDim.x) {
         const int v0 = t * 64;
-        for (int i = threadIdx.x; i < 64 * 64; i += BLK) {
-            const int r = i / 64, cc = i % 64;
-            tile[r][cc] = (v0 + r < n) ? X[(int64_t)(v0 + r) * PPR_SLOTS + cc] : 0.f;
+        for (int i = threadIdx.x; i < 64 * W; i += BLK) {
+            const int r = i / W, cc = i % W;
+            tile[r][cc] = (v0 + r < n) ? X[(int64_t)(v0 + r) * W + cc] : 0.f;
         }
         __syncthreads();
         for (int i = threadIdx.x; i < sl.n * 64; i += BLK) {
@@ -255,6 +345,12 @@ int ppr_dev_alloc(PprDev &p, mg_graph *g) {
     return MG_OK;
 }
 
+// The row width for the slots in `live`.
+static int ppr_width(uint64_t live) {
+    const int need = live ? 64 - __builtin_clzll(live) : 1;
+    return need <= 8 ? 8 : need <= 16 ? 16 : need <= 32 ? 32 : 64;
+}
+
 static PprArgs ppr_args(PprDev &p, mg_graph *g, float t) {
     PprArgs a{};
     ppr_view(g, &a.in_off, &a.in_adj);
@@ -274,30 +370,54 @@ static PprArgs ppr_args(PprDev &p, mg_graph *g, float t) {
 }
 
 int ppr_seed(PprDev &p, hipStream_t st, const SeedList &sl, const SlotList &retire) {
+    uint64_t live = p.live;
+    for (int i = 0; i < retire.n; ++i) live &= ~(1ull << retire.slot[i]);
+    for (int i = 0; i < sl.n; ++i) live |= 1ull << sl.slot[i];
+    const int w = ppr_width(live);
+    if (w != p.width) {  // rows of the new width into the other buffer
+        hipLaunchKernelGGL(ppr_repack, dim3(grid_blocks_cap((int64_t)p.n * w, BLK, 4 * MAX_GRID)), dim3(BLK), 0, st,
+                           (const float *)p.X[p.cur], p.width, p.X[p.cur ^ 1], w, p.n);
+        p.cur ^= 1;
+        p.width = w;
+    }
+    p.live = live;
     if (sl.n > 0)
-        hipLaunchKernelGGL(ppr_seed_cols, dim3(grid_blocks(p.n, BLK)), dim3(BLK), 0, st, p.X[p.cur], p.n, sl);
+        hipLaunchKernelGGL(ppr_seed_cols, dim3(grid_blocks(p.n, BLK)), dim3(BLK), 0, st, p.X[p.cur], p.n, p.width, sl);
     hipLaunchKernelGGL(ppr_set_src, dim3(1), dim3(PPR_SLOTS), 0, st, p.src, sl, retire);
     MG_HIP(hipGetLastError());
     return MG_OK;
 }
 
-int ppr_iterate(PprDev &p, mg_graph *g, hipStream_t st, float t, int iters) {
+template <int W>
+static int ppr_iterate_w(PprDev &p, mg_graph *g, hipStream_t st, float t, int iters) {
     for (int i = 0; i < iters; ++i) {
         PprArgs a = ppr_args(p, g, t);
         const int G = grid_blocks(std::max<int64_t>((int64_t)p.ntasks * WAVE, (int64_t)p.n * 4), BLK);
-        hipLaunchKernelGGL(ppr_contrib, dim3(G), dim3(BLK), 0, st, a);
-        hipLaunchKernelGGL(ppr_gather, dim3(G), dim3(BLK), 0, st, a);
-        if (p.nbig) hipLaunchKernelGGL(ppr_combine, dim3(grid_blocks((int64_t)p.nbig * WAVE, BLK)), dim3(BLK), 0, st, a);
+        hipLaunchKernelGGL(ppr_contrib<W>, dim3(grid_blocks_cap((int64_t)p.n * W, BLK, 4 * MAX_GRID)), dim3(BLK), 0,
+                           st, a);
+        hipLaunchKernelGGL(ppr_gather<W>, dim3(G), dim3(BLK), 0, st, a);
+        if (p.nbig)
+            hipLaunchKernelGGL(ppr_combine<W>, dim3(grid_blocks((int64_t)p.nbig * WAVE / ppr_groups(W), BLK) + 1),
+                               dim3(BLK), 0, st, a);
         MG_HIP(hipGetLastError());
         p.cur ^= 1;
     }
     return MG_OK;
 }
 
+int ppr_iterate(PprDev &p, mg_graph *g, hipStream_t st, float t, int iters) {
+    switch (p.width) {
+    case 8: return ppr_iterate_w<8>(p, g, st, t, iters);
+    case 16: return ppr_iterate_w<16>(p, g, st, t, iters);
+    case 32: return ppr_iterate_w<32>(p, g, st, t, iters);
+    default: return ppr_iterate_w<64>(p, g, st, t, iters);
+    }
+}
+
 int ppr_extract_rows(PprDev &p, hipStream_t st, const SlotList &sl) {
     if (sl.n <= 0) return MG_OK;
     hipLaunchKernelGGL(ppr_extract, dim3(grid_blocks((int64_t)((p.n + 63) / 64) * BLK, BLK)), dim3(BLK), 0, st,
-                       (const float *)p.X[p.cur], p.rows, p.n, sl);
+                       (const float *)p.X[p.cur], p.width, p.rows, p.n, sl);
     MG_HIP(hipGetLastError());
     return MG_OK;
 }

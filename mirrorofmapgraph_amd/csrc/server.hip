@@ -14,7 +14,10 @@
 //   3. runs a chunk of steps (a captured HIP graph of STEPS_PER_GRAPH levels; a few
 //      PageRank iterations) and copies the slot status back;
 // and ends with an event the dispatcher polls, so no round waits for another
-// engine.  An engine's device state is allocated when it first has work, within
+// engine.  Status blocks and result rows leave through copy kernels on the engine's
+// own stream, not DMA copies: a DMA copy queued behind a long kernel waits inside a
+// DMA queue that every stream's copies share, so one thin search in flight (tens of
+// milliseconds) held up every engine's status copy behind it.  An engine's device state is allocated when it first has work, within
 // the server's HBM budget (idle engines give theirs back when another needs it;
 // an engine that cannot fit even one slot leaves its queries waiting).  Results do
 // not depend on what else runs: BFS depths and SSSP distances are unique, and a
@@ -36,6 +39,7 @@ struct Query {
     int64_t id = 0;
     int graph = 0, kind = 0, src = 0;
     void *out = nullptr;
+    void *out_dev = nullptr;  // out as the device addresses it (null: host memory the device cannot write)
     bool out_on_device = false;
     double t_submit = 0, t_admit = 0, t_done = 0;
     int status = MG_OK;
@@ -64,6 +68,8 @@ struct Engine {
     bool allocated = false, inflight = false;
     int S = 0;                              // slots
     BatchDev b;                             // BFS / SSSP
+    Ctl *h_ctl_dev = nullptr;               // b.h_ctl / b.h_sc as the device addresses them
+    SlotCtl *h_sc_dev = nullptr;
     PprDev p;                               // PageRank
     hipGraphExec_t chunk = nullptr;         // captured steps
     long long delta = 1;                    // SSSP bucket width
@@ -77,11 +83,13 @@ struct Engine {
     ThinDev td;
     std::vector<ThinSlot> tslots;
     std::vector<int> free_lanes;
+    std::deque<Query *> paused;             // thin searches waiting to be launched again
     int64_t bytes() const { return allocated ? (kind == MG_Q_PPR ? p.bytes : thin ? td.bytes : b.bytes) : 0; }
 };
 
 struct Server {
     mg_server_config cfg{};
+    long long thin_slice = 0;  // wall-clock ticks of a thin launch's slice (0: none)
     std::vector<mg_graph *> graphs;
     std::vector<std::unique_ptr<Engine>> engines;  // graph * 3 + kind
     std::unordered_map<int64_t, std::unique_ptr<Query>> queries;
@@ -100,6 +108,54 @@ struct Server {
     }
 };
 
+// dst[i] = src[i]: device memory into device memory or pinned host memory.
+__global__ void copy_out_v4(const uint4 *src, uint4 *dst, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+__global__ void copy_out_w(const uint32_t *src, uint32_t *dst, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// Queues a kernel copy of `bytes` (a multiple of 4) on st; dst must be addressable by
+// the device (device memory, or pinned host memory through its device address).
+static int copy_out(void *dst, const void *src, size_t bytes, hipStream_t st) {
+    if (bytes == 0) return MG_OK;
+    if (((uintptr_t)dst | (uintptr_t)src | bytes) % 16 == 0) {
+        const int64_t n = (int64_t)(bytes / 16);
+        hipLaunchKernelGGL(copy_out_v4, dim3(grid_blocks_cap(n, BLK, 512)), dim3(BLK), 0, st, (const uint4 *)src,
+                           (uint4 *)dst, n);
+    } else {
+        const int64_t n = (int64_t)(bytes / 4);
+        hipLaunchKernelGGL(copy_out_w, dim3(grid_blocks_cap(n, BLK, 512)), dim3(BLK), 0, st, (const uint32_t *)src,
+                           (uint32_t *)dst, n);
+    }
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+// A finished query's row into its output: a kernel copy where the device can write
+// the output, else a runtime copy (host memory the runtime stages; it blocks the
+// dispatcher while it copies, as documented for mg_server_submit).
+static int deliver_row(const Query *r, const void *row, size_t bytes, hipStream_t st) {
+    if (r->out_dev) return copy_out(r->out_dev, row, bytes, st);
+    if (r->out) MG_HIP(hipMemcpyAsync(r->out, row, bytes, hipMemcpyDeviceToHost, st));
+    return MG_OK;
+}
+
+// The device address of out: itself for device memory, the mapped address of pinned
+// host memory, null for host memory the device cannot write.
+static void *device_view(void *out, bool on_device) {
+    if (!out || on_device) return out;
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, out) != hipSuccess) {
+        (void)hipGetLastError();  // not memory the runtime knows: pageable host memory
+        return nullptr;
+    }
+    return at.type == hipMemoryTypeHost ? at.devicePointer : nullptr;
+}
+
 static int engine_steps(Engine &e) {
     return e.srv->cfg.steps_per_round > 0 ? e.srv->cfg.steps_per_round : STEPS_PER_GRAPH;
 }
@@ -112,6 +168,8 @@ static void engine_free(Server &s, Engine &e) {
     if (e.kind == MG_Q_PPR) ppr_dev_free(e.p);
     else if (e.thin) thin_dev_free(e.td);
     else batch_dev_free(e.b);
+    e.h_ctl_dev = nullptr;
+    e.h_sc_dev = nullptr;
     e.free_lanes.clear();
     e.allocated = false;
     e.S = 0;
@@ -163,6 +221,11 @@ static int engine_alloc(Server &s, Engine &e) {
         rc = ppr_dev_alloc(e.p, e.g);
     } else {
         rc = batch_dev_alloc(e.b, e.g, e.kind == MG_Q_SSSP, S);
+        if (rc == MG_OK && (hipHostGetDevicePointer((void **)&e.h_ctl_dev, e.b.h_ctl, 0) != hipSuccess ||
+                            hipHostGetDevicePointer((void **)&e.h_sc_dev, e.b.h_sc, 0) != hipSuccess)) {
+            batch_dev_free(e.b);
+            return set_error(MG_ERR_DEVICE, "pinned status blocks are not mapped for the device");
+        }
     }
     if (rc) return rc;
     e.S = S;
@@ -220,20 +283,30 @@ static int thin_round(Server &s, Engine &e, bool *queued) {
         ts.busy = false;
         const double t = s.now();
         for (Query *r : ts.qs) {
-            r->te = e.td.h_stats[2 * r->slot];
-            r->nr = e.td.h_stats[2 * r->slot + 1];
+            const long long *hs = e.td.h_stats + THIN_STAT * r->slot;
+            if (hs[2] == 0) {  // paused at the end of its slice: launched again next
+                e.paused.push_back(r);
+                continue;
+            }
+            if (r->out && !r->out_dev) {  // host memory the device cannot write: a runtime copy
+                MG_HIP(hipMemcpyAsync(r->out, e.td.a.rows + (size_t)r->slot * e.g->n, 4 * (size_t)e.g->n,
+                                      hipMemcpyDeviceToHost, ts.st));
+                MG_HIP(hipStreamSynchronize(ts.st));
+            }
+            r->te = hs[0];
+            r->nr = hs[1];
             e.free_lanes.push_back(r->slot);
             r->slot = -1;
             finish_query(s, r, t);
             e.running--;
+            any_done = true;
         }
         ts.qs.clear();
-        any_done = true;
     }
     if (any_done) s.cv_done.notify_all();
     e.inflight = false;
     for (const ThinSlot &ts : e.tslots) e.inflight |= ts.busy;
-    if (e.waiting.empty()) {
+    if (e.waiting.empty() && e.paused.empty()) {
         if (e.running == 0 && e.idle_since == 0) e.idle_since = s.now();
         return MG_OK;
     }
@@ -246,10 +319,19 @@ static int thin_round(Server &s, Engine &e, bool *queued) {
         }
         if (rc) return rc;
     }
-    const size_t n = (size_t)e.g->n;
     for (ThinSlot &ts : e.tslots) {
-        if (ts.busy || e.waiting.empty() || e.free_lanes.empty()) continue;
+        if (ts.busy || (e.paused.empty() && (e.waiting.empty() || e.free_lanes.empty()))) continue;
         ThinBatch jb{};
+        jb.slice = s.thin_slice;
+        // paused searches first (oldest first), then new queries on free lanes
+        while (jb.n < THIN_MAX_JOBS && !e.paused.empty()) {
+            Query *r = e.paused.front();
+            e.paused.pop_front();
+            jb.lane[jb.n] = r->slot;
+            jb.src[jb.n] = -1;
+            jb.out[jb.n++] = r->out_dev;
+            ts.qs.push_back(r);
+        }
         while (jb.n < THIN_MAX_JOBS && !e.waiting.empty() && !e.free_lanes.empty()) {
             Query *r = e.waiting.front();
             e.waiting.pop_front();
@@ -257,16 +339,13 @@ static int thin_round(Server &s, Engine &e, bool *queued) {
             e.free_lanes.pop_back();
             r->t_admit = s.now();
             jb.lane[jb.n] = r->slot;
-            jb.src[jb.n++] = r->src;
+            jb.src[jb.n] = r->src;
+            jb.out[jb.n++] = r->out_dev;
             ts.qs.push_back(r);
             e.running++;
         }
-        int rc = thin_launch(e.td.a, jb, e.kind == MG_Q_SSSP, ts.st);
+        const int rc = thin_launch(e.td.a, jb, e.kind == MG_Q_SSSP, ts.st);
         if (rc) return rc;
-        for (Query *r : ts.qs)
-            if (r->out)
-                MG_HIP(hipMemcpyAsync(r->out, e.td.a.rows + (size_t)r->slot * n, 4 * n,
-                                      r->out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ts.st));
         MG_HIP(hipEventRecord(ts.ev, ts.st));
         ts.busy = true;
         e.inflight = true;
@@ -352,9 +431,7 @@ static int engine_round(Server &s, Engine &e, bool *queued) {
             const size_t bytes = 4 * (size_t)e.g->n;
             const void *row = e.kind == MG_Q_PPR ? (const void *)(e.p.rows + (size_t)sl * e.g->n)
                                                  : (const void *)(e.b.rows + (size_t)sl * e.g->n);
-            if (r->out)
-                MG_HIP(hipMemcpyAsync(r->out, row, bytes,
-                                      r->out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, e.st));
+            if ((rc = deliver_row(r, row, bytes, e.st))) return rc;
             if (e.kind == MG_Q_PPR) {
                 r->te = e.g->m * (int64_t)(s.cfg.ppr_iters);
                 r->nr = e.g->n;
@@ -408,8 +485,8 @@ static int engine_round(Server &s, Engine &e, bool *queued) {
         }
     }
     if (e.kind != MG_Q_PPR) {
-        MG_HIP(hipMemcpyAsync(e.b.h_ctl, e.b.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e.st));
-        MG_HIP(hipMemcpyAsync(e.b.h_sc, e.b.sc, sizeof(SlotCtl), hipMemcpyDeviceToHost, e.st));
+        if ((rc = copy_out(e.h_ctl_dev, e.b.ctl, sizeof(Ctl), e.st))) return rc;
+        if ((rc = copy_out(e.h_sc_dev, e.b.sc, sizeof(SlotCtl), e.st))) return rc;
     }
     MG_HIP(hipEventRecord(e.ev, e.st));
     e.inflight = true;
@@ -450,6 +527,11 @@ static void dispatcher(Server *sp) {
                     }
                     ts.qs.clear();
                 }
+                for (Query *r : ep->paused) {
+                    r->status = rc;
+                    finish_query(s, r, s.now());
+                }
+                ep->paused.clear();
                 ep->running = 0;
                 s.cv_done.notify_all();
             }
@@ -492,13 +574,23 @@ int mg_server_create(const mg_server_config *cfg, mg_server **out) {
     if (cfg) s->cfg = *cfg;
     else mg_server_config_default(&s->cfg);
     if (s->cfg.hbm_budget <= 0 || s->cfg.max_slots < 1 || s->cfg.max_slots > MG_BATCH_MAX || s->cfg.ppr_iters < 1 ||
-        s->cfg.steps_per_round < 1 || s->cfg.steps_per_round > 64 || s->cfg.thin_streams < 0 || s->cfg.thin_streams > 8) {
+        s->cfg.steps_per_round < 1 || s->cfg.steps_per_round > 64 || s->cfg.thin_streams < 0 || s->cfg.thin_streams > 8 ||
+        s->cfg.thin_slice_us < -1) {
         delete s;
         return set_error(MG_ERR_INVALID_ARG, "invalid server configuration");
     }
     if (hipGetDevice(&s->device) != hipSuccess) {
         delete s;
         return set_error(MG_ERR_NO_DEVICE, "no HIP device");
+    }
+    if (s->cfg.thin_slice_us >= 0) {
+        int khz = 0;  // the wall clock that wall_clock64() reads in kernels
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->device) != hipSuccess || khz <= 0) {
+            delete s;
+            return set_error(MG_ERR_DEVICE, "device wall-clock rate unavailable");
+        }
+        const long long us = s->cfg.thin_slice_us > 0 ? s->cfg.thin_slice_us : 2000;
+        s->thin_slice = std::max(1LL, us * khz / 1000);
     }
     s->th = std::thread(dispatcher, static_cast<Server *>(s));
     *out = s;
@@ -545,11 +637,16 @@ int mg_server_add_graph(mg_server *s, mg_graph *g, int32_t *graph_id) {
             hipEventCreateWithFlags(&e->ev, hipEventDisableTiming) != hipSuccess)
             return set_error(MG_ERR_DEVICE, "stream/event creation failed");
         if (e->thin) {
+            // the first launch slot uses the engine's stream: every stream holds a
+            // hardware queue while it exists, and streams beyond the queues share them
             e->tslots.resize(thin_streams(*s));
-            for (ThinSlot &ts : e->tslots)
-                if (hipStreamCreateWithFlags(&ts.st, hipStreamNonBlocking) != hipSuccess ||
+            for (size_t i = 0; i < e->tslots.size(); ++i) {
+                ThinSlot &ts = e->tslots[i];
+                if (i == 0) ts.st = e->st;
+                if ((i > 0 && hipStreamCreateWithFlags(&ts.st, hipStreamNonBlocking) != hipSuccess) ||
                     hipEventCreateWithFlags(&ts.ev, hipEventDisableTiming) != hipSuccess)
                     return set_error(MG_ERR_DEVICE, "stream/event creation failed");
+            }
         }
         s->engines.push_back(std::move(e));
     }
@@ -577,6 +674,7 @@ int mg_server_submit(mg_server *s, int32_t graph_id, int32_t kind, int32_t src, 
     q->src = src;
     q->out = out;
     q->out_on_device = out_on_device != 0;
+    q->out_dev = device_view(out, q->out_on_device);
     q->t_submit = s->now();
     s->engines[(size_t)graph_id * 3 + kind]->waiting.push_back(q.get());
     *query_id = q->id;
@@ -657,7 +755,7 @@ int mg_server_destroy(mg_server *s) {
         if (e->st) (void)hipStreamDestroy(e->st);
         for (ThinSlot &ts : e->tslots) {
             if (ts.ev) (void)hipEventDestroy(ts.ev);
-            if (ts.st) (void)hipStreamDestroy(ts.st);
+            if (ts.st && ts.st != e->st) (void)hipStreamDestroy(ts.st);
         }
     }
     delete s;

@@ -10,6 +10,12 @@
 // state: the result row doubles as the depth / distance array, beside two frontier
 // queues, a far list and two bitmaps.  Concurrent queries are concurrent workgroups
 // (one CU each), so dozens of grid queries share the GPU with no per-level launch.
+// A launch is time-sliced: a search that has run `slice` wall-clock ticks pauses at
+// the next level boundary (its queues and bitmaps stay in its lane; four scalars are
+// saved) and the server launches it again with the next admitted queries, so no
+// launch holds its hardware queue (which other streams may share) for long, and new
+// queries start within a slice.  A finished search copies its row straight into the
+// query's output.
 //
 // Semantics are the plain searches': BFS depths (level-synchronous, a vertex takes
 // the level of its first claim; unreached -1) and SSSP distances (delta-stepping
@@ -108,23 +114,72 @@ __device__ void thin_row_stats(const ThinArgs &a, const int *row, int unreached,
     }
 }
 
+// The finished row into the query's output (device memory or pinned host memory),
+// 16 bytes a store where both sides allow it.
+__device__ void thin_copy_row(const int *row, int n, void *out) {
+    int *o = (int *)out;
+    int v0 = 0;
+    if ((((uintptr_t)o | (uintptr_t)row) & 15) == 0) {
+        const uint4 *r4 = (const uint4 *)row;
+        uint4 *o4 = (uint4 *)o;
+        for (int i = threadIdx.x; i < n / 4; i += THIN_BLK) o4[i] = r4[i];
+        v0 = n / 4 * 4;
+    }
+    for (int v = v0 + (int)threadIdx.x; v < n; v += THIN_BLK) o[v] = row[v];
+}
+
+// Slice check at a level boundary (thread 0 decides, the barrier after it publishes
+// the decision): a search pauses once it has run at least one level in this launch
+// and `slice` ticks have passed since the launch started.
+__device__ __forceinline__ void thin_slice_check(const ThinBatch &jobs, long long t_start, int levels, int *s_stop) {
+    if (threadIdx.x == 0)
+        *s_stop = jobs.slice > 0 && levels > 0 && (long long)wall_clock64() - t_start >= jobs.slice;
+}
+
+// Finish: the row's counts and the finished flag, then the row into the output.
+__device__ void thin_finish(const ThinArgs &a, const ThinBatch &jobs, const int *row, int unreached, int lane_no) {
+    thin_row_stats(a, row, unreached, a.stats + THIN_STAT * lane_no);
+    if (threadIdx.x == 0) a.stats[THIN_STAT * lane_no + 2] = 1;
+    if (jobs.out[blockIdx.x]) thin_copy_row(row, a.n, jobs.out[blockIdx.x]);
+}
+
 __global__ __launch_bounds__(THIN_BLK) void thin_bfs(ThinArgs a, ThinBatch jobs) {
     __shared__ int s_pre[THIN_BLK], s_beg[THIN_BLK], s_val[THIN_BLK], s_wave[THIN_BLK / WAVE];
-    __shared__ int s_n, s_cnt;
+    __shared__ int s_n, s_cnt, s_stop;
+    const long long t_start = wall_clock64();
     const int lane_no = jobs.lane[blockIdx.x], src = jobs.src[blockIdx.x];
     const size_t n = (size_t)a.n;
     int *row = a.rows + lane_no * n;
-    int *qa = a.q + lane_no * 2 * n, *qb = qa + n;
-    for (int v = threadIdx.x; v < a.n; v += THIN_BLK) row[v] = v == src ? 0 : -1;
-    if (threadIdx.x == 0) {
-        qa[0] = src;
-        s_n = 1;
+    int *q0 = a.q + lane_no * 2 * n, *q1 = q0 + n;
+    long long *st = a.state + 4 * (size_t)lane_no;
+    int par = 0, L = 0;
+    if (src >= 0) {
+        for (int v = threadIdx.x; v < a.n; v += THIN_BLK) row[v] = v == src ? 0 : -1;
+        if (threadIdx.x == 0) {
+            q0[0] = src;
+            s_n = 1;
+        }
+    } else {  // resume: queue parity, queue size, level
+        par = (int)st[0];
+        L = (int)st[2];
+        if (threadIdx.x == 0) s_n = (int)st[1];
     }
+    int *qa = par ? q1 : q0, *qb = par ? q0 : q1;
     __syncthreads();
-    for (int L = 0;; ++L) {
+    for (int done = 0;; ++L, ++done) {
         const int nq = s_n;
         if (nq == 0) break;
-        __syncthreads();  // every thread has read s_n
+        thin_slice_check(jobs, t_start, done, &s_stop);
+        __syncthreads();  // every thread has read s_n; s_stop is set
+        if (s_stop) {
+            if (threadIdx.x == 0) {
+                st[0] = par;
+                st[1] = nq;
+                st[2] = L;
+                a.stats[THIN_STAT * lane_no + 2] = 0;
+            }
+            return;
+        }
         if (threadIdx.x == 0) s_cnt = 0;
         for (int t0 = 0; t0 < nq; t0 += THIN_BLK)
             thin_tile<false>(a, qa, nq, t0, s_pre, s_beg, s_val, s_wave, [](int, int) { return 0; },
@@ -144,9 +199,10 @@ __global__ __launch_bounds__(THIN_BLK) void thin_bfs(ThinArgs a, ThinBatch jobs)
         int *t = qa;
         qa = qb;
         qb = t;
+        par ^= 1;
         __syncthreads();
     }
-    thin_row_stats(a, row, -1, a.stats + 2 * lane_no);
+    thin_finish(a, jobs, row, -1, lane_no);
 }
 
 // Delta-stepping: the near queue holds vertices improved below the bucket bound thr
@@ -156,31 +212,56 @@ __global__ __launch_bounds__(THIN_BLK) void thin_bfs(ThinArgs a, ThinBatch jobs)
 // entries that dropped below the old bound were relaxed as near vertices already.
 __global__ __launch_bounds__(THIN_BLK) void thin_sssp(ThinArgs a, ThinBatch jobs) {
     __shared__ int s_pre[THIN_BLK], s_beg[THIN_BLK], s_val[THIN_BLK], s_wave[THIN_BLK / WAVE];
-    __shared__ int s_n, s_cnt, s_nfar, s_kept;
+    __shared__ int s_n, s_cnt, s_nfar, s_kept, s_stop;
     __shared__ long long s_thr;
+    const long long t_start = wall_clock64();
     const int lane_no = jobs.lane[blockIdx.x], src = jobs.src[blockIdx.x];
     const size_t n = (size_t)a.n;
     int *row = a.rows + lane_no * n;
-    int *qa = a.q + lane_no * 2 * n, *qb = qa + n;
+    int *q0 = a.q + lane_no * 2 * n, *q1 = q0 + n;
     int *far = a.far + lane_no * n;
     uint32_t *inq = a.bits + lane_no * 2 * (size_t)a.nwords, *farbm = inq + a.nwords;
+    long long *st = a.state + 4 * (size_t)lane_no;
     const long long delta = a.delta;
-    for (int v = threadIdx.x; v < a.n; v += THIN_BLK) row[v] = v == src ? 0 : INT_MAX;
-    for (int i = threadIdx.x; i < 2 * a.nwords; i += THIN_BLK) inq[i] = 0;
-    if (threadIdx.x == 0) {
-        qa[0] = src;
-        s_n = 1;
-        s_nfar = 0;
-        s_thr = delta;
+    int par = 0;
+    if (src >= 0) {
+        for (int v = threadIdx.x; v < a.n; v += THIN_BLK) row[v] = v == src ? 0 : INT_MAX;
+        for (int i = threadIdx.x; i < 2 * a.nwords; i += THIN_BLK) inq[i] = 0;
+        if (threadIdx.x == 0) {
+            q0[0] = src;
+            s_n = 1;
+            s_nfar = 0;
+            s_thr = delta;
+        }
+    } else {  // resume: queue parity, near-queue size, far-list size, bucket bound
+        par = (int)st[0];
+        if (threadIdx.x == 0) {
+            s_n = (int)st[1];
+            s_nfar = (int)st[2];
+            s_thr = st[3];
+        }
     }
+    int *qa = par ? q1 : q0, *qb = par ? q0 : q1;
+    int done = 0;  // levels run in this launch
     __syncthreads();
     for (;;) {
         // relax the near queue until it empties
-        for (;;) {
+        for (;; ++done) {
             const int nq = s_n;
             if (nq == 0) break;
             const long long thr = s_thr;
-            __syncthreads();  // every thread has read s_n
+            thin_slice_check(jobs, t_start, done, &s_stop);
+            __syncthreads();  // every thread has read s_n and s_thr; s_stop is set
+            if (s_stop) {
+                if (threadIdx.x == 0) {
+                    st[0] = par;
+                    st[1] = nq;
+                    st[2] = s_nfar;
+                    st[3] = thr;
+                    a.stats[THIN_STAT * lane_no + 2] = 0;
+                }
+                return;
+            }
             if (threadIdx.x == 0) s_cnt = 0;
             for (int t0 = 0; t0 < nq; t0 += THIN_BLK)
                 thin_tile<true>(
@@ -214,6 +295,7 @@ __global__ __launch_bounds__(THIN_BLK) void thin_sssp(ThinArgs a, ThinBatch jobs
             int *t = qa;
             qa = qb;
             qb = t;
+            par ^= 1;
             __syncthreads();
         }
         // next bucket: the smallest waiting distance at or above the old bound
@@ -262,12 +344,12 @@ __global__ __launch_bounds__(THIN_BLK) void thin_sssp(ThinArgs a, ThinBatch jobs
         }
         __syncthreads();
     }
-    thin_row_stats(a, row, INT_MAX, a.stats + 2 * lane_no);
+    thin_finish(a, jobs, row, INT_MAX, lane_no);
 }
 
 int64_t thin_dev_bytes(const mg_graph *g, bool sssp, int lanes) {
     const int64_t n = g->n, nwords = ((g->n + 63) / 64) * 2;
-    const int64_t per = 4 * 3 * n + (sssp ? 4 * n + 8 * nwords : 0) + 16;
+    const int64_t per = 4 * 3 * n + (sssp ? 4 * n + 8 * nwords : 0) + 8 * THIN_STAT + 32;
     return per * lanes;
 }
 
@@ -276,6 +358,7 @@ void thin_dev_free(ThinDev &t) {
     free_dev(t.a.q);
     free_dev(t.a.far);
     free_dev(t.a.bits);
+    free_dev(t.a.state);
     if (t.h_stats) (void)hipHostFree(t.h_stats);  // a.stats is this pinned array
     t = ThinDev{};
 }
@@ -291,7 +374,8 @@ int thin_dev_alloc(ThinDev &t, mg_graph *g, bool sssp, int lanes, long long delt
     t.a.nwords = ((g->n + 63) / 64) * 2;
     t.a.delta = delta;
     int rc;
-    if ((rc = alloc_i32(&t.a.rows, lanes * n)) || (rc = alloc_i32(&t.a.q, lanes * 2 * n))) {
+    if ((rc = alloc_i32(&t.a.rows, lanes * n)) || (rc = alloc_i32(&t.a.q, lanes * 2 * n)) ||
+        (rc = alloc_i32((int32_t **)&t.a.state, lanes * 8))) {
         thin_dev_free(t);
         return rc;
     }
@@ -302,7 +386,7 @@ int thin_dev_alloc(ThinDev &t, mg_graph *g, bool sssp, int lanes, long long delt
     }
     // each workgroup writes its query's counts straight into pinned host memory; they
     // are read once the launch's event has completed
-    if (hipHostMalloc((void **)&t.h_stats, 16 * (size_t)lanes, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&t.h_stats, 8 * THIN_STAT * (size_t)lanes, hipHostMallocDefault) != hipSuccess) {
         thin_dev_free(t);
         return set_error(MG_ERR_OOM, "thin search state allocation failed");
     }

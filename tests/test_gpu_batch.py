@@ -238,7 +238,8 @@ def test_ppr_batch_against_float64(mg, oracle, device, symmetric):
 
 
 def test_ppr_batch_rows_do_not_depend_on_companions(mg, device):
-    # the same source gives the same bits alone, in a batch, and in another slot
+    # the same source gives the same bits alone, in a batch, and in another slot, at
+    # every row width (8 / 16 / 32 / 64 columns for batches of 1-8 / 9-16 / 17-32 / 33-64)
     g = mg.rmat_graph(13, 16, seed=2)
     srcs = mg.pick_sources(g, 40, seed=4)
     with mg.Graph(g, symmetric=True) as dg:
@@ -246,6 +247,9 @@ def test_ppr_batch_rows_do_not_depend_on_companions(mg, device):
         for j in (0, 17, 39):
             alone = dg.ppr_batch(srcs[j:j + 1])
             assert np.array_equal(full[j].view(np.int32), alone[0].view(np.int32)), j
+        for k in (9, 20):
+            part = dg.ppr_batch(srcs[:k])
+            assert np.array_equal(full[:k].view(np.int32), part.view(np.int32)), k
         rev = dg.ppr_batch(srcs[::-1].copy())
         assert np.array_equal(full.view(np.int32), rev[::-1].view(np.int32))
         # other iteration counts and teleport values

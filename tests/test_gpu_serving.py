@@ -106,15 +106,17 @@ def test_serving_thin_searches_equal_alone(mg, device, graphs, delta):
     assert joined > 0
 
 
-def test_serving_high_diameter_graph_runs_thin(mg, oracle, device):
+@pytest.mark.parametrize("slice_us", [0, 1, -1])
+def test_serving_high_diameter_graph_runs_thin(mg, oracle, device, slice_us):
     """A 300 x 300 grid (about 600 BFS levels) is detected by the probe search and its
     BFS / SSSP engines run thin searches by default; the R-MAT graph beside it does
-    not.  Results equal the oracle's."""
+    not.  Results equal the oracle's with the default time slice, with a 1 us slice (a
+    search pauses after every level and resumes in the next launch) and unsliced."""
     grid = mg.grid_graph(300, seed=41, weight_max=16)
     rm = mg.rmat_graph(12, 16, seed=42, weight_max=64)
     dgs = [mg.Graph(grid, symmetric=True), mg.Graph(rm, symmetric=True)]
     try:
-        with mg.Server() as srv:
+        with mg.Server(thin_slice_us=slice_us) as srv:
             ids = [srv.add_graph(d) for d in dgs]
             assert srv.stats()["thin_engines"] == 2
             srcs = [int(s) for s in mg.pick_sources(grid, 12, seed=3)]
@@ -200,3 +202,50 @@ def test_serving_rejects_bad_queries(mg, device, graphs):
             assert srv.wait(q, 60000)["status"] == 0
     finally:
         unweighted.close()
+
+
+@pytest.mark.parametrize("thin", [False, True])
+def test_serving_output_memory_kinds(mg, device, graphs, thin):
+    """A query's row reaches pinned host memory (mg_host_alloc), ordinary pageable host
+    memory and device memory alike, for batched and thin engines and for personalized
+    PageRank: every copy equals the single-query result."""
+    import ctypes as C
+
+    hosts, dgs = graphs
+    hip = C.CDLL("libamdhip64.so")
+    kinds = [mg.MG_Q_BFS, mg.MG_Q_SSSP] + ([] if thin else [mg.MG_Q_PPR])
+    cfg = {"thin_levels": 1} if thin else {"thin_levels": -1}
+    dev = []
+    try:
+        with mg.Server(**cfg) as srv:
+            ids = [srv.add_graph(d) for d in dgs]
+            runs = []
+            for gi, d in enumerate(dgs):
+                n = hosts[gi].n
+                for kind in kinds:
+                    src = int(mg.pick_sources(hosts[gi], 1, seed=gi * 7 + kind)[0])
+                    dt = np.float32 if kind == mg.MG_Q_PPR else np.int32
+                    pinned = mg.PinnedArray(n, dt)
+                    pageable = np.full(n, -7, dt)
+                    p = C.c_void_p()
+                    assert hip.hipMalloc(C.byref(p), C.c_size_t(4 * n)) == 0
+                    dev.append(p)
+                    qids = [srv.submit(ids[gi], kind, src, pinned)]
+                    for ptr, on_dev in ((pageable.ctypes.data, 0), (p.value, 1)):
+                        q = C.c_int64()
+                        mg.check(mg.lib().mg_server_submit(srv._h, ids[gi], kind, src, C.c_void_p(ptr), on_dev,
+                                                           C.byref(q)))
+                        qids.append(q.value)
+                    runs.append((gi, kind, src, pinned, pageable, p, qids))
+            for *_, qids in runs:
+                for q in qids:
+                    assert srv.wait(q, 60000)["status"] == 0
+        for gi, kind, src, pinned, pageable, p, _ in runs:
+            exp = _alone(mg, dgs[gi], kind, src)
+            ondev = np.empty_like(pageable)
+            assert hip.hipMemcpy(ondev.ctypes.data_as(C.c_void_p), p, C.c_size_t(ondev.nbytes), 2) == 0
+            for got in (pinned.array, pageable, ondev):
+                assert _same(mg, kind, got, exp), (gi, kind, src, thin)
+    finally:
+        for p in dev:
+            hip.hipFree(p)
