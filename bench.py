@@ -325,12 +325,12 @@ def varied_sources_part(mg, oracle, dg, g, seed: int, log) -> dict:
 
 
 def serving_part(mg, seed: int, seconds: float, rate: float, log, mix=(0.6, 0.3, 0.1), graphs=None,
-                 cfg=None) -> dict:
+                 cfg=None, relabel: bool = True) -> dict:
     """Poisson query stream (seeded) over six resident graphs -- R-MAT scales 16, 18, 20, 22 (symmetrized,
-    weights 1..64) and 1024^2 / 2048^2 grids (weights 1..16) -- with 60% BFS, 30% SSSP and 10% personalized
-    PageRank (FP32, 20 iterations) from sources drawn among non-isolated vertices.  Latency = result copied
-    to pinned host memory minus scheduled arrival.  Every 25th query's result is compared with the same query
-    run alone afterwards."""
+    weights 1..64; with the degree-ordered twin unless relabel is False) and 1024^2 / 2048^2 grids (weights
+    1..16) -- with 60% BFS, 30% SSSP and 10% personalized PageRank (FP32, 20 iterations) from sources drawn
+    among non-isolated vertices.  Latency = result copied to pinned host memory minus scheduled arrival.
+    Every 25th query's result is compared with the same query run alone afterwards."""
     import threading
 
     hosts = [mg.rmat_graph(s, 16, seed=seed + s, weight_max=64) for s in (16, 18, 20, 22)]
@@ -339,7 +339,7 @@ def serving_part(mg, seed: int, seconds: float, rate: float, log, mix=(0.6, 0.3,
     if graphs:  # a subset of the six, for isolating engines
         keep = [names.index(x) for x in graphs]
         hosts, names = [hosts[i] for i in keep], [names[i] for i in keep]
-    dgs = [mg.Graph(h, symmetric=True) for h in hosts]
+    dgs = [mg.Graph(h, symmetric=True, relabel=relabel and x.startswith("rmat")) for h, x in zip(hosts, names)]
     rng = np.random.default_rng(seed)
     nq = int(rng.poisson(rate * seconds))
     gaps = rng.exponential(1.0 / rate, nq)
@@ -393,7 +393,8 @@ def serving_part(mg, seed: int, seconds: float, rate: float, log, mix=(0.6, 0.3,
            "aggregate_gteps": te / (t_end - t_start) / 1e6, "completed_qps": nq / ((t_end - t_start) / 1e3),
            "device_bytes": st["device_bytes"], "graphs": names, "checked_queries": len(checked),
            "thin_engines": st["thin_engines"], "dispatcher_busy_frac": st["dispatch_busy_ms"] / (t_end - t_start),
-           "dispatcher_passes": st["dispatch_passes"], "rounds": st["rounds"], "server_config": cfg}
+           "dispatcher_passes": st["dispatch_passes"], "rounds": st["rounds"], "server_config": cfg,
+           "rmat_twins": relabel}
     # service time: admitted into a slot / lane until done (the rest of the latency is waiting)
     svc = np.array([i["t_done_ms"] - i["t_admit_ms"] for i in infos])
     per = {}
@@ -607,7 +608,8 @@ def main() -> int:
             mix = tuple(float(x) for x in a.serve_mix.split(","))
             cfg = {k: int(v) for k, v in (kv.split("=") for kv in a.serve_config.split(",") if kv)}
             parts["serving"] = serving_part(mg, a.seed, a.serve_seconds, a.serve_rate, log, mix,
-                                            [x for x in a.serve_graphs.split(",") if x] or None, cfg)
+                                            [x for x in a.serve_graphs.split(",") if x] or None, cfg,
+                                            relabel=not a.no_relabel)
     except Mismatch as e:
         print(f"bench.py: PARITY FAILURE: {e}", file=sys.stderr, flush=True)
         return 1

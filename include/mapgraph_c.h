@@ -307,12 +307,16 @@ typedef struct mg_server_config {
                                  vertex, at mg_server_add_graph) takes at least this many
                                  levels run as thin searches: one workgroup runs a whole
                                  query inside one launch (0 = 256; -1 = never)          */
-    int32_t thin_lanes;       /* concurrent thin queries per (graph, kind) (0 = 32)      */
+    int32_t thin_lanes;       /* concurrent thin queries per (graph, kind) (0 = 64)      */
     int32_t thin_streams;     /* thin launches in flight per (graph, kind), 1..8 (0 = 2) */
     int32_t thin_slice_us;    /* a thin search pauses at a level boundary after this many
                                  microseconds in a launch and continues in a later one
-                                 (0 = 2000; -1 = runs to the end in one launch)          */
-    int32_t reserved[3];
+                                 (0 = 50000; -1 = runs to the end in one launch)         */
+    int32_t ppr_gate_us;      /* personalized PageRank cohorts: an idle engine starts when
+                                 ppr_batch_min queries wait or the oldest has waited this
+                                 long (0 = 20000; -1 = every query starts on arrival)   */
+    int32_t ppr_batch_min;    /* (0 = 8, at most 64)                                     */
+    int32_t reserved[1];
 } mg_server_config;
 
 typedef struct mg_query_info {

@@ -95,7 +95,8 @@ class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("max_slots", C.c_int32), ("ppr_iters", C.c_int32),
                 ("ppr_teleport", C.c_float), ("steps_per_round", C.c_int32), ("sssp_delta", C.c_int32),
                 ("thin_levels", C.c_int32), ("thin_lanes", C.c_int32), ("thin_streams", C.c_int32),
-                ("thin_slice_us", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("thin_slice_us", C.c_int32), ("ppr_gate_us", C.c_int32),
+                ("ppr_batch_min", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 MG_TUNE_BFS, MG_TUNE_SSSP, MG_TUNE_BFS_BATCH, MG_TUNE_SSSP_BATCH = 0, 1, 2, 3

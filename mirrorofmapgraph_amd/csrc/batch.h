@@ -214,6 +214,9 @@ struct PprDev {
     int *big = nullptr, *src = nullptr;   // long in-lists; per-slot sources (-1 = idle)
 };
 
+// out[s * n + v] = rows[s * n + perm[v]] for the first k rows: a twin's batched rows in
+// the caller's numbering.
+int launch_unpermute_rows(const int *perm, const int *rows, int n, int k, int *out, hipStream_t st);
 int ppr_dev_alloc(PprDev &p, mg_graph *g);
 void ppr_dev_free(PprDev &p);
 int64_t ppr_dev_bytes(const mg_graph *g);

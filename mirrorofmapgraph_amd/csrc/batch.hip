@@ -660,6 +660,13 @@ __global__ void unpermute_rows(const int *perm, const int *rows, int n, int *out
         out[base + v] = rows[base + perm[v]];
 }
 
+int launch_unpermute_rows(const int *perm, const int *rows, int n, int k, int *out, hipStream_t st) {
+    if (k <= 0 || n <= 0) return MG_OK;
+    hipLaunchKernelGGL(unpermute_rows, dim3(grid_blocks_cap(n, BLK, 64), k), dim3(BLK), 0, st, perm, rows, n, out);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
 int finish_batch(mg_graph *g, BatchDev &b, mg_stats *stats, void *out, size_t bytes, bool out_on_device,
                  const CallTimer &timer) {
     const int *result = b.rows;

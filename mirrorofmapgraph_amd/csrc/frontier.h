@@ -618,6 +618,9 @@ __global__ void iota_i32(int *p, int64_t n);
 
 // Sets up the bitmaps/queues of graph g's workspace and the expansion views.
 FrontierArgs make_frontier_args(mg_graph *g, int directed);
+// out[v] = vals[perm[v]] for v < n on st: a twin's row in the caller's numbering (perm
+// maps caller ids to twin ids); out may be device memory or pinned host memory.
+int launch_unpermute(const int *perm, const void *vals, int n, void *out, hipStream_t st);
 int launch_fill(mg_graph *g, int *p, int v, int64_t n);
 int launch_fill(mg_graph *g, uint32_t *p, uint32_t v, int64_t n);
 int launch_reached_stats(mg_graph *g, const int *vals, int unreached, mg_stats *st);

@@ -204,6 +204,13 @@ __global__ void unpermute_vals(const int *perm, const int *vals, int n, int *out
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) out[v] = vals[perm[v]];
 }
 
+int launch_unpermute(const int *perm, const void *vals, int n, void *out, hipStream_t st) {
+    hipLaunchKernelGGL(unpermute_vals, dim3(grid_blocks_cap(n, BLK, 4 * MAX_GRID)), dim3(BLK), 0, st, perm,
+                       (const int *)vals, n, (int *)out);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
 int finish_search(mg_graph *g, mg_stats *stats, const int *vals, int unreached, void *out, const void *result,
                   size_t bytes, bool out_on_device, const CallTimer &timer) {
     Workspace &ws = g->ws;

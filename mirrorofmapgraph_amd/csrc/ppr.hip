@@ -450,11 +450,16 @@ extern "C" int mg_ppr_batch(mg_graph *g, const int32_t *srcs, int32_t k, float *
     if ((rc = check_batch_sources("mg_ppr_batch", g, srcs, k))) return rc;
     if (!(o.pr_teleport >= 0.f && o.pr_teleport <= 1.f))
         return set_error(MG_ERR_INVALID_ARG, "mg_ppr_batch: pr_teleport must be in [0, 1]");
-    PprDev *&p = g->steps->pdev;
+    // the degree-ordered twin when the graph keeps one (sources renamed, rows mapped
+    // back); its in-lists are in twin order, so the sums are the twin's, as for serving
+    int32_t tsrc[PPR_SLOTS];
+    mg_graph *sg = batch_graph(g, o, srcs, k, tsrc);
+    if (sg != g) srcs = tsrc;
+    PprDev *&p = sg->steps->pdev;
     if (!p) {
         p = new (std::nothrow) PprDev();
         if (!p) return set_error(MG_ERR_OOM, "host allocation failed");
-        if ((rc = ppr_dev_alloc(*p, g))) {
+        if ((rc = ppr_dev_alloc(*p, sg))) {
             delete p;
             p = nullptr;
             return rc;
@@ -472,8 +477,14 @@ extern "C" int mg_ppr_batch(mg_graph *g, const int32_t *srcs, int32_t k, float *
     const int iters = ppr_iters_of(o);
     MG_HIP(hipEventRecord(g->ev0, g->stream));
     if ((rc = ppr_seed(*p, g->stream, sl, retire))) return rc;
-    if ((rc = ppr_iterate(*p, g, g->stream, o.pr_teleport, iters))) return rc;
+    if ((rc = ppr_iterate(*p, sg, g->stream, o.pr_teleport, iters))) return rc;
     if ((rc = ppr_extract_rows(*p, g->stream, all))) return rc;
+    const float *result = p->rows;
+    if (sg != g) {  // rows 0..k-1 back to the caller's ids, into the spare rank buffer
+        float *spare = p->X[p->cur ^ 1];
+        if ((rc = launch_unpermute_rows(sg->perm, (const int *)p->rows, g->n, k, (int *)spare, g->stream))) return rc;
+        result = spare;
+    }
     MG_HIP(hipEventRecord(g->ev1, g->stream));
     MG_HIP(hipEventSynchronize(g->ev1));
     float ms = 0;
@@ -485,7 +496,7 @@ extern "C" int mg_ppr_batch(mg_graph *g, const int32_t *srcs, int32_t k, float *
         stats->vertices_reached = (int64_t)k * g->n;
         stats->device_ms = ms;
     }
-    if ((rc = deliver(g, rank_out, p->rows, 4 * (size_t)k * g->n, o.out_on_device))) return rc;
+    if ((rc = deliver(g, rank_out, result, 4 * (size_t)k * g->n, o.out_on_device))) return rc;
     if (stats) stats->host_ms = timer.ms();
     return MG_OK;
 }

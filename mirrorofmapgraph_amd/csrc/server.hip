@@ -61,7 +61,11 @@ struct ThinSlot {
 
 struct Engine {
     Server *srv = nullptr;
-    mg_graph *g = nullptr;
+    mg_graph *g = nullptr;                  // the graph the engine searches: the registered
+                                            // graph's degree-ordered twin when it has one
+                                            // (batched engines), else the graph itself
+    const int32_t *h_perm = nullptr;        // caller id -> twin id (host), null without a twin
+    const int *d_perm = nullptr;            // the same map on the device
     int kind = 0;
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;
@@ -144,6 +148,21 @@ static int deliver_row(const Query *r, const void *row, size_t bytes, hipStream_
     return MG_OK;
 }
 
+// The same for a row of an engine that searches a twin: out[v] = row[d_perm[v]], by a
+// gather kernel straight into the output where the device can write it, else through
+// the host (the documented blocking path for other host memory).
+static int deliver_twin_row(const Query *r, const int *d_perm, const int32_t *h_perm, const void *row, int n,
+                            hipStream_t st) {
+    if (r->out_dev) return launch_unpermute(d_perm, row, n, r->out_dev, st);
+    if (!r->out) return MG_OK;
+    std::vector<int32_t> tmp((size_t)n);
+    MG_HIP(hipMemcpyAsync(tmp.data(), row, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
+    MG_HIP(hipStreamSynchronize(st));
+    int32_t *o = (int32_t *)r->out;
+    for (int v = 0; v < n; ++v) o[v] = tmp[(size_t)h_perm[v]];
+    return MG_OK;
+}
+
 // The device address of out: itself for device memory, the mapped address of pinned
 // host memory, null for host memory the device cannot write.
 static void *device_view(void *out, bool on_device) {
@@ -177,7 +196,7 @@ static void engine_free(Server &s, Engine &e) {
 
 // Allocates the engine's device state within the budget: idle engines are freed
 // first if needed, and the slot count is halved until it fits.
-static int thin_lanes(const Server &s) { return s.cfg.thin_lanes > 0 ? s.cfg.thin_lanes : 32; }
+static int thin_lanes(const Server &s) { return s.cfg.thin_lanes > 0 ? s.cfg.thin_lanes : 64; }
 // Thin launches in flight per engine.
 static int thin_streams(const Server &s) { return s.cfg.thin_streams > 0 ? s.cfg.thin_streams : 2; }
 
@@ -355,6 +374,26 @@ static int thin_round(Server &s, Engine &e, bool *queued) {
     return MG_OK;
 }
 
+// PageRank cohorts.  A sweep over the graph costs the same for one live slot as for
+// eight (rows are at least 8 wide), so serving each query on arrival keeps a lightly
+// loaded engine sweeping nonstop for one or two queries at a time.  An idle engine
+// starts when ppr_batch_min queries wait or the oldest has waited ppr_gate_us, and a
+// running cohort takes new queries during its first PPR_JOIN iterations only; the
+// others start the next cohort.
+constexpr int PPR_JOIN = 8;
+static bool ppr_admits(const Server &s, const Engine &e) {
+    if (s.cfg.ppr_gate_us < 0 || e.waiting.empty()) return true;
+    if (e.running > 0) {
+        int most = 0;
+        for (const Query *r : e.slot_q)
+            if (r) most = std::max(most, s.cfg.ppr_iters - r->iters_left);
+        return most < PPR_JOIN;
+    }
+    const int batch = s.cfg.ppr_batch_min > 0 ? s.cfg.ppr_batch_min : 8;
+    const double gate_ms = (s.cfg.ppr_gate_us > 0 ? s.cfg.ppr_gate_us : 20000) / 1000.0;
+    return (int)e.waiting.size() >= batch || s.now() - e.waiting.front()->t_submit >= gate_ms;
+}
+
 // One round of an engine; returns true if it queued work.  Called with s.mu held.
 static int engine_round(Server &s, Engine &e, bool *queued) {
     if (e.thin) return thin_round(s, e, queued);
@@ -431,7 +470,9 @@ static int engine_round(Server &s, Engine &e, bool *queued) {
             const size_t bytes = 4 * (size_t)e.g->n;
             const void *row = e.kind == MG_Q_PPR ? (const void *)(e.p.rows + (size_t)sl * e.g->n)
                                                  : (const void *)(e.b.rows + (size_t)sl * e.g->n);
-            if ((rc = deliver_row(r, row, bytes, e.st))) return rc;
+            if ((rc = e.d_perm ? deliver_twin_row(r, e.d_perm, e.h_perm, row, e.g->n, e.st)
+                               : deliver_row(r, row, bytes, e.st)))
+                return rc;
             if (e.kind == MG_Q_PPR) {
                 r->te = e.g->m * (int64_t)(s.cfg.ppr_iters);
                 r->nr = e.g->n;
@@ -445,7 +486,8 @@ static int engine_round(Server &s, Engine &e, bool *queued) {
     SeedList seeds{};
     std::vector<bool> busy(e.S, false);
     for (int sl = 0; sl < e.S; ++sl) busy[sl] = e.slot_q[sl] != nullptr;
-    for (int sl = 0; sl < e.S && !e.waiting.empty(); ++sl) {
+    const bool admit = e.kind != MG_Q_PPR || ppr_admits(s, e);
+    for (int sl = 0; sl < e.S && admit && !e.waiting.empty(); ++sl) {
         if (busy[sl]) continue;
         Query *r = e.waiting.front();
         e.waiting.pop_front();
@@ -455,7 +497,7 @@ static int engine_round(Server &s, Engine &e, bool *queued) {
         e.slot_q[sl] = r;
         busy[sl] = true;
         seeds.slot[seeds.n] = sl;
-        seeds.src[seeds.n++] = r->src;
+        seeds.src[seeds.n++] = e.h_perm ? e.h_perm[r->src] : r->src;
         e.running++;
     }
     if (e.kind == MG_Q_BFS) {
@@ -575,7 +617,8 @@ int mg_server_create(const mg_server_config *cfg, mg_server **out) {
     else mg_server_config_default(&s->cfg);
     if (s->cfg.hbm_budget <= 0 || s->cfg.max_slots < 1 || s->cfg.max_slots > MG_BATCH_MAX || s->cfg.ppr_iters < 1 ||
         s->cfg.steps_per_round < 1 || s->cfg.steps_per_round > 64 || s->cfg.thin_streams < 0 || s->cfg.thin_streams > 8 ||
-        s->cfg.thin_slice_us < -1) {
+        s->cfg.thin_slice_us < -1 || s->cfg.ppr_gate_us < -1 || s->cfg.ppr_batch_min < 0 ||
+        s->cfg.ppr_batch_min > PPR_SLOTS) {
         delete s;
         return set_error(MG_ERR_INVALID_ARG, "invalid server configuration");
     }
@@ -589,7 +632,7 @@ int mg_server_create(const mg_server_config *cfg, mg_server **out) {
             delete s;
             return set_error(MG_ERR_DEVICE, "device wall-clock rate unavailable");
         }
-        const long long us = s->cfg.thin_slice_us > 0 ? s->cfg.thin_slice_us : 2000;
+        const long long us = s->cfg.thin_slice_us > 0 ? s->cfg.thin_slice_us : 50000;
         s->thin_slice = std::max(1LL, us * khz / 1000);
     }
     s->th = std::thread(dispatcher, static_cast<Server *>(s));
@@ -630,9 +673,16 @@ int mg_server_add_graph(mg_server *s, mg_graph *g, int32_t *graph_id) {
     for (int k = 0; k < 3; ++k) {
         auto e = std::make_unique<Engine>();
         e->srv = s;
-        e->g = g;
         e->kind = k;
         e->thin = thin && k != MG_Q_PPR;
+        // batched engines search the twin (sources renamed on admission, rows mapped
+        // back on delivery); a thin search writes its row straight into the output, so
+        // thin engines keep the caller's numbering
+        e->g = g->twin && !e->thin ? g->twin : g;
+        if (e->g != g) {
+            e->h_perm = g->h_perm.data();
+            e->d_perm = g->twin->perm;
+        }
         if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&e->ev, hipEventDisableTiming) != hipSuccess)
             return set_error(MG_ERR_DEVICE, "stream/event creation failed");

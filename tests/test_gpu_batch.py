@@ -237,6 +237,21 @@ def test_ppr_batch_against_float64(mg, oracle, device, symmetric):
         assert np.max(np.abs(got[j].astype(np.float64) - exp)) <= PPR_TOL, j
 
 
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_ppr_batch_on_relabelled_twin(mg, oracle, device, symmetric):
+    # a graph with a degree-ordered twin runs the batch there (renamed sources, rows
+    # mapped back): within the tolerance of the oracle, and the same bits alone
+    g = mg.rmat_graph(12, 16, seed=15, symmetric=symmetric)
+    srcs = mg.pick_sources(g, 24, seed=16)
+    with mg.Graph(g, symmetric=symmetric, build_csc=not symmetric, relabel=True) as dg:
+        got = dg.ppr_batch(srcs)
+        alone = dg.ppr_batch(srcs[5:6])
+    assert np.array_equal(got[5].view(np.int32), alone[0].view(np.int32))
+    for j in range(0, 24, 5):
+        exp = oracle.ppr(g.row_offsets, g.col_idx, int(srcs[j]), iters=20, teleport=0.15)
+        assert np.max(np.abs(got[j].astype(np.float64) - exp)) <= PPR_TOL, j
+
+
 def test_ppr_batch_rows_do_not_depend_on_companions(mg, device):
     # the same source gives the same bits alone, in a batch, and in another slot, at
     # every row width (8 / 16 / 32 / 64 columns for batches of 1-8 / 9-16 / 17-32 / 33-64)

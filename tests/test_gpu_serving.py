@@ -249,3 +249,84 @@ def test_serving_output_memory_kinds(mg, device, graphs, thin):
     finally:
         for p in dev:
             hip.hipFree(p)
+
+
+def test_serving_ppr_cohorts(mg, device, graphs):
+    """Personalized PageRank queries start in cohorts: with ppr_batch_min = 4 and a long
+    gate, three queries wait until a fourth arrives; then all four run together, and
+    every row equals the single-query result bit for bit."""
+    hosts, dgs = graphs
+    srcs = [int(s) for s in mg.pick_sources(hosts[0], 4, seed=12)]
+    with mg.Server(ppr_batch_min=4, ppr_gate_us=30_000_000) as srv:
+        gid = srv.add_graph(dgs[0])
+        outs = [mg.PinnedArray(hosts[0].n, np.float32) for _ in srcs]
+        qids = [srv.submit(gid, mg.MG_Q_PPR, s, o) for s, o in zip(srcs[:3], outs[:3])]
+        with pytest.raises(mg.MgError) as e:
+            srv.wait(qids[0], 300)  # not started: the cohort is not full and the gate is 30 s
+        assert e.value.code == mg.MG_ERR_TIMEOUT
+        t4 = time.perf_counter()
+        qids.append(srv.submit(gid, mg.MG_Q_PPR, srcs[3], outs[3]))
+        infos = [srv.wait(q, 60000) for q in qids]
+        assert time.perf_counter() - t4 < 20
+    admitted = [i["t_admit_ms"] for i in infos]
+    assert min(admitted) >= infos[3]["t_submit_ms"]  # nobody started before the fourth query
+    for s, o in zip(srcs, outs):
+        assert _same(mg, mg.MG_Q_PPR, o.array, _alone(mg, dgs[0], mg.MG_Q_PPR, s))
+
+
+def test_serving_on_degree_ordered_twins(mg, device):
+    """On graphs that keep a degree-ordered twin, the batched engines search the twin:
+    sources are renamed on admission and rows mapped back on delivery, into pinned,
+    pageable and device memory.  BFS / SSSP rows equal the plain graph's; personalized
+    PageRank rows equal mg_ppr_batch on the same graph (which runs on the twin too),
+    bit for bit."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    hosts = [mg.rmat_graph(12, 16, seed=51, weight_max=64), mg.rmat_graph(11, 8, seed=52, weight_max=16)]
+    dgs = [mg.Graph(h, symmetric=True, relabel=True) for h in hosts]
+    plain = [mg.Graph(h, symmetric=True) for h in hosts]
+    dev = []
+    try:
+        with mg.Server(thin_levels=-1) as srv:
+            ids = [srv.add_graph(d) for d in dgs]
+            runs = []
+            for i, (gi, kind, src) in enumerate(_queries(mg, hosts, 48, seed=53)):
+                n = hosts[gi].n
+                dt = np.float32 if kind == mg.MG_Q_PPR else np.int32
+                how = i % 3
+                if how == 0:
+                    buf = mg.PinnedArray(n, dt)
+                    q = srv.submit(ids[gi], kind, src, buf)
+                else:
+                    if how == 1:
+                        buf = np.full(n, -7, dt)
+                        ptr, on_dev = buf.ctypes.data, 0
+                    else:
+                        p = C.c_void_p()
+                        assert hip.hipMalloc(C.byref(p), C.c_size_t(4 * n)) == 0
+                        dev.append(p)
+                        buf, ptr, on_dev = p, p.value, 1
+                    qq = C.c_int64()
+                    mg.check(mg.lib().mg_server_submit(srv._h, ids[gi], kind, src, C.c_void_p(ptr), on_dev,
+                                                       C.byref(qq)))
+                    q = qq.value
+                runs.append((gi, kind, src, how, buf, dt, q))
+            for *_, q in runs:
+                assert srv.wait(q, 60000)["status"] == 0
+        for gi, kind, src, how, buf, dt, _ in runs:
+            if how == 0:
+                got = buf.array
+            elif how == 1:
+                got = buf
+            else:
+                got = np.empty(hosts[gi].n, dt)
+                assert hip.hipMemcpy(got.ctypes.data_as(C.c_void_p), buf, C.c_size_t(got.nbytes), 2) == 0
+            assert _same(mg, kind, got, _alone(mg, dgs[gi], kind, src)), (gi, kind, src, how)
+            if kind != mg.MG_Q_PPR:
+                assert np.array_equal(got, _alone(mg, plain[gi], kind, src)), (gi, kind, src, how)
+    finally:
+        for p in dev:
+            hip.hipFree(p)
+        for d in dgs + plain:
+            d.close()
