@@ -226,6 +226,7 @@ __device__ __forceinline__ long long pull_best(const SsspArgs &a, int beg, int e
 // mn takes the smallest improved distance (the next level's fmin).
 constexpr int DENSE_WIN = 2;  // windows in flight (4 and 8: more registers, fewer waves, slower)
 constexpr int DENSE_PER = DENSE_V / WAVE;  // vertices per lane in a range task
+constexpr int DENSE_DIRECT = 16;           // candidates a window folds in with LDS atomics
 
 // Range index: a range task's lists concatenated, for the wave that walks them.  Bit p
 // of `heads` is set where a non-empty list starts at position p of the concatenation
@@ -315,6 +316,7 @@ __device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, in
     const int *bs = lds.base[wv];
     int *mins = lds.val[wv];
     const uint64_t upto = lane == WAVE - 1 ? ~0ull : ((2ull << lane) - 1);  // lanes 0..lane
+    const int fmin32 = fmin < INT_MAX ? (int)fmin : INT_MAX;
     // vertex va + 64 j + lane is list 64 j + lane
     int beg[DENSE_PER], len[DENSE_PER], cur[DENSE_PER], rank[DENSE_PER];
 #pragma unroll
@@ -361,16 +363,29 @@ __device__ __forceinline__ void dense_range(const SsspArgs &a, int L, int va, in
 #pragma unroll
             for (int k = 0; k < DENSE_WIN; ++k) wt[k] = a.w[slot[k]];
         }
+        // 32-bit forms: fmin + w < bound as w < bound - fmin (weights are non-negative,
+        // fmin32 in [0, INT_MAX]), and dist[u] + w in unsigned arithmetic (an unreached
+        // INT_MAX plus a weight below INT_MAX does not wrap)
         bool need[DENSE_WIN];
         int du[DENSE_WIN];
 #pragma unroll
-        for (int k = 0; k < DENSE_WIN; ++k) need[k] = ok[k] && fmin + wt[k] < (long long)bound[k];
+        for (int k = 0; k < DENSE_WIN; ++k) need[k] = ok[k] && wt[k] < bound[k] - fmin32;
 #pragma unroll
         for (int k = 0; k < DENSE_WIN; ++k) du[k] = a.dist[need[k] ? u[k] : va];
 #pragma unroll
         for (int k = 0; k < DENSE_WIN; ++k) {
-            const long long d = (long long)du[k] + wt[k];
-            int best = need[k] && d < INT_MAX ? (int)d : INT_MAX;
+            const uint32_t d = (uint32_t)du[k] + (uint32_t)wt[k];
+            int best = need[k] && d < (uint32_t)INT_MAX ? (int)d : INT_MAX;
+            // lanes with a candidate below their owner's minimum: none (the common case
+            // once the bound prunes) skips the window's reduction; a few fold theirs in
+            // with LDS atomics; otherwise a segmented scan reduces each owner's run first
+            const bool cand = ok[k] && best < bound[k];
+            const uint64_t cm = wave_ballot(cand);
+            if (cm == 0) continue;
+            if (__popcll(cm) <= DENSE_DIRECT) {
+                if (cand) atomicMin(&mins[own[k]], best);
+                continue;
+            }
             // runs: one per owner (lane 0 starts one); the lanes past the end follow the
             // end bit and form one more run
             const int start = 63 - __builtin_clzll((w[k] | 1ull) & upto);
