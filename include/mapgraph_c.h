@@ -316,7 +316,10 @@ typedef struct mg_server_config {
                                  ppr_batch_min queries wait or the oldest has waited this
                                  long (0 = 20000; -1 = every query starts on arrival)   */
     int32_t ppr_batch_min;    /* (0 = 8, at most 64)                                     */
-    int32_t reserved[1];
+    int32_t plain_kinds;      /* bit k set (1 << MG_Q_BFS, 1 << MG_Q_SSSP): those queries on
+                                 graphs that are not thin run one at a time as whole-GPU
+                                 searches (mg_bfs / mg_sssp, the graph's twin included) on
+                                 a worker thread per (graph, kind) instead of in batches */
 } mg_server_config;
 
 typedef struct mg_query_info {

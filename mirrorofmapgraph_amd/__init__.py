@@ -96,7 +96,7 @@ class ServerConfig(C.Structure):
                 ("ppr_teleport", C.c_float), ("steps_per_round", C.c_int32), ("sssp_delta", C.c_int32),
                 ("thin_levels", C.c_int32), ("thin_lanes", C.c_int32), ("thin_streams", C.c_int32),
                 ("thin_slice_us", C.c_int32), ("ppr_gate_us", C.c_int32),
-                ("ppr_batch_min", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("ppr_batch_min", C.c_int32), ("plain_kinds", C.c_int32)]
 
 
 MG_TUNE_BFS, MG_TUNE_SSSP, MG_TUNE_BFS_BATCH, MG_TUNE_SSSP_BATCH = 0, 1, 2, 3

@@ -25,6 +25,7 @@
 #include "batch.h"
 #include "thin.h"
 
+#include <atomic>
 #include <climits>
 #include <condition_variable>
 #include <deque>
@@ -82,6 +83,12 @@ struct Engine {
     std::vector<Query *> retiring;          // rows copied in the round in flight
     int running = 0;
     double idle_since = 0;
+    // plain engines: one whole-GPU search per query (mg_bfs / mg_sssp) on a worker thread
+    bool plain = false;
+    std::thread worker;
+    std::mutex wmu;                         // the worker's wake-up (wake / quit under wmu)
+    std::condition_variable wcv;
+    bool wake = false, quit = false;
     // thin engines (BFS / SSSP on a high-diameter graph): one workgroup per query
     bool thin = false;
     ThinDev td;
@@ -97,8 +104,16 @@ struct Server {
     std::vector<mg_graph *> graphs;
     std::vector<std::unique_ptr<Engine>> engines;  // graph * 3 + kind
     std::unordered_map<int64_t, std::unique_ptr<Query>> queries;
+    // mu guards the queues and query states.  The dispatcher holds it through its
+    // passes; other threads announce themselves in `want` (fair_lock) and the
+    // dispatcher steps aside between passes while any waits.  Waiting for a query
+    // does not sleep on mu: finished queries bump done_seq under dmu and wake dcv.
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::atomic<int> want{0};
+    std::condition_variable cv_work;
+    std::mutex dmu;
+    std::condition_variable dcv;
+    std::atomic<uint64_t> done_seq{0};
     std::thread th;
     bool stop = false;
     int device = 0;
@@ -286,6 +301,19 @@ static void finish_query(Server &s, Query *q, double t) {
     s.stats.completed++;
     s.stats.edges_traversed += q->te;
     s.stats.latency_ms_sum += q->t_done - q->t_submit;
+    {
+        std::lock_guard<std::mutex> g(s.dmu);
+        s.done_seq.fetch_add(1);
+    }
+    s.dcv.notify_all();
+}
+
+// mu for a thread other than the dispatcher (see Server::want).
+static std::unique_lock<std::mutex> fair_lock(Server &s) {
+    s.want.fetch_add(1);
+    std::unique_lock<std::mutex> lk(s.mu);
+    s.want.fetch_sub(1);
+    return lk;
 }
 
 // One round of a thin engine: finished launches hand their queries back (rows and
@@ -293,7 +321,6 @@ static void finish_query(Server &s, Query *q, double t) {
 // launched, one workgroup each, on every idle launch stream.  Called with s.mu held.
 static int thin_round(Server &s, Engine &e, bool *queued) {
     *queued = false;
-    bool any_done = false;
     for (ThinSlot &ts : e.tslots) {
         if (!ts.busy) continue;
         const hipError_t q = hipEventQuery(ts.ev);
@@ -318,11 +345,9 @@ static int thin_round(Server &s, Engine &e, bool *queued) {
             r->slot = -1;
             finish_query(s, r, t);
             e.running--;
-            any_done = true;
         }
         ts.qs.clear();
     }
-    if (any_done) s.cv_done.notify_all();
     e.inflight = false;
     for (const ThinSlot &ts : e.tslots) e.inflight |= ts.busy;
     if (e.waiting.empty() && e.paused.empty()) {
@@ -396,6 +421,8 @@ static bool ppr_admits(const Server &s, const Engine &e) {
 
 // One round of an engine; returns true if it queued work.  Called with s.mu held.
 static int engine_round(Server &s, Engine &e, bool *queued) {
+    *queued = false;
+    if (e.plain) return MG_OK;  // served by its worker thread
     if (e.thin) return thin_round(s, e, queued);
     *queued = false;
     std::vector<int> finished;
@@ -414,7 +441,6 @@ static int engine_round(Server &s, Engine &e, bool *queued) {
             r->slot = -1;
             finish_query(s, r, t);
         }
-        if (!e.retiring.empty()) s.cv_done.notify_all();
         e.retiring.clear();
         if (e.kind != MG_Q_PPR && e.b.h_ctl->changed & (1 << 30))
             return set_error(MG_ERR_OVERFLOW, "serving engine task queue overflow");
@@ -537,6 +563,53 @@ static int engine_round(Server &s, Engine &e, bool *queued) {
     return MG_OK;
 }
 
+// A plain engine's worker: takes its engine's queries in arrival order and runs each
+// as an ordinary search on the registered graph (its stream, workspace and twin;
+// the graph's lock serialises it with the graph's other callers), without the server
+// lock held.  A lightly loaded batched engine pays nearly a full round of device work
+// per step whatever its slot count; one whole-GPU search per query costs the search
+// alone (R-MAT scale 22 SSSP: about 1.3 ms).
+static void plain_worker(Server *sp, Engine *ep) {
+    Server &s = *sp;
+    Engine &e = *ep;
+    (void)hipSetDevice(s.device);
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> wl(e.wmu);
+            e.wcv.wait(wl, [&] { return e.wake || e.quit; });
+            if (e.quit) return;
+            e.wake = false;
+        }
+        for (;;) {  // every query queued so far
+            std::unique_lock<std::mutex> lk = fair_lock(s);
+            if (s.stop || e.waiting.empty()) break;
+            Query *r = e.waiting.front();
+            e.waiting.pop_front();
+            r->t_admit = s.now();
+            e.running++;
+            lk.unlock();
+            mg_options o;
+            mg_options_default(&o);
+            o.out_on_device = r->out_on_device ? 1 : 0;
+            mg_stats st;
+            int32_t *out = (int32_t *)r->out;
+            const int rc =
+                e.kind == MG_Q_BFS ? mg_bfs(e.g, r->src, out, &o, &st) : mg_sssp(e.g, r->src, out, &o, &st);
+            lk.lock();
+            if (rc) {
+                r->status = rc;
+                s.error = rc;
+            } else {
+                r->te = st.edges_traversed;
+                r->nr = st.vertices_reached;
+            }
+            e.running--;
+            finish_query(s, r, s.now());
+            s.stats.rounds++;
+        }
+    }
+}
+
 static void dispatcher(Server *sp) {
     Server &s = *sp;
     (void)hipSetDevice(s.device);
@@ -575,16 +648,20 @@ static void dispatcher(Server *sp) {
                 }
                 ep->paused.clear();
                 ep->running = 0;
-                s.cv_done.notify_all();
             }
             any |= q;
-            pending |= ep->inflight || ep->running > 0 || !ep->waiting.empty();
+            pending |= !ep->plain && (ep->inflight || ep->running > 0 || !ep->waiting.empty());
         }
         s.stats.dispatch_busy_ms += s.now() - t_pass;
         s.stats.dispatch_passes++;
         if (!any) {
             // nothing queued this pass: wait briefly for events or new queries
             s.cv_work.wait_for(lk, std::chrono::microseconds(pending ? 20 : 2000));
+        } else if (s.want.load() > 0) {
+            // others wait for the lock: let them in before the next pass (bounded)
+            lk.unlock();
+            for (int i = 0; i < 256 && s.want.load() > 0; ++i) std::this_thread::yield();
+            lk.lock();
         }
     }
 }
@@ -618,7 +695,7 @@ int mg_server_create(const mg_server_config *cfg, mg_server **out) {
     if (s->cfg.hbm_budget <= 0 || s->cfg.max_slots < 1 || s->cfg.max_slots > MG_BATCH_MAX || s->cfg.ppr_iters < 1 ||
         s->cfg.steps_per_round < 1 || s->cfg.steps_per_round > 64 || s->cfg.thin_streams < 0 || s->cfg.thin_streams > 8 ||
         s->cfg.thin_slice_us < -1 || s->cfg.ppr_gate_us < -1 || s->cfg.ppr_batch_min < 0 ||
-        s->cfg.ppr_batch_min > PPR_SLOTS) {
+        s->cfg.ppr_batch_min > PPR_SLOTS || (s->cfg.plain_kinds & ~3)) {
         delete s;
         return set_error(MG_ERR_INVALID_ARG, "invalid server configuration");
     }
@@ -667,7 +744,7 @@ int mg_server_add_graph(mg_server *s, mg_graph *g, int32_t *graph_id) {
             thin = st.iterations >= thin_levels;
         }
     }
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lk = fair_lock(*s);
     const int id = (int)s->graphs.size();
     s->graphs.push_back(g);
     for (int k = 0; k < 3; ++k) {
@@ -675,13 +752,20 @@ int mg_server_add_graph(mg_server *s, mg_graph *g, int32_t *graph_id) {
         e->srv = s;
         e->kind = k;
         e->thin = thin && k != MG_Q_PPR;
+        e->plain = !e->thin && k != MG_Q_PPR && ((s->cfg.plain_kinds >> k) & 1);
         // batched engines search the twin (sources renamed on admission, rows mapped
         // back on delivery); a thin search writes its row straight into the output, so
-        // thin engines keep the caller's numbering
-        e->g = g->twin && !e->thin ? g->twin : g;
+        // thin engines keep the caller's numbering; plain engines call the graph's own
+        // searches, which use its twin themselves
+        e->g = g->twin && !e->thin && !e->plain ? g->twin : g;
         if (e->g != g) {
             e->h_perm = g->h_perm.data();
             e->d_perm = g->twin->perm;
+        }
+        if (e->plain) {  // runs on the graph's stream
+            e->worker = std::thread(plain_worker, static_cast<Server *>(s), e.get());
+            s->engines.push_back(std::move(e));
+            continue;
         }
         if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&e->ev, hipEventDisableTiming) != hipSuccess)
@@ -708,7 +792,7 @@ int mg_server_submit(mg_server *s, int32_t graph_id, int32_t kind, int32_t src, 
                      int64_t *query_id) {
     clear_error();
     if (!s || !query_id) return set_error(MG_ERR_INVALID_ARG, "null argument");
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lk = fair_lock(*s);
     if (graph_id < 0 || graph_id >= (int)s->graphs.size()) return set_error(MG_ERR_INVALID_ARG, "unknown graph %d", graph_id);
     if (kind < MG_Q_BFS || kind > MG_Q_PPR) return set_error(MG_ERR_INVALID_ARG, "unknown query kind %d", kind);
     mg_graph *g = s->graphs[graph_id];
@@ -726,11 +810,21 @@ int mg_server_submit(mg_server *s, int32_t graph_id, int32_t kind, int32_t src, 
     q->out_on_device = out_on_device != 0;
     q->out_dev = device_view(out, q->out_on_device);
     q->t_submit = s->now();
-    s->engines[(size_t)graph_id * 3 + kind]->waiting.push_back(q.get());
+    Engine &e = *s->engines[(size_t)graph_id * 3 + kind];
+    e.waiting.push_back(q.get());
     *query_id = q->id;
     s->queries.emplace(q->id, std::move(q));
     s->stats.submitted++;
-    s->cv_work.notify_one();
+    lk.unlock();
+    if (e.plain) {
+        {
+            std::lock_guard<std::mutex> g(e.wmu);
+            e.wake = true;
+        }
+        e.wcv.notify_one();
+    } else {
+        s->cv_work.notify_one();
+    }
     return MG_OK;
 }
 
@@ -751,17 +845,29 @@ static void fill_info(const Query &q, mg_query_info *info) {
 int mg_server_wait(mg_server *s, int64_t query_id, int32_t timeout_ms, mg_query_info *info) {
     clear_error();
     if (!s) return set_error(MG_ERR_INVALID_ARG, "null server");
-    std::unique_lock<std::mutex> lk(s->mu);
+    const auto deadline = std::chrono::steady_clock::now() + std::chrono::milliseconds(std::max(timeout_ms, 0));
+    std::unique_lock<std::mutex> lk = fair_lock(*s);
     auto it = s->queries.find(query_id);
     if (it == s->queries.end()) return set_error(MG_ERR_INVALID_ARG, "unknown query %lld", (long long)query_id);
     Query *q = it->second.get();
-    auto ready = [&] { return q->done; };
-    if (timeout_ms < 0) s->cv_done.wait(lk, ready);
-    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
-        return set_error(MG_ERR_TIMEOUT, "query %lld not done after %d ms", (long long)query_id, timeout_ms);
+    while (!q->done) {
+        // sleep on the completion counter, not on mu (read under mu: a query that
+        // finishes after this read bumps it)
+        const uint64_t seen = s->done_seq.load();
+        lk.unlock();
+        {
+            std::unique_lock<std::mutex> dl(s->dmu);
+            auto moved = [&] { return s->done_seq.load() != seen; };
+            if (timeout_ms < 0) s->dcv.wait(dl, moved);
+            else s->dcv.wait_until(dl, deadline, moved);
+        }
+        lk = fair_lock(*s);
+        if (!q->done && timeout_ms >= 0 && std::chrono::steady_clock::now() >= deadline)
+            return set_error(MG_ERR_TIMEOUT, "query %lld not done after %d ms", (long long)query_id, timeout_ms);
+    }
     fill_info(*q, info);
     const int st = q->status;
-    s->queries.erase(it);
+    s->queries.erase(query_id);  // by key: the map may have rehashed while the lock was released
     if (st) return set_error(st, "query %lld failed", (long long)query_id);
     return MG_OK;
 }
@@ -776,7 +882,7 @@ int mg_server_now_ms(mg_server *s, double *ms) {
 int mg_server_get_stats(mg_server *s, mg_server_stats *st) {
     clear_error();
     if (!s || !st) return set_error(MG_ERR_INVALID_ARG, "null argument");
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lk = fair_lock(*s);
     *st = s->stats;
     st->device_bytes = s->bytes_used;
     st->thin_engines = 0;
@@ -789,11 +895,21 @@ int mg_server_destroy(mg_server *s) {
     clear_error();
     if (!s) return MG_OK;
     {
-        std::lock_guard<std::mutex> lk(s->mu);
+        std::unique_lock<std::mutex> lk = fair_lock(*s);
         s->stop = true;
         s->cv_work.notify_all();
     }
+    for (auto &e : s->engines)
+        if (e && e->plain) {
+            {
+                std::lock_guard<std::mutex> g(e->wmu);
+                e->quit = true;
+            }
+            e->wcv.notify_one();
+        }
     if (s->th.joinable()) s->th.join();
+    for (auto &e : s->engines)
+        if (e && e->worker.joinable()) e->worker.join();
     (void)hipSetDevice(s->device);
     for (auto &e : s->engines) {
         if (!e) continue;

@@ -330,3 +330,38 @@ def test_serving_on_degree_ordered_twins(mg, device):
             hip.hipFree(p)
         for d in dgs + plain:
             d.close()
+
+
+def test_serving_plain_engines(mg, device, graphs):
+    """plain_kinds: BFS / SSSP queries run one at a time as whole-GPU searches on a worker
+    thread per (graph, kind) (PageRank stays batched); rows into pinned and pageable
+    memory equal the single-query results, with the same TEPS counts."""
+    import ctypes as C
+
+    hosts, dgs = graphs
+    qs = _queries(mg, hosts, 90, seed=61)
+    with mg.Server(plain_kinds=(1 << mg.MG_Q_BFS) | (1 << mg.MG_Q_SSSP)) as srv:
+        ids = [srv.add_graph(d) for d in dgs]
+        outs, qids = [], []
+        for i, (gi, kind, src) in enumerate(qs):
+            dt = np.float32 if kind == mg.MG_Q_PPR else np.int32
+            if i % 2:
+                out = np.full(hosts[gi].n, -7, dt)
+                q = C.c_int64()
+                mg.check(mg.lib().mg_server_submit(srv._h, ids[gi], kind, src, C.c_void_p(out.ctypes.data), 0,
+                                                   C.byref(q)))
+                qids.append(q.value)
+            else:
+                out = mg.PinnedArray(hosts[gi].n, dt)
+                qids.append(srv.submit(ids[gi], kind, src, out))
+            outs.append(out)
+        infos = [srv.wait(q, 60000) for q in qids]
+        assert srv.stats()["completed"] == len(qs)
+    for (gi, kind, src), out, info in zip(qs, outs, infos):
+        got = out if isinstance(out, np.ndarray) else out.array
+        exp = _alone(mg, dgs[gi], kind, src)
+        assert _same(mg, kind, got, exp), (gi, kind, src)
+        if kind != mg.MG_Q_PPR:
+            reached = exp >= 0 if kind == mg.MG_Q_BFS else exp != 2**31 - 1
+            assert info["vertices_reached"] == int(reached.sum())
+            assert info["edges_traversed"] == int(np.diff(hosts[gi].row_offsets)[reached].sum())

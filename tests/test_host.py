@@ -52,7 +52,7 @@ def test_server_config_validation(mg):
     # an invalid configuration is refused before the server touches a device
     for bad in ({"max_slots": 0}, {"max_slots": 257}, {"steps_per_round": 65}, {"thin_streams": 9},
                 {"thin_streams": -1}, {"thin_slice_us": -2}, {"ppr_gate_us": -2},
-                {"ppr_batch_min": 65}, {"ppr_iters": 0}):
+                {"ppr_batch_min": 65}, {"plain_kinds": 4}, {"ppr_iters": 0}):
         with pytest.raises(mg.MgError) as e:
             mg.Server(**bad)
         assert e.value.code == mg.MG_ERR_INVALID_ARG, bad
