@@ -391,6 +391,9 @@ def serving_part(mg, seed: int, seconds: float, rate: float, log, mix=(0.6, 0.3,
            "steady_p50_ms": float(np.percentile(lat[steady], 50)) if steady.any() else None,
            "steady_p99_ms": float(np.percentile(lat[steady], 99)) if steady.any() else None,
            "aggregate_gteps": te / (t_end - t_start) / 1e6, "completed_qps": nq / ((t_end - t_start) / 1e3),
+           # the stream as it arrived, and how long the server took to drain it after the last arrival
+           # (completed_qps counts that drain: a server that keeps up completes nq / (window + drain))
+           "arrived_qps": nq / (arrive[-1] / 1e3) if nq else 0.0, "drain_ms": t_end - (t_start + arrive[-1]),
            "device_bytes": st["device_bytes"], "graphs": names, "checked_queries": len(checked),
            "thin_engines": st["thin_engines"], "dispatcher_busy_frac": st["dispatch_busy_ms"] / (t_end - t_start),
            "dispatcher_passes": st["dispatch_passes"], "rounds": st["rounds"], "server_config": cfg,

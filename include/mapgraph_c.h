@@ -308,18 +308,20 @@ typedef struct mg_server_config {
                                  levels run as thin searches: one workgroup runs a whole
                                  query inside one launch (0 = 256; -1 = never)          */
     int32_t thin_lanes;       /* concurrent thin queries per (graph, kind) (0 = 64)      */
-    int32_t thin_streams;     /* thin launches in flight per (graph, kind), 1..8 (0 = 2) */
+    int32_t thin_streams;     /* thin launches in flight per (graph, kind), 1..8 (0 = 1) */
     int32_t thin_slice_us;    /* a thin search pauses at a level boundary after this many
                                  microseconds in a launch and continues in a later one
-                                 (0 = 50000; -1 = runs to the end in one launch)         */
+                                 (0 = 5000; -1 = runs to the end in one launch)          */
     int32_t ppr_gate_us;      /* personalized PageRank cohorts: an idle engine starts when
                                  ppr_batch_min queries wait or the oldest has waited this
                                  long (0 = 20000; -1 = every query starts on arrival)   */
     int32_t ppr_batch_min;    /* (0 = 8, at most 64)                                     */
     int32_t plain_kinds;      /* bit k set (1 << MG_Q_BFS, 1 << MG_Q_SSSP): those queries on
-                                 graphs that are not thin run one at a time as whole-GPU
-                                 searches (mg_bfs / mg_sssp, the graph's twin included) on
-                                 a worker thread per (graph, kind) instead of in batches */
+                                 graphs that are not thin run as whole-GPU searches
+                                 (mg_bfs / mg_sssp, the graph's twin included), two at a
+                                 time per (graph, kind), each worker on its own view of the
+                                 graph and dedicated stream, instead of in batches
+                                 (default 1 << MG_Q_SSSP; 0 = all batched)               */
 } mg_server_config;
 
 typedef struct mg_query_info {

@@ -217,7 +217,8 @@ struct PprDev {
 // out[s * n + v] = rows[s * n + perm[v]] for the first k rows: a twin's batched rows in
 // the caller's numbering.
 int launch_unpermute_rows(const int *perm, const int *rows, int n, int k, int *out, hipStream_t st);
-int ppr_dev_alloc(PprDev &p, mg_graph *g);
+// Allocates and clears p for graph g; setup copies are ordered on st.
+int ppr_dev_alloc(PprDev &p, mg_graph *g, hipStream_t st);
 void ppr_dev_free(PprDev &p);
 int64_t ppr_dev_bytes(const mg_graph *g);
 // Seeds the listed slots (rank column = e_src) after retiring the `retire` slots.

@@ -131,6 +131,8 @@ struct mg_graph {
     int device = 0;
     hipStream_t stream = nullptr;
     bool stream_borrowed = false;  // a twin runs on its parent's stream
+    bool arrays_borrowed = false;  // a search view: the device arrays belong to another graph
+    bool no_capture = false;       // steps run as direct launches (no captured step graphs)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int32_t n = 0;
     int64_t m = 0;
@@ -263,6 +265,16 @@ inline int not_converged(int64_t max_steps) {
 // Captured step graphs are on unless MG_NO_GRAPH is set to a non-zero value (then
 // every step is launched directly; read once per process).
 bool graphs_enabled();
+// A stream on a hardware queue of its own: streams created with a CU mask (here every
+// CU) are not pooled with the process's other streams, so a long kernel queued on
+// another stream never sits ahead of this one's work.  Such a stream synchronises with
+// the null stream (it is not created non-blocking), so no step graph is captured on it.
+hipError_t dedicated_stream(int device, hipStream_t *st);
+// A search view of g: g's device arrays (not owned; g must outlive the view) with its
+// own dedicated stream, events, lock, workspace and steps (direct launches), and a view
+// of g's twin on the same stream.  Searches on the view give g's results; they run
+// beside searches on g and on other views.
+int graph_view_create(mg_graph *g, mg_graph **out);
 
 // Generic GAS enactor over the ported programs (engine = MG_ENGINE_GAS).
 struct GasStateBase;
@@ -299,7 +311,7 @@ int mg::drive(mg_graph *g, Step step, int64_t max_steps, int kind, uint64_t key,
         ctl = ws.ctl;
         h_ctl = ws.h_ctl;
     }
-    if (kind >= 0 && kind < NKINDS && graphs_enabled()) {
+    if (kind >= 0 && kind < NKINDS && graphs_enabled() && !g->no_capture) {
         StepGraph &sg = g->steps->sg[kind];
         auto capture = [&](int nsteps, hipGraphExec_t *out) -> int {
             MG_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));

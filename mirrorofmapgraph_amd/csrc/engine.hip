@@ -261,6 +261,63 @@ int deliver(mg_graph *g, void *dst, const void *src, size_t bytes, bool out_on_d
     return MG_OK;
 }
 
+hipError_t dedicated_stream(int device, hipStream_t *st) {
+    int cus = 0;
+    hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) return e;
+    std::vector<uint32_t> mask((size_t)std::max(1, (cus + 31) / 32), 0xffffffffu);
+    return hipExtStreamCreateWithCUMask(st, (uint32_t)mask.size(), mask.data());
+}
+
+// The arrays and weight summary of `from` on the view `v` (borrowed).
+static void borrow_arrays(mg_graph *v, const mg_graph *from) {
+    v->arrays_borrowed = true;
+    v->no_capture = true;
+    v->row_off = from->row_off;
+    v->col = from->col;
+    v->w = from->w;
+    v->col_off = from->col_off;
+    v->row_idx = from->row_idx;
+    v->csc_eid = from->csc_eid;
+    v->min_weight = from->min_weight;
+    v->max_weight = from->max_weight;
+    v->mean_weight = from->mean_weight;
+    v->perm = from->perm;
+}
+
+int graph_view_create(mg_graph *g, mg_graph **out) {
+    *out = nullptr;
+    mg_graph *v = nullptr;
+    int rc = graph_shell(g->device, g->n, g->m, g->flags & ~MG_GRAPH_RELABEL, &v);
+    if (rc) return rc;
+    auto fail = [&](int code) {
+        mg_graph_destroy(v);
+        return code;
+    };
+    (void)hipStreamDestroy(v->stream);
+    v->stream = nullptr;
+    if (dedicated_stream(g->device, &v->stream) != hipSuccess) {
+        v->stream = nullptr;
+        return fail(set_error(MG_ERR_DEVICE, "dedicated stream creation failed"));
+    }
+    borrow_arrays(v, g);
+    if (g->twin) {
+        mg_graph *t = nullptr;
+        if ((rc = graph_shell(g->device, g->n, g->m, g->twin->flags, &t))) return fail(rc);
+        v->twin = t;  // released with the view from here on
+        (void)hipStreamDestroy(t->stream);
+        t->stream = v->stream;
+        t->stream_borrowed = true;
+        borrow_arrays(t, g->twin);
+        if ((rc = dalloc(&t->unperm, (size_t)g->n))) return fail(rc);
+        v->h_perm = g->h_perm;
+        if ((rc = ensure_workspace(t))) return fail(rc);
+    }
+    if ((rc = ensure_workspace(v))) return fail(rc);
+    *out = v;
+    return MG_OK;
+}
+
 }  // namespace mg
 
 using namespace mg;
@@ -484,15 +541,17 @@ int mg_graph_destroy(mg_graph *g) {
     }
     if (g->twin) mg_graph_destroy(g->twin);
     g->twin = nullptr;
-    dfree(g->perm);
     dfree(g->unperm);
     free_workspace(g->ws);
-    dfree(g->row_off);
-    dfree(g->col);
-    dfree(g->w);
-    dfree(g->col_off);
-    dfree(g->row_idx);
-    dfree(g->csc_eid);
+    if (!g->arrays_borrowed) {
+        dfree(g->perm);
+        dfree(g->row_off);
+        dfree(g->col);
+        dfree(g->w);
+        dfree(g->col_off);
+        dfree(g->row_idx);
+        dfree(g->csc_eid);
+    }
     if (g->ev0) (void)hipEventDestroy(g->ev0);
     if (g->ev1) (void)hipEventDestroy(g->ev1);
     if (g->stream && !g->stream_borrowed) (void)hipStreamDestroy(g->stream);

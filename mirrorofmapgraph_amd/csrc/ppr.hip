@@ -277,14 +277,14 @@ int64_t ppr_dev_bytes(const mg_graph *g) {
            12LL * (g->m / PPR_T + 64) + 4LL * PPR_SLOTS + 4LL * PPR_SLOTS * n;
 }
 
-int ppr_dev_alloc(PprDev &p, mg_graph *g) {
+int ppr_dev_alloc(PprDev &p, mg_graph *g, hipStream_t st) {
     ppr_dev_free(p);
     const int *off = nullptr, *adj = nullptr;
     if (!ppr_view(g, &off, &adj))
         return set_error(MG_ERR_UNSUPPORTED, "personalized PageRank needs MG_GRAPH_SYMMETRIC or MG_GRAPH_BUILD_CSC");
     const int n = g->n;
     std::vector<int> h_off((size_t)n + 1);
-    MG_HIP(copy_on(g->stream, h_off.data(), off, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
+    MG_HIP(copy_on(st, h_off.data(), off, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
     std::vector<ChunkRec> tasks;
     std::vector<int> big;
     int npart = 0;
@@ -328,15 +328,15 @@ int ppr_dev_alloc(PprDev &p, mg_graph *g) {
     A((void **)&p.big, 4 * big.size());
     A((void **)&p.src, 4 * PPR_SLOTS);
     A((void **)&p.rows, 4 * (size_t)PPR_SLOTS * n);
-    if (e == hipSuccess) e = copy_on(g->stream, p.tasks, tasks.data(), sizeof(ChunkRec) * tasks.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !big.empty()) e = copy_on(g->stream, p.big, big.data(), 4 * big.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = copy_on(st, p.tasks, tasks.data(), sizeof(ChunkRec) * tasks.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !big.empty()) e = copy_on(st, p.big, big.data(), 4 * big.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         std::vector<int> none(PPR_SLOTS, -1);
-        e = copy_on(g->stream, p.src, none.data(), 4 * PPR_SLOTS, hipMemcpyHostToDevice);
+        e = copy_on(st, p.src, none.data(), 4 * PPR_SLOTS, hipMemcpyHostToDevice);
     }
-    if (e == hipSuccess) e = hipMemsetAsync(p.X[0], 0, 4 * (size_t)PPR_SLOTS * n, g->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(p.X[1], 0, 4 * (size_t)PPR_SLOTS * n, g->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(p.X[0], 0, 4 * (size_t)PPR_SLOTS * n, st);
+    if (e == hipSuccess) e = hipMemsetAsync(p.X[1], 0, 4 * (size_t)PPR_SLOTS * n, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         ppr_dev_free(p);
         return set_error(MG_ERR_OOM, "personalized PageRank state allocation failed: %s", hipGetErrorString(e));
@@ -459,7 +459,7 @@ extern "C" int mg_ppr_batch(mg_graph *g, const int32_t *srcs, int32_t k, float *
     if (!p) {
         p = new (std::nothrow) PprDev();
         if (!p) return set_error(MG_ERR_OOM, "host allocation failed");
-        if ((rc = ppr_dev_alloc(*p, sg))) {
+        if ((rc = ppr_dev_alloc(*p, sg, g->stream))) {
             delete p;
             p = nullptr;
             return rc;

@@ -85,10 +85,12 @@ struct Engine {
     double idle_since = 0;
     // plain engines: one whole-GPU search per query (mg_bfs / mg_sssp) on a worker thread
     bool plain = false;
-    std::thread worker;
-    std::mutex wmu;                         // the worker's wake-up (wake / quit under wmu)
+    std::vector<mg_graph *> views;          // a plain engine's search views (owned), one per worker
+    std::vector<std::thread> workers;
+    std::mutex wmu;                         // the workers' wake-up (wake_gen / quit under wmu)
     std::condition_variable wcv;
-    bool wake = false, quit = false;
+    uint64_t wake_gen = 0;
+    bool quit = false;
     // thin engines (BFS / SSSP on a high-diameter graph): one workgroup per query
     bool thin = false;
     ThinDev td;
@@ -213,7 +215,7 @@ static void engine_free(Server &s, Engine &e) {
 // first if needed, and the slot count is halved until it fits.
 static int thin_lanes(const Server &s) { return s.cfg.thin_lanes > 0 ? s.cfg.thin_lanes : 64; }
 // Thin launches in flight per engine.
-static int thin_streams(const Server &s) { return s.cfg.thin_streams > 0 ? s.cfg.thin_streams : 2; }
+static int thin_streams(const Server &s) { return s.cfg.thin_streams > 0 ? s.cfg.thin_streams : 1; }
 
 static int engine_alloc(Server &s, Engine &e) {
     if (e.thin) {
@@ -252,7 +254,7 @@ static int engine_alloc(Server &s, Engine &e) {
     if (S < 1 || s.bytes_used + need(S) > s.cfg.hbm_budget) return MG_ERR_OOM;  // wait for memory
     int rc;
     if (e.kind == MG_Q_PPR) {
-        rc = ppr_dev_alloc(e.p, e.g);
+        rc = ppr_dev_alloc(e.p, e.g, e.st);
     } else {
         rc = batch_dev_alloc(e.b, e.g, e.kind == MG_Q_SSSP, S);
         if (rc == MG_OK && (hipHostGetDevicePointer((void **)&e.h_ctl_dev, e.b.h_ctl, 0) != hipSuccess ||
@@ -564,21 +566,24 @@ static int engine_round(Server &s, Engine &e, bool *queued) {
 }
 
 // A plain engine's worker: takes its engine's queries in arrival order and runs each
-// as an ordinary search on the registered graph (its stream, workspace and twin;
-// the graph's lock serialises it with the graph's other callers), without the server
-// lock held.  A lightly loaded batched engine pays nearly a full round of device work
+// as an ordinary search on its own view of the registered graph (the graph's arrays
+// and twin, the view's dedicated stream, workspace and lock), without the server lock
+// held; PLAIN_WORKERS workers per engine overlap their searches.  A lightly loaded batched engine pays nearly a full round of device work
 // per step whatever its slot count; one whole-GPU search per query costs the search
 // alone (R-MAT scale 22 SSSP: about 1.3 ms).
-static void plain_worker(Server *sp, Engine *ep) {
+constexpr int PLAIN_WORKERS = 2;  // searches of one plain engine in flight together
+
+static void plain_worker(Server *sp, Engine *ep, mg_graph *view) {
     Server &s = *sp;
     Engine &e = *ep;
     (void)hipSetDevice(s.device);
+    uint64_t seen = 0;
     for (;;) {
         {
             std::unique_lock<std::mutex> wl(e.wmu);
-            e.wcv.wait(wl, [&] { return e.wake || e.quit; });
+            e.wcv.wait(wl, [&] { return e.wake_gen != seen || e.quit; });
             if (e.quit) return;
-            e.wake = false;
+            seen = e.wake_gen;
         }
         for (;;) {  // every query queued so far
             std::unique_lock<std::mutex> lk = fair_lock(s);
@@ -594,7 +599,7 @@ static void plain_worker(Server *sp, Engine *ep) {
             mg_stats st;
             int32_t *out = (int32_t *)r->out;
             const int rc =
-                e.kind == MG_Q_BFS ? mg_bfs(e.g, r->src, out, &o, &st) : mg_sssp(e.g, r->src, out, &o, &st);
+                e.kind == MG_Q_BFS ? mg_bfs(view, r->src, out, &o, &st) : mg_sssp(view, r->src, out, &o, &st);
             lk.lock();
             if (rc) {
                 r->status = rc;
@@ -682,6 +687,7 @@ void mg_server_config_default(mg_server_config *c) {
     c->ppr_teleport = 0.15f;
     c->steps_per_round = STEPS_PER_GRAPH;
     c->sssp_delta = 0;
+    c->plain_kinds = 1 << MG_Q_SSSP;
 }
 
 int mg_server_create(const mg_server_config *cfg, mg_server **out) {
@@ -709,7 +715,7 @@ int mg_server_create(const mg_server_config *cfg, mg_server **out) {
             delete s;
             return set_error(MG_ERR_DEVICE, "device wall-clock rate unavailable");
         }
-        const long long us = s->cfg.thin_slice_us > 0 ? s->cfg.thin_slice_us : 50000;
+        const long long us = s->cfg.thin_slice_us > 0 ? s->cfg.thin_slice_us : 5000;
         s->thin_slice = std::max(1LL, us * khz / 1000);
     }
     s->th = std::thread(dispatcher, static_cast<Server *>(s));
@@ -762,26 +768,53 @@ int mg_server_add_graph(mg_server *s, mg_graph *g, int32_t *graph_id) {
             e->h_perm = g->h_perm.data();
             e->d_perm = g->twin->perm;
         }
-        if (e->plain) {  // runs on the graph's stream
-            e->worker = std::thread(plain_worker, static_cast<Server *>(s), e.get());
+        if (e->plain) {
+            // search views of the graph on dedicated streams: their searches never queue
+            // behind another engine's kernels, nor behind the caller's own calls on g.
+            // Views that do not fit the HBM budget leave the engine batched.
+            int64_t total = 0;
+            for (int w = 0; w < PLAIN_WORKERS; ++w) {
+                mg_graph *v = nullptr;
+                int rc = graph_view_create(g, &v);
+                int64_t vb = 0;
+                if (rc == MG_OK) rc = mg_graph_device_bytes(v, &vb);
+                if (rc) {
+                    if (v) mg_graph_destroy(v);
+                    for (mg_graph *x : e->views) mg_graph_destroy(x);
+                    return rc;
+                }
+                total += vb;
+                e->views.push_back(v);
+            }
+            if (s->bytes_used + total <= s->cfg.hbm_budget) {
+                s->bytes_used += total;
+                for (mg_graph *v : e->views) e->workers.emplace_back(plain_worker, static_cast<Server *>(s), e.get(), v);
+                s->engines.push_back(std::move(e));
+                continue;
+            }
+            for (mg_graph *x : e->views) mg_graph_destroy(x);
+            e->views.clear();
+            e->plain = false;
+            e->g = g->twin ? g->twin : g;
+            if (e->g != g) {
+                e->h_perm = g->h_perm.data();
+                e->d_perm = g->twin->perm;
+            }
+        }
+        if (e->thin) {
+            // every launch slot on a dedicated stream: a thin launch runs for up to a
+            // slice, and a pooled stream would put it ahead of other streams' work
+            e->tslots.resize(thin_streams(*s));
+            for (ThinSlot &ts : e->tslots)
+                if (dedicated_stream(s->device, &ts.st) != hipSuccess ||
+                    hipEventCreateWithFlags(&ts.ev, hipEventDisableTiming) != hipSuccess)
+                    return set_error(MG_ERR_DEVICE, "stream/event creation failed");
             s->engines.push_back(std::move(e));
             continue;
         }
         if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&e->ev, hipEventDisableTiming) != hipSuccess)
             return set_error(MG_ERR_DEVICE, "stream/event creation failed");
-        if (e->thin) {
-            // the first launch slot uses the engine's stream: every stream holds a
-            // hardware queue while it exists, and streams beyond the queues share them
-            e->tslots.resize(thin_streams(*s));
-            for (size_t i = 0; i < e->tslots.size(); ++i) {
-                ThinSlot &ts = e->tslots[i];
-                if (i == 0) ts.st = e->st;
-                if ((i > 0 && hipStreamCreateWithFlags(&ts.st, hipStreamNonBlocking) != hipSuccess) ||
-                    hipEventCreateWithFlags(&ts.ev, hipEventDisableTiming) != hipSuccess)
-                    return set_error(MG_ERR_DEVICE, "stream/event creation failed");
-            }
-        }
         s->engines.push_back(std::move(e));
     }
     *graph_id = id;
@@ -819,9 +852,9 @@ int mg_server_submit(mg_server *s, int32_t graph_id, int32_t kind, int32_t src, 
     if (e.plain) {
         {
             std::lock_guard<std::mutex> g(e.wmu);
-            e.wake = true;
+            e.wake_gen++;
         }
-        e.wcv.notify_one();
+        e.wcv.notify_all();
     } else {
         s->cv_work.notify_one();
     }
@@ -905,11 +938,16 @@ int mg_server_destroy(mg_server *s) {
                 std::lock_guard<std::mutex> g(e->wmu);
                 e->quit = true;
             }
-            e->wcv.notify_one();
+            e->wcv.notify_all();
         }
     if (s->th.joinable()) s->th.join();
     for (auto &e : s->engines)
-        if (e && e->worker.joinable()) e->worker.join();
+        if (e)
+            for (std::thread &t : e->workers)
+                if (t.joinable()) t.join();
+    for (auto &e : s->engines)
+        if (e)
+            for (mg_graph *v : e->views) mg_graph_destroy(v);
     (void)hipSetDevice(s->device);
     for (auto &e : s->engines) {
         if (!e) continue;
@@ -921,7 +959,7 @@ int mg_server_destroy(mg_server *s) {
         if (e->st) (void)hipStreamDestroy(e->st);
         for (ThinSlot &ts : e->tslots) {
             if (ts.ev) (void)hipEventDestroy(ts.ev);
-            if (ts.st && ts.st != e->st) (void)hipStreamDestroy(ts.st);
+            if (ts.st) (void)hipStreamDestroy(ts.st);
         }
     }
     delete s;

@@ -47,10 +47,13 @@ def _queries(mg, hosts, count, seed):
     return qs
 
 
-def test_serving_mixed_equals_alone(mg, device, graphs):
+@pytest.mark.parametrize("plain_kinds", [None, 0])
+def test_serving_mixed_equals_alone(mg, device, graphs, plain_kinds):
+    # default settings (SSSP as plain whole-GPU searches) and every kind batched
     hosts, dgs = graphs
     qs = _queries(mg, hosts, 150, seed=1)
-    with mg.Server(max_slots=16, steps_per_round=2) as srv:
+    extra = {} if plain_kinds is None else {"plain_kinds": plain_kinds}
+    with mg.Server(max_slots=16, steps_per_round=2, **extra) as srv:
         ids = [srv.add_graph(d) for d in dgs]
         outs, qids = [], []
         for gi, kind, src in qs:
@@ -340,8 +343,17 @@ def test_serving_plain_engines(mg, device, graphs):
 
     hosts, dgs = graphs
     qs = _queries(mg, hosts, 90, seed=61)
+    direct = {}
+
+    def direct_calls():  # the caller's own searches on the same graphs, beside the server's
+        for gi, d in enumerate(dgs):
+            s = int(mg.pick_sources(hosts[gi], 1, seed=70 + gi)[0])
+            direct[gi] = (s, d.sssp(s), d.bfs(s))
+
     with mg.Server(plain_kinds=(1 << mg.MG_Q_BFS) | (1 << mg.MG_Q_SSSP)) as srv:
         ids = [srv.add_graph(d) for d in dgs]
+        th = threading.Thread(target=direct_calls)
+        th.start()
         outs, qids = [], []
         for i, (gi, kind, src) in enumerate(qs):
             dt = np.float32 if kind == mg.MG_Q_PPR else np.int32
@@ -356,7 +368,10 @@ def test_serving_plain_engines(mg, device, graphs):
                 qids.append(srv.submit(ids[gi], kind, src, out))
             outs.append(out)
         infos = [srv.wait(q, 60000) for q in qids]
+        th.join()
         assert srv.stats()["completed"] == len(qs)
+    for gi, (s, dist, depth) in direct.items():
+        assert np.array_equal(dist, dgs[gi].sssp(s)) and np.array_equal(depth, dgs[gi].bfs(s))
     for (gi, kind, src), out, info in zip(qs, outs, infos):
         got = out if isinstance(out, np.ndarray) else out.array
         exp = _alone(mg, dgs[gi], kind, src)
