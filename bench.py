@@ -200,6 +200,18 @@ def grid_part(mg, oracle, seed: int, reps: int, log) -> dict:
     return out
 
 
+def calibration_copy(nbytes: int) -> None:
+    """A device-to-device copy of nbytes (the runtime's copy kernel): the known traffic that scales the
+    FETCH_SIZE / WRITE_SIZE counters of the same run."""
+    hip = C.CDLL("libamdhip64.so")  # the runtime the engine already loaded
+    p, q = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(q), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(q, p, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(p)
+    hip.hipFree(q)
+
+
 def pagerank_part(mg, oracle, seed: int, reps: int, log, relabel: bool = True) -> dict:
     """PageRank on the directed scale-22 R-MAT (dangling vertices kept), all-vertex Jacobi, damping 0.85,
     FP32, 50 iterations (the L1 stop at 1e-6 never triggers in FP32 at this size).  Ranks are compared with
@@ -469,6 +481,9 @@ def main() -> int:
     ap.add_argument("--pmc-pass", action="store_true",
                     help="for counter runs: one warm-up and one measured SSSP, CC and BFS search, then a "
                          "256 MiB device-to-device copy (the calibration kernel); prints per-search stats")
+    ap.add_argument("--pmc-pagerank", action="store_true",
+                    help="for counter runs: one warm-up and one measured 50-iteration PageRank on the directed "
+                         "scale-22 R-MAT (with its twin), then the calibration copy")
     a = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -481,6 +496,16 @@ def main() -> int:
 
     mg.check(mg.lib().mg_device_init(0))
     log = (lambda *x: print(*x, file=sys.stderr, flush=True)) if a.verbose else (lambda *x: None)
+
+    if a.pmc_pagerank:
+        host = mg.rmat_graph(22, 16, seed=a.seed, symmetric=False)
+        with mg.Graph(host, build_csc=True, relabel=not a.no_relabel) as pg:
+            o = mg.options(pr_mode=mg.MG_PR_ALL, iter_num=50, pr_tol=1e-6)
+            pg.pagerank(o, copy=False)
+            _, st = pg.pagerank(o, stats=True, copy=False)
+            calibration_copy(256 << 20)
+        print(json.dumps({"pagerank": st.as_dict(), "calibration_copy_bytes": 256 << 20}))
+        return 0
 
     t0 = time.perf_counter()
     g = mg.rmat_graph(a.scale, 16, seed=a.seed, weight_max=64)
@@ -500,14 +525,8 @@ def main() -> int:
             fn()
             _, st = fn()
             out[name] = st.as_dict()
-        hip = C.CDLL("libamdhip64.so")  # the runtime the engine already loaded
         nbytes = 256 << 20
-        p, q = C.c_void_p(), C.c_void_p()
-        assert hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(q), C.c_size_t(nbytes)) == 0
-        assert hip.hipMemcpy(q, p, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
-        assert hip.hipDeviceSynchronize() == 0
-        hip.hipFree(p)
-        hip.hipFree(q)
+        calibration_copy(nbytes)
         out["calibration_copy_bytes"] = nbytes
         print(json.dumps(out))
         dg.close()
