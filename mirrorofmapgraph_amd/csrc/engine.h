@@ -102,6 +102,7 @@ struct Workspace {
     LevelPart *part = nullptr;  // 3 * BIG_GRID slots
     Ctl *ctl = nullptr;
     Ctl *h_ctl = nullptr;     // pinned host mirror
+    unsigned long long *h_reach = nullptr;  // pinned: TEPS count and reached vertices of a result
     // PageRank state (allocated on first use)
     float *rank = nullptr;
     float *contrib[2] = {nullptr, nullptr};

@@ -75,6 +75,7 @@ static void free_workspace(Workspace &ws) {
     dfree(ws.part);
     dfree(ws.ctl);
     if (ws.h_ctl) (void)hipHostFree(ws.h_ctl);
+    if (ws.h_reach) (void)hipHostFree(ws.h_reach);
     dfree(ws.rank);
     dfree(ws.contrib[0]);
     dfree(ws.contrib[1]);
@@ -113,6 +114,7 @@ int ensure_workspace(mg_graph *g) {
         return rc;
     }
     hipError_t e = hipHostMalloc((void **)&ws.h_ctl, sizeof(Ctl), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&ws.h_reach, 2 * sizeof(unsigned long long), hipHostMallocDefault);
     if (e != hipSuccess) {
         free_workspace(ws);
         return set_error(MG_ERR_OOM, "hipHostMalloc failed: %s", hipGetErrorString(e));

@@ -623,7 +623,6 @@ FrontierArgs make_frontier_args(mg_graph *g, int directed);
 int launch_unpermute(const int *perm, const void *vals, int n, void *out, hipStream_t st);
 int launch_fill(mg_graph *g, int *p, int v, int64_t n);
 int launch_fill(mg_graph *g, uint32_t *p, uint32_t v, int64_t n);
-int launch_reached_stats(mg_graph *g, const int *vals, int unreached, mg_stats *st);
 int ensure_pull_tasks(mg_graph *g, const int *d_in_off);
 // Tasks of the dense SSSP pull levels: {v, beg, end} = slots [beg, end) of the
 // in-list of a vertex v with more than DENSE_BIG in-edges (DENSE_T per piece), or

@@ -164,18 +164,15 @@ __global__ __launch_bounds__(1024) void reached_stats(const int *vals, int unrea
     }
 }
 
-int launch_reached_stats(mg_graph *g, const int *vals, int unreached, mg_stats *st) {
-    if (!st) return MG_OK;
+// Queues the TEPS count over `vals` on the graph's stream; the two counts land in the
+// pinned ws.h_reach once the stream has drained.
+static int queue_reached_stats(mg_graph *g, const int *vals, int unreached) {
     unsigned long long *d = (unsigned long long *)g->ws.chunks;  // 16 bytes of scratch
     MG_HIP(hipMemsetAsync(d, 0, 16, g->stream));
     const int blocks = std::min(256, std::max(1, (g->n + 1023) / 1024));
     hipLaunchKernelGGL(reached_stats, dim3(blocks), dim3(1024), 0, g->stream, vals, unreached, g->row_off, g->n, d);
     MG_HIP(hipGetLastError());
-    unsigned long long h[2];
-    MG_HIP(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, g->stream));
-    MG_HIP(hipStreamSynchronize(g->stream));
-    st->edges_traversed = (int64_t)h[0];
-    st->vertices_reached = (int64_t)h[1];
+    MG_HIP(hipMemcpyAsync(g->ws.h_reach, d, 16, hipMemcpyDeviceToHost, g->stream));
     return MG_OK;
 }
 
@@ -221,11 +218,17 @@ int finish_search(mg_graph *g, mg_stats *stats, const int *vals, int unreached, 
         MG_HIP(hipGetLastError());
         result = g->unperm;
     }
+    // the closing event, then the control block and the TEPS count behind it: one wait
     MG_HIP(hipEventRecord(g->ev1, g->stream));
-    MG_HIP(hipEventSynchronize(g->ev1));
+    MG_HIP(hipMemcpyAsync(ws.h_ctl, ws.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
+    const bool reach = stats && vals;
+    if (reach) {
+        const int rc = queue_reached_stats(g, vals, unreached);
+        if (rc) return rc;
+    }
+    MG_HIP(hipStreamSynchronize(g->stream));
     float ms = 0;
     MG_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
-    MG_HIP(copy_on(g->stream, ws.h_ctl, ws.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
     if (ws.h_ctl->changed & (1 << 30))
         return set_error(MG_ERR_OVERFLOW, "edge-chunk queue overflow (capacity %lld)", (long long)ws.chunk_cap);
     if (stats) {
@@ -234,9 +237,9 @@ int finish_search(mg_graph *g, mg_stats *stats, const int *vals, int unreached, 
         stats->bottom_up_levels = ws.h_ctl->bu_levels;
         stats->edges_examined = ws.h_ctl->edges_examined;
         stats->device_ms = ms;
-        if (vals) {
-            int rc = launch_reached_stats(g, vals, unreached, stats);
-            if (rc) return rc;
+        if (reach) {
+            stats->edges_traversed = (int64_t)ws.h_reach[0];
+            stats->vertices_reached = (int64_t)ws.h_reach[1];
         }
     }
     int rc = deliver(g, out, result, bytes, out_on_device);
