@@ -605,8 +605,13 @@ static int bfs_run(mg_graph *g, int src, int *depth_out, const mg_options &o, mg
         return MG_OK;
     };
     const int64_t max_steps = std::min<int64_t>((int64_t)o.iter_num, (int64_t)g->n) + 3;
-    if ((rc = drive(g, step, max_steps, KIND_BFS, mix64((uint64_t)o.directed + 2 * (uint64_t)a.G)))) return rc;
-    rc = finish_search(g, stats, ws.vals, -1, depth_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
+    bool tail_queued = false;
+    auto tail = [&]() { return queue_finish(g, stats != nullptr, ws.vals, -1, ws.vals, 4 * (size_t)g->n); };
+    if ((rc = drive(g, step, max_steps, KIND_BFS, mix64((uint64_t)o.directed + 2 * (uint64_t)a.G), nullptr, nullptr,
+                    tail, &tail_queued)))
+        return rc;
+    rc = finish_search(g, stats, ws.vals, -1, depth_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer,
+                       tail_queued);
     if (rc == MG_OK && o.verbose) rc = print_trace(g, "bfs", "top-down", "bottom-up");
     return rc;
 }

@@ -103,6 +103,7 @@ struct Workspace {
     Ctl *ctl = nullptr;
     Ctl *h_ctl = nullptr;     // pinned host mirror
     unsigned long long *h_reach = nullptr;  // pinned: TEPS count and reached vertices of a result
+    unsigned long long *d_reach = nullptr;  // their device accumulators
     // PageRank state (allocated on first use)
     float *rank = nullptr;
     float *contrib[2] = {nullptr, nullptr};
@@ -257,9 +258,15 @@ struct StepGraph {
 // Returns MG_OK once the device sets ctl->done; a search still running after
 // max_steps steps is an error (every iteration limit sets done on the device, so
 // this never ends a normal search early).
-template <class Step>
+struct NoTail {
+    int operator()() const { return 0; }
+};
+// tail (optional) queues the work that follows a finished search.  When the step count
+// is known from the previous search it is queued behind those steps, before the one
+// wait; *tail_queued then says whether the search finished there with the tail run.
+template <class Step, class Tail = NoTail>
 int drive(mg_graph *g, Step step, int64_t max_steps, int kind = -1, uint64_t key = 0, Ctl *ctl = nullptr,
-          Ctl *h_ctl = nullptr);
+          Ctl *h_ctl = nullptr, Tail tail = Tail(), bool *tail_queued = nullptr);
 inline int not_converged(int64_t max_steps) {
     return set_error(MG_ERR_DEVICE, "the search did not finish within %lld steps", (long long)max_steps);
 }
@@ -305,9 +312,11 @@ struct mg_graph_steps {
 // Queues steps until the device reports `done`.  Steps run from a captured HIP graph
 // (kind >= 0) or as direct launches; the host reads the control block only between
 // batches of steps (1, 2, 4, ... graphs), never once per level.
-template <class Step>
-int mg::drive(mg_graph *g, Step step, int64_t max_steps, int kind, uint64_t key, Ctl *ctl, Ctl *h_ctl) {
+template <class Step, class Tail>
+int mg::drive(mg_graph *g, Step step, int64_t max_steps, int kind, uint64_t key, Ctl *ctl, Ctl *h_ctl, Tail tail,
+              bool *tail_queued) {
     Workspace &ws = g->ws;
+    if (tail_queued) *tail_queued = false;
     if (!ctl) {
         ctl = ws.ctl;
         h_ctl = ws.h_ctl;
@@ -365,10 +374,17 @@ int mg::drive(mg_graph *g, Step step, int64_t max_steps, int kind, uint64_t key,
                 MG_HIP(hipGraphLaunch(sg.part[left], g->stream));
                 queued += left;
             }
+            // the search most likely ends here: queue its tail before the wait (if not,
+            // the caller queues it again after the remaining steps)
+            if (tail_queued) {
+                const int trc = tail();
+                if (trc) return trc;
+            }
             MG_HIP(hipMemcpyAsync(h_ctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
             MG_HIP(hipStreamSynchronize(g->stream));
             if (h_ctl->done) {
                 finish();
+                if (tail_queued) *tail_queued = true;
                 return MG_OK;
             }
         }

@@ -76,6 +76,7 @@ static void free_workspace(Workspace &ws) {
     dfree(ws.ctl);
     if (ws.h_ctl) (void)hipHostFree(ws.h_ctl);
     if (ws.h_reach) (void)hipHostFree(ws.h_reach);
+    dfree(ws.d_reach);
     dfree(ws.rank);
     dfree(ws.contrib[0]);
     dfree(ws.contrib[1]);
@@ -108,7 +109,7 @@ int ensure_workspace(mg_graph *g) {
         (rc = dalloc(&ws.q[2], n)) || (rc = dalloc(&ws.hard, n)) || (rc = dalloc(&ws.visited, ws.nwords)) ||
         (rc = dalloc(&ws.bm[0], ws.nwords)) || (rc = dalloc(&ws.bm[1], ws.nwords)) ||
         (rc = dalloc(&ws.bm[2], ws.nwords)) || (rc = dalloc(&ws.chunks, ws.chunk_cap)) ||
-        (rc = dalloc(&ws.part, 3 * BIG_GRID)) || (rc = dalloc(&ws.ctl, 1)) ||
+        (rc = dalloc(&ws.part, 3 * BIG_GRID)) || (rc = dalloc(&ws.ctl, 1)) || (rc = dalloc(&ws.d_reach, 2)) ||
         (rc = dalloc(&ws.trace, 2 * (size_t)TRACE_CAP))) {
         free_workspace(ws);
         return rc;

@@ -632,7 +632,10 @@ int ensure_dense_tasks(mg_graph *g, const int *d_in_off);
 // Prints the per-level trace of the search that just finished to stderr (the
 // reference's verbose per-iteration frontier sizes).
 int print_trace(mg_graph *g, const char *what, const char *dir0, const char *dir1);
+// Queues what follows a finished search: the result back to the caller's ids (a twin's
+// search), the closing event and, with_stats, the TEPS count over vals.
+int queue_finish(mg_graph *g, bool with_stats, const int *vals, int unreached, const void *result, size_t bytes);
 int finish_search(mg_graph *g, mg_stats *stats, const int *vals, int unreached, void *out, const void *result,
-                  size_t bytes, bool out_on_device, const CallTimer &timer);
+                  size_t bytes, bool out_on_device, const CallTimer &timer, bool tail_queued = false);
 
 }  // namespace mg

@@ -167,7 +167,7 @@ __global__ __launch_bounds__(1024) void reached_stats(const int *vals, int unrea
 // Queues the TEPS count over `vals` on the graph's stream; the two counts land in the
 // pinned ws.h_reach once the stream has drained.
 static int queue_reached_stats(mg_graph *g, const int *vals, int unreached) {
-    unsigned long long *d = (unsigned long long *)g->ws.chunks;  // 16 bytes of scratch
+    unsigned long long *d = g->ws.d_reach;
     MG_HIP(hipMemsetAsync(d, 0, 16, g->stream));
     const int blocks = std::min(256, std::max(1, (g->n + 1023) / 1024));
     hipLaunchKernelGGL(reached_stats, dim3(blocks), dim3(1024), 0, g->stream, vals, unreached, g->row_off, g->n, d);
@@ -208,25 +208,30 @@ int launch_unpermute(const int *perm, const void *vals, int n, void *out, hipStr
     return MG_OK;
 }
 
-int finish_search(mg_graph *g, mg_stats *stats, const int *vals, int unreached, void *out, const void *result,
-                  size_t bytes, bool out_on_device, const CallTimer &timer) {
-    Workspace &ws = g->ws;
+int queue_finish(mg_graph *g, bool with_stats, const int *vals, int unreached, const void *result, size_t bytes) {
     if (g->perm && result && bytes == 4 * (size_t)g->n) {
         // a search on a twin: its per-vertex result (int32 or float) goes back to the caller's ids
         hipLaunchKernelGGL(unpermute_vals, dim3(grid_blocks_cap(g->n, BLK, 4 * MAX_GRID)), dim3(BLK), 0, g->stream,
                            g->perm, (const int *)result, g->n, g->unperm);
         MG_HIP(hipGetLastError());
-        result = g->unperm;
     }
-    // the closing event, then the control block and the TEPS count behind it: one wait
-    MG_HIP(hipEventRecord(g->ev1, g->stream));
-    MG_HIP(hipMemcpyAsync(ws.h_ctl, ws.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
+    MG_HIP(hipEventRecord(g->ev1, g->stream));  // the end of the timed search
+    if (with_stats && vals) return queue_reached_stats(g, vals, unreached);
+    return MG_OK;
+}
+
+int finish_search(mg_graph *g, mg_stats *stats, const int *vals, int unreached, void *out, const void *result,
+                  size_t bytes, bool out_on_device, const CallTimer &timer, bool tail_queued) {
+    Workspace &ws = g->ws;
     const bool reach = stats && vals;
-    if (reach) {
-        const int rc = queue_reached_stats(g, vals, unreached);
+    if (!tail_queued) {
+        // the closing event, then the control block and the TEPS count behind it: one wait
+        int rc = queue_finish(g, stats != nullptr, vals, unreached, result, bytes);
         if (rc) return rc;
-    }
-    MG_HIP(hipStreamSynchronize(g->stream));
+        MG_HIP(hipMemcpyAsync(ws.h_ctl, ws.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
+        MG_HIP(hipStreamSynchronize(g->stream));
+    }  // else drive queued the same work and waited for it
+    if (g->perm && result && bytes == 4 * (size_t)g->n) result = g->unperm;
     float ms = 0;
     MG_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
     if (ws.h_ctl->changed & (1 << 30))

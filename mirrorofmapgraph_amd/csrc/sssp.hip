@@ -1121,9 +1121,12 @@ static int sssp_run(mg_graph *g, int src, int *dist_out, const mg_options &o, mg
     key = mix64(key ^ (uint64_t)a.delta);
     key = mix64(key ^ (uint64_t)a.bucket_edges);
     key = mix64(key ^ (uint64_t)a.G);
-    if ((rc = drive(g, step, max_steps, KIND_SSSP, key)))
+    bool tail_queued = false;
+    auto tail = [&]() { return queue_finish(g, stats != nullptr, ws.vals, INT_MAX, ws.vals, 4 * (size_t)g->n); };
+    if ((rc = drive(g, step, max_steps, KIND_SSSP, key, nullptr, nullptr, tail, &tail_queued)))
         return rc;
-    rc = finish_search(g, stats, ws.vals, INT_MAX, dist_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer);
+    rc = finish_search(g, stats, ws.vals, INT_MAX, dist_out, ws.vals, 4 * (size_t)g->n, o.out_on_device, timer,
+                       tail_queued);
     if (rc == MG_OK && o.verbose) rc = print_trace(g, "sssp", "push", "pull");
     return rc;
 }
