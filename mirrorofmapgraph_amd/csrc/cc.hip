@@ -80,12 +80,31 @@ __global__ void cc_init(FrontierArgs fa, int *p) {
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl_reset(fa.ctl, fa.n, 1, fa.n);
 }
 
-// Afforest step 1: unite v with its r-th neighbour.
-__global__ void cc_link_round(FrontierArgs fa, int *p, int r) {
+// Afforest step 1, first neighbour, with the parents' initialisation: parent[v] =
+// min(v, first neighbour of v).  Only the thread of v writes parent[v] here, and every
+// parent points to a smaller id or to itself, so the pointers form a forest whose
+// roots are minima without any compare-and-swap.  A first neighbour larger than v
+// is united in the second pass.
+__global__ void cc_link_first(FrontierArgs fa, int *p) {
     const int *off = fa.off[0], *adj = fa.adj[0];
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < fa.n; v += gridDim.x * blockDim.x) {
-        const int e = off[v] + r;
-        if (e < off[v + 1]) uf_unite(p, v, adj[e]);
+        const int e = off[v];
+        p[v] = e < off[v + 1] ? min(v, adj[e]) : v;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl_reset(fa.ctl, fa.n, 1, fa.n);
+}
+
+// Afforest step 1, second pass: the first neighbour where it is larger than v, and
+// the second neighbour, by union-find.
+__global__ void cc_link_second(FrontierArgs fa, int *p) {
+    const int *off = fa.off[0], *adj = fa.adj[0];
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < fa.n; v += gridDim.x * blockDim.x) {
+        const int e = off[v], end = off[v + 1];
+        if (e < end) {
+            const int u0 = adj[e];
+            if (u0 > v) uf_unite(p, v, u0);
+            if (e + 1 < end) uf_unite(p, v, adj[e + 1]);
+        }
     }
 }
 
@@ -206,16 +225,16 @@ extern "C" int mg_cc(mg_graph *g, int32_t *label_out, const mg_options *opt, mg_
     const int G = grid_blocks_cap(std::max<int64_t>(g->n / 4, g->m / 64), BLK, BIG_GRID);
     const int Gn = grid_blocks_cap(g->n, BLK, BIG_GRID);
     MG_HIP(hipEventRecord(g->ev0, g->stream));
-    hipLaunchKernelGGL(cc_init, dim3(Gn), dim3(BLK), 0, g->stream, fa, p);
     if ((g->flags & MG_GRAPH_SYMMETRIC) && o.engine != MG_ENGINE_GAS && g->n > 0) {
-        for (int r = 0; r < CC_SAMPLE_ROUNDS; ++r) {
-            hipLaunchKernelGGL(cc_link_round, dim3(Gn), dim3(BLK), 0, g->stream, fa, p, r);
-            hipLaunchKernelGGL(cc_compress, dim3(Gn), dim3(BLK), 0, g->stream, fa, p);
-        }
+        hipLaunchKernelGGL(cc_link_first, dim3(Gn), dim3(BLK), 0, g->stream, fa, p);
+        hipLaunchKernelGGL(cc_compress, dim3(Gn), dim3(BLK), 0, g->stream, fa, p);
+        hipLaunchKernelGGL(cc_link_second, dim3(Gn), dim3(BLK), 0, g->stream, fa, p);
+        hipLaunchKernelGGL(cc_compress, dim3(Gn), dim3(BLK), 0, g->stream, fa, p);
         hipLaunchKernelGGL(cc_sample, dim3(1), dim3(CC_SAMPLES), 0, g->stream, fa, (const int *)p,
                            (uint64_t)0x5eed);
         hipLaunchKernelGGL(cc_finish, dim3(G), dim3(BLK), 0, g->stream, fa, p);
     } else {
+        hipLaunchKernelGGL(cc_init, dim3(Gn), dim3(BLK), 0, g->stream, fa, p);
         hipLaunchKernelGGL(cc_hook, dim3(G), dim3(BLK), 0, g->stream, fa, p);
     }
     hipLaunchKernelGGL(cc_hook_chunks, dim3(G), dim3(BLK), 0, g->stream, fa, p);
