@@ -398,6 +398,10 @@ def serving_part(mg, seed: int, seconds: float, rate: float, log, mix=(0.6, 0.3,
     te = sum(i["edges_traversed"] for i in infos)
     # steady state: queries arriving after the first second and before the arrival window ends
     steady = (arrive > 1000.0) & (arrive <= seconds * 1e3)
+    done_rel = np.array([i["t_done_ms"] - t_start for i in infos])
+    span = (arrive[-1] - 1000.0) / 1e3 if nq else 0.0
+    steady_rate = float(((done_rel > 1000.0) & (done_rel <= arrive[-1])).sum() / span) if span > 0 else None
+    steady_in = float(((arrive > 1000.0) & (arrive <= arrive[-1])).sum() / span) if span > 0 else None
     res = {"queries": nq, "rate_qps": rate, "arrival_window_s": seconds, "mix_bfs_sssp_ppr": list(mix), "p50_ms": float(np.percentile(lat, 50)),
            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
            "steady_p50_ms": float(np.percentile(lat[steady], 50)) if steady.any() else None,
@@ -406,6 +410,11 @@ def serving_part(mg, seed: int, seconds: float, rate: float, log, mix=(0.6, 0.3,
            # the stream as it arrived, and how long the server took to drain it after the last arrival
            # (completed_qps counts that drain: a server that keeps up completes nq / (window + drain))
            "arrived_qps": nq / (arrive[-1] / 1e3) if nq else 0.0, "drain_ms": t_end - (t_start + arrive[-1]),
+           # completions per second while the stream is still arriving (from its first second to its last
+           # arrival): the service rate under the sustained load, equal to the arrival rate when the server keeps
+           # up and below it when a backlog grows
+           "steady_completed_qps": steady_rate,
+           "steady_arrived_qps": steady_in,
            "device_bytes": st["device_bytes"], "graphs": names, "checked_queries": len(checked),
            "thin_engines": st["thin_engines"], "dispatcher_busy_frac": st["dispatch_busy_ms"] / (t_end - t_start),
            "dispatcher_passes": st["dispatch_passes"], "rounds": st["rounds"], "server_config": cfg,
@@ -422,7 +431,8 @@ def serving_part(mg, seed: int, seconds: float, rate: float, log, mix=(0.6, 0.3,
                                      "service_p50_ms": float(np.percentile(svc[sel], 50))}
     res["per_kind_graph"] = per
     log(f"serving: {nq} queries, p50 {res['p50_ms']:.2f} ms, p99 {res['p99_ms']:.2f} ms, "
-        f"{res['aggregate_gteps']:.1f} GTEPS, {res['completed_qps']:.0f} q/s")
+        f"{res['aggregate_gteps']:.1f} GTEPS, {res['completed_qps']:.0f} q/s "
+        f"(steady: {res['steady_completed_qps'] or 0:.0f} completed, {res['steady_arrived_qps'] or 0:.0f} arrived)")
     return res
 
 
