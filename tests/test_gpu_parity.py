@@ -82,6 +82,44 @@ def test_cc_random(mg, oracle, device, symmetric):
         assert np.array_equal(dg.cc(), oracle.cc(g.row_offsets, g.col_idx))
 
 
+def _structured_cc_graph(mg, dense):
+    """Symmetric graphs whose first neighbours are often larger than the vertex itself.  dense: three
+    bipartite blocks (every vertex of a block's lower half joined to 12 of its upper half, so each lower
+    vertex's neighbours are all larger) and isolated vertices, mean degree >= 8; sparse: a star centred on
+    the last id, two paths, a descending chain and isolated vertices."""
+    rng = np.random.default_rng(5)
+    src, dst = [], []
+    if dense:
+        n, base = 3 * 1200 + 50, 0
+        for _ in range(3):
+            lo = np.arange(base, base + 600)
+            for _ in range(12):
+                src.append(lo)
+                dst.append(rng.integers(base + 600, base + 1200, lo.size))
+            base += 1200
+    else:
+        n = 5000
+        src.append(np.arange(0, 1000))
+        dst.append(np.full(1000, n - 1))
+        src.append(np.arange(1000, 1999))
+        dst.append(np.arange(1001, 2000))
+        src.append(np.arange(3999, 3000, -1))
+        dst.append(np.arange(3998, 2999, -1))
+        src.append(np.arange(2000, 2500))
+        dst.append(np.arange(2001, 2501)[::-1])
+    s_, d_ = np.concatenate(src).astype(np.int32), np.concatenate(dst).astype(np.int32)
+    flags = mg.MG_CSR_DEDUPE | mg.MG_CSR_SORT_COLS | mg.MG_CSR_DROP_LOOPS | mg.MG_CSR_SYMMETRIZE
+    return mg.csr_from_edges(n, s_, d_, flags)
+
+
+@pytest.mark.parametrize("dense", [True, False])
+def test_cc_structured(mg, oracle, device, dense):
+    g = _structured_cc_graph(mg, dense)
+    assert (g.m >= 8 * g.n) == dense  # the plain-store first pass, or union-find rounds
+    with mg.Graph(g, symmetric=True) as dg:
+        assert np.array_equal(dg.cc(), oracle.cc(g.row_offsets, g.col_idx))
+
+
 # ---------------------------------------------------------------- R-MAT / grid
 @pytest.fixture(scope="module")
 def s16(mg):
