@@ -29,7 +29,7 @@
 
 namespace mg {
 
-constexpr int PR_T1 = 32;
+constexpr int PR_T1 = 64;
 constexpr int PR_T2 = 1024;
 constexpr int PR_MAX_GRID = 32768;  // workgroups of pr_k1 / pr_k2
 
